@@ -1,0 +1,136 @@
+/*
+ * titan_hip.h -- C ABI of the MI355X (gfx950) implementation of TITAN's
+ * chi(omega) path: H(k) assembly, batched G(k,z) = (z - H(k))^-1 on the
+ * complex energy contour, the k- and energy-summed bare susceptibility
+ * chi0(q,omega), and the RPA Dyson step chi = (I + chi0 U)^-1 chi0.
+ *
+ * Conventions (identical to TITAN's Fortran arrays, so a bind(C) interface
+ * passes them without copies):
+ *   - complex numbers are interleaved (re, im) doubles = complex(c_double_complex);
+ *   - matrices are column-major: A(i,j) at A[i + j*lda];
+ *   - Hamiltonian index (atom i, spin s, orbital mu) -> (2*i + s)*nOrb + mu
+ *     (atom-major, spin up block then spin down block; reference 'ia' map,
+ *     source/mod_System.F90:101-155);
+ *   - chi index (sigma, i, mu, nu) -> 4*((i*nOrb + mu)*nOrb + nu) + sigma with
+ *     sigma = 0..3 for (+, up, down, -) (reference 'sigmaimunu2i',
+ *     source/mod_System.F90:158-223);
+ *   - atoms and hop endpoints are 0-based.
+ *
+ * Status codes: 0 = success; > 0 = a pivot of (z - H) or (I + chi0 U) is
+ * exactly zero (LAPACK 'info' semantics, as invers, source/mod_tools.F90:
+ * 796-829); < 0 = error (see TH_E_* below).  The Fortran shim maps any
+ * nonzero status to abortProgram (source/mod_mpi_pars.F90:59-72).
+ *
+ * Threading: one handle per process and device; calls on one handle must not
+ * overlap.  The handle owns every device buffer; the caller owns all host
+ * arrays, and no host pointer is retained after a call returns.
+ */
+#ifndef TITAN_HIP_H
+#define TITAN_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct th_handle th_handle;
+
+#define TH_OK            0
+#define TH_E_ARG        -1   /* illegal argument (size, null pointer, range) */
+#define TH_E_NODEVICE   -2   /* no HIP device / library built without gfx950 code */
+#define TH_E_ALLOC      -3   /* device allocation failed */
+#define TH_E_HIP        -4   /* HIP runtime error (message via th_last_error) */
+#define TH_E_STATE      -5   /* call order violated (e.g. chi0 before the k-mesh is set) */
+#define TH_E_UNSUPPORTED -6  /* size outside what the kernels support (see th_limits) */
+
+/* Precision modes of th_chi0 / th_chi_sweep. */
+#define TH_PREC_FP64   0     /* G, chi0 contraction and sums in FP64 */
+#define TH_PREC_MIXED  1     /* G in FP32 + one FP64-residual refinement step, FP32 MFMA
+                                contraction with FP32 in-tile accumulation, FP64 contour
+                                sums and FP64 Dyson */
+
+/* Creates a handle on the current HIP device and uploads the tight-binding
+ * model: the local Hamiltonian h0 (d x d complex, d = 2*nOrb*nA) built by
+ * hamilt_local (source/mod_hamiltonian.F90:35-87) and the hop table that
+ * calchk sums (source/mod_hamiltonian.F90:183-243):
+ *   H(k)[block jb[n], block ib[n]] += exp(i k.R_n) t_n   (both spins)
+ * with R = 3*nhop cell vectors, t = nhop blocks of nOrb x nOrb complex
+ * (column-major, rows = orbitals of atom jb[n]; TITAN's Neighbors(n)%t0i).
+ * Replaces the set-up half of eintshechi (source/eintshechi.F90:36-44). */
+int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const int *ib,
+              const double *t, const double *h0, th_handle **out);
+
+/* Imaginary-axis (E,k) work list, the flattened E_k_imag_mesh of
+ * genLocalEKMesh (source/adaptiveMesh.F90:76-161): point p has wave vector
+ * k[3p..3p+2], energy y[p] (z = E_F + i(y + eta)) and weight w[p] = wght(E) *
+ * w_k.  Meshes may differ per energy (adaptive meshes).  n >= 1. */
+int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, const double *w);
+
+/* Real-axis k-mesh (realBZ%kp / realBZ%w, source/mod_BrillouinZone.F90:
+ * 66-108), combined with the n_x real energies given to th_chi0. */
+int th_set_real_kmesh(th_handle *h, long nk, const double *k, const double *w);
+
+/* chi0(q, omega) for one frequency: the body of eintshechi(q, e)
+ * (source/eintshechi.F90:2-178):
+ *   chi0 = (1/2pi) [ sum_imag w (Gq G + (G Gq)^*)  +  sum_real ... ]
+ * with Gq = G(k+q, E_F+omega+i(y+eta)), G = G(k, E_F+i(y+etap)) on the
+ * imaginary axis and, when |omega| >= 1e-15, Gq = G(k+q, x+omega+i eta),
+ * G = G(k, x+i etap) for x in x2 (weights p2, from generate_real_epoints,
+ * source/EnergyIntegration.F90:99-122).  chi0_out is D x D complex,
+ * D = 4*nOrb*nOrb*nA (chiorb_hf, source/mod_susceptibilities.F90:8). */
+int th_chi0(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
+            int n_x, const double *x2, const double *p2, int precision, double *chi0_out);
+
+/* RPA Dyson step chi = (I + chi0 U)^-1 chi0 in FP64 on the GPU
+ * (source/calculate_chi.F90:84-89; U = Umatorb of
+ * build_identity_and_U_matrix, source/mod_susceptibilities.F90:106-159).
+ * Evaluated through the support of U (push-through identity), which gives
+ * the same matrix.  cond_est (may be NULL) receives the 1-norm condition
+ * number of the reduced matrix (I + chi0[S,S] U[S,S]) that carries all of
+ * the near-pole ill-conditioning. */
+int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out, double *cond_est);
+
+/* Full frequency sweep on the device (calculate_chi's omega loop,
+ * source/calculate_chi.F90:66-107): for each omega[w] computes chi0 and chi
+ * and their orbital traces
+ *   schi[(sigma,i),(sigma',j)] = sum_{mu,nu} chi[(sigma,i,mu,mu),(sigma',j,nu,nu)]
+ * (4nA x 4nA complex, index sigma*nA + i).  Real-axis nodes are generated on
+ * the host per omega with parts3 x n3gl Gauss-Legendre points.  Any of the
+ * outputs may be NULL.  chi_full_out, if given, receives the last omega's
+ * full chi (D x D).  cond_out[w] as th_dyson. */
+int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega, double Ef,
+                 double eta, double etap, int parts3, int n3gl, const double *U, int precision,
+                 double *schi0_out, double *schi_out, double *cond_out, double *chi_full_out);
+
+/* Releases every device buffer of the handle. */
+int th_destroy(th_handle *h);
+
+/* Text of the last error on this thread (never NULL). */
+const char *th_last_error(void);
+
+/* Limits of this build: max atom-block size (2*nOrb), max d handled by the
+ * LDS-resident inversion, max supported nA. */
+int th_limits(int *max_nOrb, int *max_d, int *max_nA);
+
+/* Device time of the most recent th_chi0 / th_chi_sweep call split by stage
+ * (milliseconds, measured with HIP events on the handle's stream):
+ * t[0] green-function kernels, t[1] contraction GEMM, t[2] reduction +
+ * finalize, t[3] Dyson, t[4] whole call.  Also the number of kernel
+ * launches of each kind in n[0..3]. */
+int th_last_timing(th_handle *h, double *t, long *n);
+
+
+/* Kernel-level test hooks (no counterpart in the reference; used by the
+ * GPU unit tests to check one kernel at a time).
+ * thx_green_panel: runs the green-function kernel in 'mode' (0 imag A,
+ *   1 imag B, 2 real A, 3 real B) for pairs p0..p0+np-1 and returns the
+ *   panel [np][nA*nA][2][Mpad] as doubles, Mpad = ceil((2 nOrb)^2/112)*112.
+ * thx_contract: uploads panels A, B of that layout, runs the contraction and
+ *   returns the accumulator [nA*nA][2][Mpad][Mpad] (re block, im block). */
+int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, double omega, double Ef,
+                    double eta, double etap, int prec, int n_x, const double *x2, const double *p2, double *out);
+int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,231 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 CPU
+oracle and the reference's Fe_bulk_susc fixture.
+
+Tolerances: FP64 path, relative Frobenius error <= 1e-10 against the
+oracle on identical inputs; fixture comparisons at the fixture's own print
+precision (10 significant digits)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from titan_amd import binding as B
+from titan_amd import configs as CF
+from titan_amd import contour as Ct
+from titan_amd.system import TBModel
+
+pytestmark = pytest.mark.gpu
+_dp = C.POINTER(C.c_double)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _hooks():
+    L = B.load()
+    L.thx_green_panel.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_long, _dp, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.thx_contract.argtypes = [C.c_void_p, _dp, _dp, C.c_long, C.c_int, _dp]
+    return L
+
+
+def _work(model, nk, parts=1, n1gl=8):
+    y, wy = Ct.imag_epoints(parts, n1gl)
+    dims = nk if isinstance(nk, tuple) else (nk, nk, nk)
+    kp, wk = Ct.uniform_kmesh(model.b, *dims)
+    K = np.tile(kp, (len(y), 1))
+    Y = np.repeat(y, len(wk))
+    W = np.outer(wy, wk).ravel()
+    return (K, Y, W), (kp, wk)
+
+
+@pytest.fixture(scope="module")
+def fe(fe_fixture):
+    return TBModel(fe_fixture["system"]), O.OracleSystem(fe_fixture["system"])
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    spec = CF.system(0)
+    return TBModel(spec), O.OracleSystem(spec)
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    spec = CF.system(2)
+    return TBModel(spec), O.OracleSystem(spec)
+
+
+# --------------------------------------------------------------------------
+# kernel level
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("which", ["fe", "cfg2"])
+def test_green_panels_match_numpy(which, fe, cfg2):
+    m, _ = {"fe": fe, "cfg2": cfg2}[which]
+    L = _hooks()
+    h = m.handle()
+    (K, Y, W), (kp, wk) = _work(m, 2, n1gl=3)
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    bs, nA = 2 * m.nOrb, m.nA
+    Mpad = ((bs * bs + 111) // 112) * 112
+    q = np.array([0.013, -0.021, 0.007])
+    om, eta, etap = 3e-3, 1e-3, 7e-4
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 3)
+    npair = 6
+    for mode in range(4):
+        out = np.zeros(npair * nA * nA * 2 * Mpad)
+        rc = L.thx_green_panel(h._h, mode, 0, npair, q.ctypes.data_as(_dp), om, m.Ef, eta, etap, 0, len(x2),
+                               x2.ctypes.data_as(_dp), p2.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        assert rc == 0, L.th_last_error()
+        out = out.reshape(npair, nA * nA, 2, Mpad)
+        assert np.abs(out[..., bs * bs:]).max() == 0.0
+        for p in range(npair):
+            if mode <= 1:
+                k, yv, w = K[p], Y[p], W[p]
+                if mode == 0:
+                    G = np.linalg.inv((m.Ef + om + 1j * (yv + eta)) * np.eye(m.d) - m.hk(k + q))
+                else:
+                    G = np.linalg.inv((m.Ef + 1j * (yv + etap)) * np.eye(m.d) - m.hk(k))
+            else:
+                ie, ik = divmod(p, len(wk))
+                k, w = kp[ik], p2[ie] * wk[ik]
+                if mode == 2:
+                    G = np.linalg.inv((x2[ie] + om + 1j * eta) * np.eye(m.d) - m.hk(k + q))
+                else:
+                    G = np.linalg.inv((x2[ie] + 1j * etap) * np.eye(m.d) - m.hk(k))
+            for ij in range(nA * nA):
+                i, j = divmod(ij, nA)
+                if mode == 0:
+                    ref = w * G[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
+                elif mode == 1:
+                    ref = G[j * bs:(j + 1) * bs, i * bs:(i + 1) * bs]
+                elif mode == 2:
+                    Ah = G - G.conj().T
+                    ref = -1j * w * Ah[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
+                else:
+                    ref = G.conj().T[j * bs:(j + 1) * bs, i * bs:(i + 1) * bs]
+                got = (out[p, ij, 0, :bs * bs] + 1j * out[p, ij, 1, :bs * bs]).reshape(bs, bs)
+                assert rel(got, ref) < 1e-12, (mode, p, ij, rel(got, ref))
+    h.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(B.PREC_FP64, 1e-13), (B.PREC_MIXED, 2e-6)])
+@pytest.mark.parametrize("npair", [1, 37, 5000])
+def test_contraction_matches_numpy(fe, prec, tol, npair):
+    m, _ = fe
+    L = _hooks()
+    h = m.handle()
+    Mpad = 336
+    rng = np.random.default_rng(npair)
+    A = rng.normal(size=(npair, 2, Mpad))
+    Bm = rng.normal(size=(npair, 2, Mpad))
+    A[:, :, 324:] = 0
+    Bm[:, :, 324:] = 0
+    if prec == B.PREC_MIXED:
+        A = A.astype(np.float32).astype(np.float64)
+        Bm = Bm.astype(np.float32).astype(np.float64)
+    acc = np.zeros(2 * Mpad * Mpad)
+    rc = L.thx_contract(h._h, np.ascontiguousarray(A).ctypes.data_as(_dp),
+                        np.ascontiguousarray(Bm).ctypes.data_as(_dp), npair, prec, acc.ctypes.data_as(_dp))
+    assert rc == 0, L.th_last_error()
+    acc = acc.reshape(2, Mpad, Mpad)
+    T = acc[0] + 1j * acc[1]
+    Ac = A[:, 0] + 1j * A[:, 1]
+    Bc = Bm[:, 0] + 1j * Bm[:, 1]
+    ref = Ac.T @ Bc
+    assert rel(T, ref) < tol
+    h.close()
+
+
+# --------------------------------------------------------------------------
+# chi0 and Dyson against the oracle (FP64 path, <= 1e-10)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("which,nk,q,omega", [
+    ("fe", 3, [0, 0, 0], 0.0),
+    ("fe", 3, [0.05, -0.02, 0.11], 4e-3),
+    ("cfg0", 4, [0, 0, 0], 2e-3),
+    ("cfg2", 2, [0, 0, 0], 0.0),
+    ("cfg2", 2, [0.03, 0.0, -0.04], 5e-3),
+])
+def test_chi0_fp64_matches_oracle(which, nk, q, omega, fe, cfg0, cfg2):
+    m, s = {"fe": fe, "cfg0": cfg0, "cfg2": cfg2}[which]
+    (K, Y, W), (kp, wk) = _work(m, nk, n1gl=8)
+    eta, etap = 1e-3, 6e-4
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    x2, p2 = Ct.real_epoints(m.Ef, omega, 1, 6)
+    g = h.chi0(q, omega, m.Ef, eta, etap, x2, p2)
+    if abs(omega) >= 1e-15:
+        o = s.chi0(q, omega, eta, etap, (K, Y, W), (kp, wk), (x2, p2))
+    else:
+        o = s.chi0(q, omega, eta, etap, (K, Y, W))
+    assert rel(g, o) < 1e-10, rel(g, o)
+    # Dyson on the same chi0
+    chi, cond = h.dyson(o, m.Umat)
+    oc = O.dyson(o, s.umat())
+    assert rel(chi, oc) < 1e-10 * max(1.0, cond), (rel(chi, oc), cond)
+    h.close()
+
+
+def test_sweep_equals_single_calls(cfg0):
+    m, s = cfg0
+    (K, Y, W), (kp, wk) = _work(m, 3, n1gl=6)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    oms = [0.0, 1e-3, 7e-3]
+    s0, sc, cond = h.sweep([0, 0, 0], oms, m.Ef, 1e-3, 1e-3, 1, 5, m.Umat)
+    for w, om in enumerate(oms):
+        x2, p2 = Ct.real_epoints(m.Ef, om, 1, 5)
+        c0 = h.chi0([0, 0, 0], om, m.Ef, 1e-3, 1e-3, x2, p2)
+        chi, _ = h.dyson(c0, m.Umat)
+        assert np.abs(O.trace(1, 9, c0) - s0[w]).max() <= 1e-12 * np.abs(s0[w]).max()
+        assert np.abs(O.trace(1, 9, chi) - sc[w]).max() <= 1e-12 * np.abs(sc[w]).max()
+    h.close()
+
+
+def test_repeat_calls_are_bitwise_identical(cfg0):
+    m, _ = cfg0
+    (K, Y, W), (kp, wk) = _work(m, 3, n1gl=6)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    x2, p2 = Ct.real_epoints(m.Ef, 2e-3, 1, 5)
+    a = h.chi0([0, 0, 0], 2e-3, m.Ef, 1e-3, 1e-3, x2, p2)
+    b = h.chi0([0, 0, 0], 2e-3, m.Ef, 1e-3, 1e-3, x2, p2)
+    assert np.array_equal(a, b)
+    h.close()
+
+
+# --------------------------------------------------------------------------
+# the reference's own fixture, computed on the GPU
+# --------------------------------------------------------------------------
+
+def test_gpu_reproduces_fe_bulk_susc_fixture(fe, fe_fixture):
+    m, _ = fe
+    run = fe_fixture["run"]
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    n = round(run["nkpt"] ** (1 / 3))
+    kp, wk = Ct.uniform_kmesh(m.b, n, n, n)
+    h = m.handle()
+    h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+    h.set_real_kmesh(kp, wk)
+    oms = Ct.omega_grid(run["emin"], run["emax"], run["nEner"])
+    assert len(oms) == len(fe_fixture["chi"])
+    s0, sc, cond = h.sweep(run["q"], oms, m.Ef, run["eta"], run["etap"], run["parts3"], run["n3gl"], m.Umat)
+    for w in range(len(oms)):
+        rhf = np.array(fe_fixture["chihf"][w]["schi"])
+        rr = np.array(fe_fixture["chi"][w]["schi"])
+        rhf = rhf[..., 0] + 1j * rhf[..., 1]
+        rr = rr[..., 0] + 1j * rr[..., 1]
+        big = np.abs(rhf) > 1e-9
+        assert np.max(np.abs(s0[w][big] - rhf[big]) / np.abs(rhf[big])) < 1e-9
+        bigr = np.abs(rr) > 1e-3
+        assert np.max(np.abs(sc[w][bigr] - rr[bigr]) / np.abs(rr[bigr])) < 2e-8
+    h.close()

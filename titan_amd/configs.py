@@ -1,0 +1,148 @@
+"""Seeded synthetic systems of the benchmark and parity workloads.
+
+Energies are drawn in eV and converted to Ry with TITAN's ry2ev = 13.6
+(mod_io.F90:183).  Every system is a plain dict accepted by
+titan_amd.system.TBModel; run parameters (mesh, contour, omega grid) come
+with it.  The seed is stored in the dict.
+
+  cfg0  bcc, 1 site, s+p+d (18 x 18 H), 2 shells, 16^3 k, 64 + 32 energies,
+        16 omega in [0, 0.01] Ry, eta = 5e-4 Ry
+  cfg1  fcc(001) slab, 8 sites (144 x 144 H), layer-dependent on-site
+        shifts, 128^2 k, 64 + 32 energies, 32 omega
+  cfg2  bcc supercell, 3 sites (54 x 54 H), 48^3 k, 64 + 32 energies,
+        64 omega clustered near the transverse (FMR) pole
+  cfg3  cfg0 system at 24^3 k, 32 omega (adjoint workload)
+"""
+import numpy as np
+
+RY2EV = 13.6
+
+
+def _ry(x):
+    return np.asarray(x, dtype=float) / RY2EV
+
+
+def seeded_type(rng, nstages):
+    """On-site energies and SK integrals of one element, in Ry."""
+    sk = []
+    for _ in range(nstages):
+        p = rng.uniform(-1.2, 1.2, size=10)
+        p[0] = rng.uniform(-1.5, -0.5)          # sss
+        sk.append(_ry(p).tolist())
+    on = rng.uniform(-2.0, 8.0, size=4)           # s, p, t2g, eg
+    onsite = _ry([on[0], on[1], on[1], on[1], on[2], on[2], on[2], on[3], on[3]]).tolist()
+    return {"onsite": onsite, "sk": sk}
+
+
+def _bcc_vectors(a0):
+    return (0.5 * a0 * np.array([[1, 1, -1], [1, -1, 1], [-1, 1, 1]], dtype=float))
+
+
+def fermi_level(model, filling, nk=8):
+    """E_F at which the k-averaged occupation of H(k) equals 'filling'
+    electrons per cell (bisection on a uniform nk^sysdim mesh)."""
+    from titan_amd.contour import uniform_kmesh
+    dims = (nk, nk, nk) if model.sysdim == 3 else ((nk, nk, 1) if model.sysdim == 2 else (nk, 1, 1))
+    kp, w = uniform_kmesh(model.b, *dims)
+    ev = np.array([np.linalg.eigvalsh(model.hk(k)) for k in kp])
+    lo, hi = ev.min(), ev.max()
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        n = np.sum(w[:, None] * (ev <= mid))
+        if n < filling:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def _finish(spec, filling):
+    from titan_amd.system import TBModel
+    spec["Ef"] = 0.0
+    m = TBModel(spec)
+    spec["Ef"] = float(fermi_level(m, filling))
+    return spec
+
+
+def bcc_1site(seed=0, a0=5.30, nstages=2, lam_d_ev=0.06, U_ev=1.0, m_d=2.2):
+    rng = np.random.default_rng(seed)
+    ty = seeded_type(rng, nstages)
+    ty["lambda"] = [0.0, float(_ry(lam_d_ev))]
+    ty["U"] = [0.0, float(_ry(U_ev))]
+    mz = [0.0] * 4 + [m_d / 5] * 5
+    spec = {
+        "name": "bcc 1-site seeded", "seed": seed, "sysdim": 3, "a": _bcc_vectors(a0).tolist(),
+        "pos": [[0.0, 0.0, 0.0]], "type": [0], "orbs": list(range(1, 10)), "nStages": nstages,
+        "relTol": 0.05, "types": [ty], "rho": [[0.0] * 9], "mx": [[0.0] * 9], "my": [[0.0] * 9],
+        "mz": [mz], "socscale": 1.0,
+    }
+    return _finish(spec, filling=8.0)
+
+
+def bcc_supercell(nsites=3, seed=2, a0=5.30, nstages=2, lam_d_ev=0.06, U_ev=1.0, m_d=2.2):
+    """nsites bcc primitive cells stacked along a1, one element."""
+    rng = np.random.default_rng(seed)
+    ty = seeded_type(rng, nstages)
+    ty["lambda"] = [0.0, float(_ry(lam_d_ev))]
+    ty["U"] = [0.0, float(_ry(U_ev))]
+    a = _bcc_vectors(a0)
+    A = np.array([nsites * a[0], a[1], a[2]])
+    pos = [(k * a[0]).tolist() for k in range(nsites)]
+    mz = [0.0] * 4 + [m_d / 5] * 5
+    spec = {
+        "name": "bcc %d-site supercell seeded" % nsites, "seed": seed, "sysdim": 3, "a": A.tolist(),
+        "pos": pos, "type": [0] * nsites, "orbs": list(range(1, 10)), "nStages": nstages, "relTol": 0.05,
+        "types": [ty], "rho": [[0.0] * 9] * nsites, "mx": [[0.0] * 9] * nsites, "my": [[0.0] * 9] * nsites,
+        "mz": [mz] * nsites, "socscale": 1.0,
+    }
+    return _finish(spec, filling=8.0 * nsites)
+
+
+def fcc001_slab(nlayers=8, seed=1, a0=6.65, nstages=2, lam_d_ev=0.06, U_ev=1.0, m_d=2.2):
+    """fcc(001) slab: square in-plane lattice a0/sqrt(2), layers a0/2 apart,
+    ABAB stacking; one element with layer-dependent on-site shifts."""
+    rng = np.random.default_rng(seed)
+    base = seeded_type(rng, nstages)
+    types = []
+    for _ in range(nlayers):
+        t = {"onsite": (np.asarray(base["onsite"]) + _ry(rng.uniform(-0.3, 0.3))).tolist(), "sk": base["sk"],
+             "lambda": [0.0, float(_ry(lam_d_ev))], "U": [0.0, float(_ry(U_ev))]}
+        types.append(t)
+    h = 0.5 * a0
+    a = np.array([[h, h, 0.0], [-h, h, 0.0], [0.0, 0.0, 0.0]])
+    pos = [[(l % 2) * h, 0.0, l * h] for l in range(nlayers)]
+    mz = [0.0] * 4 + [m_d / 5] * 5
+    spec = {
+        "name": "fcc(001) %d-layer slab seeded" % nlayers, "seed": seed, "sysdim": 2, "a": a.tolist(),
+        "pos": pos, "type": list(range(nlayers)), "orbs": list(range(1, 10)), "nStages": nstages,
+        "relTol": 0.05, "types": types, "rho": [[0.0] * 9] * nlayers, "mx": [[0.0] * 9] * nlayers,
+        "my": [[0.0] * 9] * nlayers, "mz": [mz] * nlayers, "socscale": 1.0,
+    }
+    return _finish(spec, filling=8.0 * nlayers)
+
+
+def run_params(cfg):
+    """Mesh / contour / omega parameters of each configuration."""
+    if cfg == 0:
+        return {"nk": (16, 16, 16), "parts": 1, "n1gl": 64, "parts3": 1, "n3gl": 32, "eta": 5e-4, "etap": 5e-4,
+                "omega": np.linspace(0.0, 0.01, 16).tolist(), "q": [0.0, 0.0, 0.0]}
+    if cfg == 1:
+        return {"nk": (128, 128, 1), "parts": 1, "n1gl": 64, "parts3": 1, "n3gl": 32, "eta": 5e-4, "etap": 5e-4,
+                "omega": np.linspace(0.0, 0.01, 32).tolist(), "q": [0.0, 0.0, 0.0]}
+    if cfg == 2:
+        return {"nk": (48, 48, 48), "parts": 1, "n1gl": 64, "parts3": 1, "n3gl": 32, "eta": 5e-4, "etap": 5e-4,
+                "omega": None, "q": [0.0, 0.0, 0.0]}
+    if cfg == 3:
+        return {"nk": (24, 24, 24), "parts": 1, "n1gl": 64, "parts3": 1, "n3gl": 32, "eta": 5e-4, "etap": 5e-4,
+                "omega": np.linspace(0.0, 0.01, 32).tolist(), "q": [0.0, 0.0, 0.0]}
+    raise ValueError("unknown configuration %r" % cfg)
+
+
+def system(cfg):
+    if cfg in (0, 3):
+        return bcc_1site()
+    if cfg == 1:
+        return fcc001_slab()
+    if cfg == 2:
+        return bcc_supercell(3)
+    raise ValueError("unknown configuration %r" % cfg)

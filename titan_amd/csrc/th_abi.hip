@@ -1,0 +1,562 @@
+// th_abi.hip -- the C ABI (include/titan_hip.h): handle life cycle, device
+// buffers and the per-frequency orchestration of the gfx950 kernels.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "th_internal.h"
+
+namespace {
+thread_local std::string g_err = "";
+
+// Gauss-Legendre nodes on [x1,x2] (Newton on P_n from the Tricomi guess),
+// the rule of EnergyIntegration.F90:125-157, for the real-axis nodes of
+// th_chi_sweep (EnergyIntegration.F90:99-122).
+void gauleg(double x1, double x2, int n, double *x, double *w)
+{
+    const int m = (n + 1) / 2;
+    const double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+    for (int i = 1; i <= m; ++i) {
+        double z = std::cos(3.141592654 * (i - 0.25) / (n + 0.5)), z1 = 0.0, pp = 0.0;
+        do {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+        } while (std::fabs(z - z1) > 3.0e-14);
+        x[i - 1] = xm - xl * z;
+        x[n - i] = xm + xl * z;
+        w[i - 1] = (2.0 * xl) / ((1.0 - z * z) * pp * pp);
+        w[n - i] = w[i - 1];
+    }
+}
+
+struct status_block {
+    int *d = nullptr;
+};
+
+}  // namespace
+
+void th_set_error(const std::string &msg) { g_err = msg; }
+
+// per-handle device status word (first zero pivot, INT_MAX if none); kept
+// in a side table so the handle layout stays plain
+static std::vector<std::pair<th_handle *, int *>> g_status;
+
+int *th_status_word(th_handle *h)
+{
+    for (auto &e : g_status)
+        if (e.first == h) return e.second;
+    return nullptr;
+}
+
+static int reset_status(th_handle *h)
+{
+    const int v = INT_MAX;
+    TH_HIP(hipMemcpyAsync(th_status_word(h), &v, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    return TH_OK;
+}
+
+int th_check_status(th_handle *h)
+{
+    int v = INT_MAX;
+    TH_HIP(hipMemcpyAsync(&v, th_status_word(h), sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    TH_HIP(hipStreamSynchronize(h->stream));
+    if (v != INT_MAX) {
+        th_set_error("exactly singular pivot in an inversion, step " + std::to_string(v));
+        return v;
+    }
+    return TH_OK;
+}
+
+namespace {
+
+// stage timing with HIP events on the handle's stream
+struct stage_timer {
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    th_handle *h;
+    explicit stage_timer(th_handle *hh) : h(hh) {}
+    int begin(int stage)
+    {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+        hipEventRecord(a, h->stream);
+        ev.push_back({stage, {a, b}});
+        return (int)ev.size() - 1;
+    }
+    void end(int id)
+    {
+        if (id >= 0) hipEventRecord(ev[id].second.second, h->stream);
+    }
+    void collect()
+    {
+        for (int s = 0; s < 5; ++s) h->t_ms[s] = 0.0;
+        for (auto &e : ev) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) h->t_ms[e.first] += ms;
+        }
+    }
+    ~stage_timer()
+    {
+        for (auto &e : ev) {
+            hipEventDestroy(e.second.first);
+            hipEventDestroy(e.second.second);
+        }
+    }
+};
+
+size_t prec_size(int prec) { return prec == TH_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+int ensure_panels(th_handle *h, long pairs)
+{
+    const size_t need = (size_t)pairs * h->nA2 * 2 * h->Mpad * sizeof(double);
+    if (need <= h->panel_bytes) return TH_OK;
+    hipFree(h->d_pA);
+    hipFree(h->d_pB);
+    h->d_pA = h->d_pB = nullptr;
+    h->panel_bytes = 0;
+    TH_HIP(hipMalloc(&h->d_pA, need));
+    TH_HIP(hipMalloc(&h->d_pB, need));
+    h->panel_bytes = need;
+    return TH_OK;
+}
+
+long batch_pairs(th_handle *h, long total)
+{
+    const size_t per = (size_t)h->nA2 * 2 * h->Mpad * sizeof(double);
+    const size_t budget = (size_t)512 << 20;
+    long b = (long)std::max<size_t>(TH_KC, budget / per);
+    return std::min(b, std::max<long>(total, 1));
+}
+
+// imaginary-axis B panels G(k, Ef + i(y+etap)) do not depend on omega:
+// computed once per (Ef, etap, precision) and kept while they fit
+int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &tm, bool *have)
+{
+    *have = false;
+    if (h->d_bcache && h->bcache_prec == prec && h->bcache_key[0] == Ef && h->bcache_key[1] == etap) {
+        *have = true;
+        return TH_OK;
+    }
+    hipFree(h->d_bcache);
+    h->d_bcache = nullptr;
+    h->bcache_prec = -1;
+    const size_t need = (size_t)h->n_im * h->nA2 * 2 * h->Mpad * prec_size(prec);
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return TH_OK;
+    if (need > freeb / 2 || need > ((size_t)64 << 30)) return TH_OK;   // recompute per batch instead
+    if (hipMalloc(&h->d_bcache, need) != hipSuccess) {
+        h->d_bcache = nullptr;
+        (void)hipGetLastError();
+        return TH_OK;
+    }
+    const long bp = batch_pairs(h, h->n_im) * 4;
+    for (long p0 = 0; p0 < h->n_im; p0 += bp) {
+        th_green_args a{};
+        a.mode = TH_G_IMAG_B;
+        a.p0 = p0;
+        a.np = std::min(bp, h->n_im - p0);
+        a.Ef = Ef;
+        a.etap = etap;
+        a.prec = prec;
+        a.panel = (char *)h->d_bcache + (size_t)p0 * h->nA2 * 2 * h->Mpad * prec_size(prec);
+        int id = tm.begin(0);
+        int rc = th_launch_green(h, a);
+        tm.end(id);
+        if (rc) return rc;
+    }
+    h->bcache_prec = prec;
+    h->bcache_key[0] = Ef;
+    h->bcache_key[1] = etap;
+    *have = true;
+    return TH_OK;
+}
+
+// chi0 of one omega into h->d_chi0 (device)
+int chi0_device(th_handle *h, const double *q, double omega, double Ef, double eta, double etap, int n_x,
+                const double *x2, const double *p2, int prec, stage_timer &tm)
+{
+    if (!h->d_kim || h->n_im <= 0) {
+        th_set_error("th_set_imag_points must be called before th_chi0");
+        return TH_E_STATE;
+    }
+    const bool real_axis = std::fabs(omega) >= 1e-15 && n_x > 0;
+    if (real_axis && (!h->d_kre || h->nk_re <= 0)) {
+        th_set_error("th_set_real_kmesh must be called before th_chi0 at omega != 0");
+        return TH_E_STATE;
+    }
+    if (real_axis) {
+        if (n_x > h->nx_cap) {
+            hipFree(h->d_x2);
+            hipFree(h->d_p2);
+            TH_HIP(hipMalloc(&h->d_x2, sizeof(double) * n_x));
+            TH_HIP(hipMalloc(&h->d_p2, sizeof(double) * n_x));
+            h->nx_cap = n_x;
+        }
+        TH_HIP(hipMemcpyAsync(h->d_x2, x2, sizeof(double) * n_x, hipMemcpyHostToDevice, h->stream));
+        TH_HIP(hipMemcpyAsync(h->d_p2, p2, sizeof(double) * n_x, hipMemcpyHostToDevice, h->stream));
+    }
+    const size_t nacc = (size_t)2 * h->nA2 * 2 * h->Mpad * h->Mpad;
+    int rc = th_launch_zero(h, h->d_acc, nacc);
+    if (rc) return rc;
+    bool have_b = false;
+    rc = ensure_bcache(h, Ef, etap, prec, tm, &have_b);
+    if (rc) return rc;
+    const size_t pstride = (size_t)h->nA2 * 2 * h->Mpad * prec_size(prec);
+    // imaginary axis
+    const long bp = batch_pairs(h, h->n_im);
+    if ((rc = ensure_panels(h, bp))) return rc;
+    for (long p0 = 0; p0 < h->n_im; p0 += bp) {
+        const long np = std::min(bp, h->n_im - p0);
+        th_green_args a{};
+        a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
+        a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
+        a.mode = TH_G_IMAG_A;
+        a.panel = h->d_pA;
+        int id = tm.begin(0);
+        if ((rc = th_launch_green(h, a))) return rc;
+        const void *pB = nullptr;
+        if (have_b) {
+            pB = (const char *)h->d_bcache + (size_t)p0 * pstride;
+        } else {
+            a.mode = TH_G_IMAG_B;
+            a.panel = h->d_pB;
+            if ((rc = th_launch_green(h, a))) return rc;
+            pB = h->d_pB;
+        }
+        tm.end(id);
+        id = tm.begin(1);
+        if ((rc = th_launch_contract(h, h->d_pA, pB, np, prec, 0))) return rc;
+        tm.end(id);
+    }
+    // real axis
+    if (real_axis) {
+        const long n_re = (long)n_x * h->nk_re;
+        const long br = batch_pairs(h, n_re);
+        if ((rc = ensure_panels(h, br))) return rc;
+        for (long p0 = 0; p0 < n_re; p0 += br) {
+            const long np = std::min(br, n_re - p0);
+            th_green_args a{};
+            a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
+            a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
+            a.mode = TH_G_REAL_A;
+            a.panel = h->d_pA;
+            int id = tm.begin(0);
+            if ((rc = th_launch_green(h, a))) return rc;
+            a.mode = TH_G_REAL_B;
+            a.panel = h->d_pB;
+            if ((rc = th_launch_green(h, a))) return rc;
+            tm.end(id);
+            id = tm.begin(1);
+            if ((rc = th_launch_contract(h, h->d_pA, h->d_pB, np, prec, 1))) return rc;
+            tm.end(id);
+        }
+    }
+    int id = tm.begin(2);
+    rc = th_launch_finalize(h, h->d_chi0);
+    tm.end(id);
+    return rc;
+}
+
+bool bad(const void *p) { return p == nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+const char *th_last_error(void) { return g_err.c_str(); }
+
+int th_limits(int *max_nOrb, int *max_d, int *max_nA)
+{
+    if (max_nOrb) *max_nOrb = TH_MAX_NORB;
+    if (max_d) *max_d = TH_MAX_D_LDS;
+    if (max_nA) *max_nA = TH_MAX_NA;
+    return TH_OK;
+}
+
+int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const int *ib, const double *t,
+              const double *h0, th_handle **out)
+{
+    if (!out) { th_set_error("th_create: out is NULL"); return TH_E_ARG; }
+    *out = nullptr;
+    if (nA < 1 || nA > TH_MAX_NA || nOrb < 1 || nOrb > TH_MAX_NORB || nhop < 0 || bad(h0) ||
+        (nhop > 0 && (bad(R) || bad(jb) || bad(ib) || bad(t)))) {
+        th_set_error("th_create: bad sizes or NULL arrays");
+        return TH_E_ARG;
+    }
+    for (int n = 0; n < nhop; ++n)
+        if (jb[n] < 0 || jb[n] >= nA || ib[n] < 0 || ib[n] >= nA) {
+            th_set_error("th_create: hop endpoint outside 0..nA-1 at entry " + std::to_string(n));
+            return TH_E_ARG;
+        }
+    const int d = 2 * nOrb * nA;
+    if (d > TH_MAX_D_LDS) {
+        th_set_error("th_create: d = 2*nOrb*nA = " + std::to_string(d) + " exceeds " + std::to_string(TH_MAX_D_LDS));
+        return TH_E_UNSUPPORTED;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        th_set_error("th_create: no HIP device");
+        return TH_E_NODEVICE;
+    }
+    th_handle *h = new th_handle();
+    TH_HIP(hipGetDevice(&h->device));
+    hipDeviceProp_t prop;
+    TH_HIP(hipGetDeviceProperties(&prop, h->device));
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
+        th_set_error(std::string("th_create: device is ") + prop.gcnArchName + ", this library is built for gfx950");
+        delete h;
+        return TH_E_NODEVICE;
+    }
+    TH_HIP(hipStreamCreate(&h->stream));
+    h->nA = nA; h->nOrb = nOrb; h->bs = 2 * nOrb; h->d = d; h->D = 4 * nOrb * nOrb * nA; h->nhop = nhop;
+    h->M = h->bs * h->bs; h->Mpad = th_mpad(h->M); h->nA2 = nA * nA;
+    const int ntile = h->Mpad / TH_TILE;
+    h->splits = std::max(1, (2 * prop.multiProcessorCount) / (ntile * ntile * h->nA2));
+    int *st = nullptr;
+    TH_HIP(hipMalloc(&st, sizeof(int)));
+    g_status.push_back({h, st});
+    if (nhop > 0) {
+        TH_HIP(hipMalloc(&h->d_R, sizeof(double) * 3 * nhop));
+        TH_HIP(hipMalloc(&h->d_jb, sizeof(int) * nhop));
+        TH_HIP(hipMalloc(&h->d_ib, sizeof(int) * nhop));
+        TH_HIP(hipMalloc(&h->d_t, sizeof(double2) * (size_t)nhop * nOrb * nOrb));
+        TH_HIP(hipMemcpy(h->d_R, R, sizeof(double) * 3 * nhop, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_jb, jb, sizeof(int) * nhop, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_ib, ib, sizeof(int) * nhop, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_t, t, sizeof(double2) * (size_t)nhop * nOrb * nOrb, hipMemcpyHostToDevice));
+    }
+    TH_HIP(hipMalloc(&h->d_h0, sizeof(double2) * (size_t)d * d));
+    TH_HIP(hipMemcpy(h->d_h0, h0, sizeof(double2) * (size_t)d * d, hipMemcpyHostToDevice));
+    const size_t MM = (size_t)h->Mpad * h->Mpad;
+    TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)h->splits * h->nA2 * 2 * MM));
+    TH_HIP(hipMalloc(&h->d_acc, sizeof(double) * (size_t)2 * h->nA2 * 2 * MM));
+    TH_HIP(hipMalloc(&h->d_chi0, sizeof(double2) * (size_t)h->D * h->D));
+    TH_HIP(hipMalloc(&h->d_chi, sizeof(double2) * (size_t)h->D * h->D));
+    *out = h;
+    return TH_OK;
+}
+
+int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, const double *w)
+{
+    if (!h || n < 1 || bad(k) || bad(y) || bad(w)) { th_set_error("th_set_imag_points: bad arguments"); return TH_E_ARG; }
+    hipFree(h->d_kim); hipFree(h->d_yim); hipFree(h->d_wim); hipFree(h->d_bcache);
+    h->d_kim = h->d_yim = h->d_wim = nullptr;
+    h->d_bcache = nullptr;
+    h->bcache_prec = -1;
+    TH_HIP(hipMalloc(&h->d_kim, sizeof(double) * 3 * n));
+    TH_HIP(hipMalloc(&h->d_yim, sizeof(double) * n));
+    TH_HIP(hipMalloc(&h->d_wim, sizeof(double) * n));
+    TH_HIP(hipMemcpy(h->d_kim, k, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    TH_HIP(hipMemcpy(h->d_yim, y, sizeof(double) * n, hipMemcpyHostToDevice));
+    TH_HIP(hipMemcpy(h->d_wim, w, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->n_im = n;
+    return TH_OK;
+}
+
+int th_set_real_kmesh(th_handle *h, long nk, const double *k, const double *w)
+{
+    if (!h || nk < 1 || bad(k) || bad(w)) { th_set_error("th_set_real_kmesh: bad arguments"); return TH_E_ARG; }
+    hipFree(h->d_kre); hipFree(h->d_wre);
+    h->d_kre = h->d_wre = nullptr;
+    TH_HIP(hipMalloc(&h->d_kre, sizeof(double) * 3 * nk));
+    TH_HIP(hipMalloc(&h->d_wre, sizeof(double) * nk));
+    TH_HIP(hipMemcpy(h->d_kre, k, sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
+    TH_HIP(hipMemcpy(h->d_wre, w, sizeof(double) * nk, hipMemcpyHostToDevice));
+    h->nk_re = nk;
+    return TH_OK;
+}
+
+int th_chi0(th_handle *h, const double *q, double omega, double Ef, double eta, double etap, int n_x,
+            const double *x2, const double *p2, int precision, double *chi0_out)
+{
+    if (!h || bad(q) || bad(chi0_out) || (precision != TH_PREC_FP64 && precision != TH_PREC_MIXED) ||
+        (std::fabs(omega) >= 1e-15 && n_x > 0 && (bad(x2) || bad(p2))) || n_x < 0) {
+        th_set_error("th_chi0: bad arguments");
+        return TH_E_ARG;
+    }
+    stage_timer tm(h);
+    int id = tm.begin(4);
+    for (auto &c : h->n_launch) c = 0;
+    int rc = reset_status(h);
+    if (!rc) rc = chi0_device(h, q, omega, Ef, eta, etap, n_x, x2, p2, precision, tm);
+    tm.end(id);
+    if (rc) return rc;
+    TH_HIP(hipMemcpyAsync(chi0_out, h->d_chi0, sizeof(double2) * (size_t)h->D * h->D, hipMemcpyDeviceToHost,
+                          h->stream));
+    rc = th_check_status(h);
+    tm.collect();
+    return rc;
+}
+
+int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out, double *cond_est)
+{
+    if (!h || bad(chi0) || bad(U) || bad(chi_out)) { th_set_error("th_dyson: bad arguments"); return TH_E_ARG; }
+    const size_t n = (size_t)h->D * h->D;
+    int rc = reset_status(h);
+    if (rc) return rc;
+    TH_HIP(hipMemcpyAsync(h->d_chi0, chi0, sizeof(double2) * n, hipMemcpyHostToDevice, h->stream));
+    stage_timer tm(h);
+    int id = tm.begin(3);
+    rc = th_dyson_device(h, h->d_chi0, reinterpret_cast<const double2 *>(U), h->d_chi, cond_est);
+    tm.end(id);
+    if (rc) return rc;
+    TH_HIP(hipMemcpyAsync(chi_out, h->d_chi, sizeof(double2) * n, hipMemcpyDeviceToHost, h->stream));
+    rc = th_check_status(h);
+    tm.collect();
+    return rc;
+}
+
+int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega, double Ef, double eta,
+                 double etap, int parts3, int n3gl, const double *U, int precision, double *schi0_out,
+                 double *schi_out, double *cond_out, double *chi_full_out)
+{
+    if (!h || bad(q) || n_omega < 1 || bad(omega) || parts3 < 1 || n3gl < 1 || bad(U) ||
+        (precision != TH_PREC_FP64 && precision != TH_PREC_MIXED)) {
+        th_set_error("th_chi_sweep: bad arguments");
+        return TH_E_ARG;
+    }
+    const int n4 = 4 * h->nA, n_x = parts3 * n3gl;
+    std::vector<double> x2(n_x), p2(n_x);
+    double2 *d_s = nullptr;
+    TH_HIP(hipMalloc(&d_s, sizeof(double2) * 2 * n4 * n4));
+    stage_timer tm(h);
+    int idall = tm.begin(4);
+    for (auto &c : h->n_launch) c = 0;
+    int rc = reset_status(h);
+    for (int w = 0; w < n_omega && !rc; ++w) {
+        const double e = omega[w];
+        if (std::fabs(e) >= 1e-15)
+            for (int k = 1; k <= parts3; ++k) {
+                double e1 = Ef - ((parts3 - (k - 1)) * e / parts3);
+                double e2 = Ef - ((parts3 - k) * e / parts3);
+                gauleg(e1, e2, n3gl, x2.data() + (k - 1) * n3gl, p2.data() + (k - 1) * n3gl);
+            }
+        rc = chi0_device(h, q, e, Ef, eta, etap, n_x, x2.data(), p2.data(), precision, tm);
+        if (rc) break;
+        int id = tm.begin(3);
+        double cond = 0.0;
+        rc = th_dyson_device(h, h->d_chi0, reinterpret_cast<const double2 *>(U), h->d_chi, cond_out ? &cond : nullptr);
+        if (!rc) rc = th_launch_trace(h, h->d_chi0, d_s);
+        if (!rc) rc = th_launch_trace(h, h->d_chi, d_s + n4 * n4);
+        tm.end(id);
+        if (rc) break;
+        if (cond_out) cond_out[w] = cond;
+        if (schi0_out)
+            TH_HIP(hipMemcpyAsync(schi0_out + (size_t)w * 2 * n4 * n4, d_s, sizeof(double2) * n4 * n4,
+                                  hipMemcpyDeviceToHost, h->stream));
+        if (schi_out)
+            TH_HIP(hipMemcpyAsync(schi_out + (size_t)w * 2 * n4 * n4, d_s + n4 * n4, sizeof(double2) * n4 * n4,
+                                  hipMemcpyDeviceToHost, h->stream));
+        TH_HIP(hipStreamSynchronize(h->stream));
+    }
+    tm.end(idall);
+    if (!rc && chi_full_out)
+        TH_HIP(hipMemcpyAsync(chi_full_out, h->d_chi, sizeof(double2) * (size_t)h->D * h->D, hipMemcpyDeviceToHost,
+                              h->stream));
+    if (!rc) rc = th_check_status(h);
+    tm.collect();
+    hipFree(d_s);
+    return rc;
+}
+
+int th_last_timing(th_handle *h, double *t, long *n)
+{
+    if (!h) return TH_E_ARG;
+    if (t)
+        for (int s = 0; s < 5; ++s) t[s] = h->t_ms[s];
+    if (n)
+        for (int s = 0; s < 4; ++s) n[s] = h->n_launch[s];
+    return TH_OK;
+}
+
+int th_destroy(th_handle *h)
+{
+    if (!h) return TH_OK;
+    hipStreamSynchronize(h->stream);
+    void *bufs[] = {h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
+                    h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
+                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    for (size_t i = 0; i < g_status.size(); ++i)
+        if (g_status[i].first == h) {
+            hipFree(g_status[i].second);
+            g_status.erase(g_status.begin() + i);
+            break;
+        }
+    hipStreamDestroy(h->stream);
+    delete h;
+    return TH_OK;
+}
+
+
+/* ---- kernel-level test hooks (used by the GPU unit tests) ---- */
+
+int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, double omega, double Ef,
+                    double eta, double etap, int prec, int n_x, const double *x2, const double *p2, double *out)
+{
+    if (!h || np < 1 || bad(q) || bad(out)) { th_set_error("thx_green_panel: bad arguments"); return TH_E_ARG; }
+    long total = (mode <= TH_G_IMAG_B) ? h->n_im : (long)n_x * h->nk_re;
+    if (p0 < 0 || p0 + np > total) { th_set_error("thx_green_panel: pair range"); return TH_E_ARG; }
+    int rc = reset_status(h);
+    if (rc) return rc;
+    if (mode >= TH_G_REAL_A) {
+        if (n_x > h->nx_cap) {
+            hipFree(h->d_x2); hipFree(h->d_p2);
+            TH_HIP(hipMalloc(&h->d_x2, sizeof(double) * n_x));
+            TH_HIP(hipMalloc(&h->d_p2, sizeof(double) * n_x));
+            h->nx_cap = n_x;
+        }
+        TH_HIP(hipMemcpy(h->d_x2, x2, sizeof(double) * n_x, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_p2, p2, sizeof(double) * n_x, hipMemcpyHostToDevice));
+    }
+    if ((rc = ensure_panels(h, np))) return rc;
+    th_green_args a{};
+    a.mode = mode; a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
+    a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec; a.panel = h->d_pA;
+    if ((rc = th_launch_green(h, a))) return rc;
+    const size_t n = (size_t)np * h->nA2 * 2 * h->Mpad;
+    if (prec == TH_PREC_FP64) {
+        TH_HIP(hipMemcpyAsync(out, h->d_pA, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    } else {
+        std::vector<float> tmp(n);
+        TH_HIP(hipMemcpyAsync(tmp.data(), h->d_pA, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+        TH_HIP(hipStreamSynchronize(h->stream));
+        for (size_t e = 0; e < n; ++e) out[e] = tmp[e];
+    }
+    return th_check_status(h);
+}
+
+int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out)
+{
+    if (!h || np < 1 || bad(A) || bad(B) || bad(acc_out)) { th_set_error("thx_contract: bad arguments"); return TH_E_ARG; }
+    int rc = ensure_panels(h, np);
+    if (rc) return rc;
+    const size_t n = (size_t)np * h->nA2 * 2 * h->Mpad;
+    if (prec == TH_PREC_FP64) {
+        TH_HIP(hipMemcpy(h->d_pA, A, sizeof(double) * n, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_pB, B, sizeof(double) * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> a(n), b(n);
+        for (size_t e = 0; e < n; ++e) { a[e] = (float)A[e]; b[e] = (float)B[e]; }
+        TH_HIP(hipMemcpy(h->d_pA, a.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_pB, b.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    const size_t nacc = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
+    if ((rc = th_launch_zero(h, h->d_acc, nacc))) return rc;
+    if ((rc = th_launch_contract(h, h->d_pA, h->d_pB, np, prec, 0))) return rc;
+    TH_HIP(hipMemcpyAsync(acc_out, h->d_acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, h->stream));
+    TH_HIP(hipStreamSynchronize(h->stream));
+    return TH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,259 @@
+// th_gemm.hip -- the chi0 contraction on the MFMA units of gfx950.
+//
+// eintshechi's six-deep loop (eintshechi.F90:83-105) accumulates, for every
+// atom pair (i,j), sum_p w_p Gq_p(i,j)[ab] G_p(j,i)[cd] over the (k,E)
+// pairs p.  With the atom blocks flattened to vectors of length
+// M = (2 nOrb)^2 this is one complex GEMM per atom pair,
+//     T(i,j) = sum_p A_p B_p^T,   A_p = vec(w_p Gq_p(i,j)), B_p = vec(G_p(j,i)),
+// with K = number of pairs.  The second term of the reference,
+// conj(G(i,j) Gq(j,i)), is T^dagger in the chi index space (applied in the
+// finalize kernel), so it costs no extra MFMA work.  The real-axis integral
+// (eintshechi.F90:143-165) is the same GEMM with A = -i w (Gq - Gq^dagger),
+// B = G^dagger, accumulated separately because it is not symmetrised.
+//
+// Kernel layout: a workgroup owns one 112 x 112 output tile of one atom pair
+// for one K-split.  Seven waves each hold a 16 x 112 strip as 7 MFMA blocks
+// (re and im accumulators, 4 real MFMAs per complex block step).  Panels
+// stream through LDS in stages of TH_KC pairs, double buffered with a
+// register prefetch.  Partial tiles go to a split buffer and are summed in
+// a fixed order (bitwise reproducible).
+#include "th_internal.h"
+
+namespace {
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+template <typename P> struct mfma_t;
+// Output layout of the 16x16 block: lane l, register q holds row
+// row(l >> 4, q) and column l & 15 (measured on gfx950 with thx_contract).
+template <> struct mfma_t<double> {
+    typedef dv4 acc;
+    static __device__ inline acc mac(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int g, int q) { return g + 4 * q; }
+};
+template <> struct mfma_t<float> {
+    typedef fv4 acc;
+    static __device__ inline acc mac(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int g, int q) { return 4 * g + q; }
+};
+
+constexpr int NTH = TH_GEMM_WAVES * 64;   // 448 threads
+
+// 16-byte vector of panel elements
+template <typename P> struct vec16;
+template <> struct vec16<double> { typedef double2 type; static constexpr int n = 2; };
+template <> struct vec16<float> { typedef float4 type; static constexpr int n = 4; };
+
+template <typename P>
+__global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, const P *__restrict__ B,
+                                                       long np, int Mpad, int nA2, int ntile, int splits,
+                                                       long kper, double *__restrict__ part)
+{
+    typedef typename vec16<P>::type V;
+    constexpr int VN = vec16<P>::n;
+    constexpr int ROWV = TH_TILE / VN;                 // vectors per 112-wide row
+    constexpr int STAGE_V = TH_KC * 2 * ROWV;          // vectors per matrix per stage
+    constexpr int PER_T = (STAGE_V + NTH - 1) / NTH;   // vectors per thread per matrix
+    __shared__ __align__(16) P As[2][TH_KC][2][TH_TILE];
+    __shared__ __align__(16) P Bs[2][TH_KC][2][TH_TILE];
+
+    // XCD-aware order: consecutive logical ids (the tiles of one split) land
+    // on the same XCD, whose L2 then serves their shared panel rows.
+    const int nwg = gridDim.x;
+    const int b = blockIdx.x;
+    int lid = b;
+    if ((nwg & 7) == 0) lid = (b & 7) * (nwg >> 3) + (b >> 3);
+    const int tile = lid % (ntile * ntile);
+    const int rest = lid / (ntile * ntile);
+    const int split = rest % splits;
+    const int ij = rest / splits;
+    const int tm = tile / ntile, tn = tile - tm * ntile;
+    const int r0 = tm * TH_TILE, c0 = tn * TH_TILE;
+    const long kb = (long)split * kper;
+    const long ke = min(np, kb + kper);
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const size_t pstride = (size_t)nA2 * 2 * Mpad;       // elements per pair in a panel
+    const P *Ab = A + (size_t)ij * 2 * Mpad + r0;
+    const P *Bb = B + (size_t)ij * 2 * Mpad + c0;
+
+    typename mfma_t<P>::acc accr[7], acci[7];
+    for (int j = 0; j < 7; ++j)
+        for (int q = 0; q < 4; ++q) { accr[j][q] = 0; acci[j][q] = 0; }
+
+    V ra[PER_T], rb[PER_T];
+    auto gload = [&](long k0) {
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            int v = tid + u * NTH;
+            V za, zb;
+            for (int q = 0; q < VN; ++q) { reinterpret_cast<P *>(&za)[q] = 0; reinterpret_cast<P *>(&zb)[q] = 0; }
+            if (v < STAGE_V) {
+                int kk = v / (2 * ROWV), rem = v - kk * 2 * ROWV, comp = rem / ROWV, col = rem - comp * ROWV;
+                long kp = k0 + kk;
+                if (kp < ke) {
+                    za = *reinterpret_cast<const V *>(Ab + kp * pstride + (size_t)comp * Mpad + col * VN);
+                    zb = *reinterpret_cast<const V *>(Bb + kp * pstride + (size_t)comp * Mpad + col * VN);
+                }
+            }
+            ra[u] = za;
+            rb[u] = zb;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            int v = tid + u * NTH;
+            if (v < STAGE_V) {
+                int kk = v / (2 * ROWV), rem = v - kk * 2 * ROWV, comp = rem / ROWV, col = rem - comp * ROWV;
+                *reinterpret_cast<V *>(&As[buf][kk][comp][col * VN]) = ra[u];
+                *reinterpret_cast<V *>(&Bs[buf][kk][comp][col * VN]) = rb[u];
+            }
+        }
+    };
+
+    const long nstage = (ke > kb) ? (ke - kb + TH_KC - 1) / TH_KC : 0;
+    if (nstage > 0) gload(kb);
+    for (long s = 0; s < nstage; ++s) {
+        const int buf = (int)(s & 1);
+        sstore(buf);
+        __syncthreads();
+        if (s + 1 < nstage) gload(kb + (s + 1) * TH_KC);
+#pragma unroll
+        for (int kk = 0; kk < TH_KC; kk += 4) {
+            const int pr = kk + (lane >> 4);
+            const int m = wave * 16 + (lane & 15);
+            const P ar = As[buf][pr][0][m], ai = As[buf][pr][1][m];
+            const P nai = -ai;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int n = j * 16 + (lane & 15);
+                const P br = Bs[buf][pr][0][n], bi = Bs[buf][pr][1][n];
+                accr[j] = mfma_t<P>::mac(ar, br, accr[j]);
+                accr[j] = mfma_t<P>::mac(nai, bi, accr[j]);
+                acci[j] = mfma_t<P>::mac(ar, bi, acci[j]);
+                acci[j] = mfma_t<P>::mac(ai, br, acci[j]);
+            }
+        }
+    }
+    // partial tile -> part[split][ij][comp][Mpad][Mpad]
+    double *pr_ = part + (((size_t)split * nA2 + ij) * 2 + 0) * Mpad * Mpad;
+    double *pi_ = pr_ + (size_t)Mpad * Mpad;
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = r0 + wave * 16 + mfma_t<P>::row(lane >> 4, q);
+            const int col = c0 + j * 16 + (lane & 15);
+            pr_[(size_t)row * Mpad + col] = (double)accr[j][q];
+            pi_[(size_t)row * Mpad + col] = (double)acci[j][q];
+        }
+}
+
+// acc += sum_s part[s], elementwise, fixed order
+__global__ void reduce_kernel(const double *__restrict__ part, double *__restrict__ acc, int splits, size_t n)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < splits; ++k) s += part[(size_t)k * n + e];
+    acc[e] += s;
+}
+
+__global__ void zero_kernel(double *p, size_t n)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = 0.0;
+}
+
+// chi0[X,Y] with X = (sigma,i,mu,nu), Y = (sigma',j,gamma,xi) (column-major D x D)
+__global__ void finalize_kernel(const double *__restrict__ acc, double2 *__restrict__ chi0, int nA, int nOrb,
+                                int Mpad, int D)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * D) return;
+    const int X = (int)(e % D), Y = (int)(e / D);
+    const int bs = 2 * nOrb, nA2 = nA * nA;
+    int s1 = X & 3, r = X >> 2;
+    const int nu = r % nOrb; r /= nOrb;
+    const int mu = r % nOrb;
+    const int i = r / nOrb;
+    int s2 = Y & 3;
+    r = Y >> 2;
+    const int xi = r % nOrb; r /= nOrb;
+    const int ga = r % nOrb;
+    const int j = r / nOrb;
+    // spin labels: a = (dn,up,dn,up), c = (up,up,dn,dn) for sigma = (+,up,dn,-)
+    const int a1 = (s1 == 0 || s1 == 2) ? 1 : 0, c1 = (s1 >= 2) ? 1 : 0;
+    const int a2 = (s2 == 0 || s2 == 2) ? 1 : 0, c2 = (s2 >= 2) ? 1 : 0;
+    const size_t MM = (size_t)Mpad * Mpad;
+    const double *AI = acc;                              // imaginary axis [nA2][2][Mpad][Mpad]
+    const double *AR = acc + (size_t)nA2 * 2 * MM;       // real axis
+    // term 1: T(i,j)[r][c]
+    const int rr = (a1 * nOrb + nu) * bs + (a2 * nOrb + ga);
+    const int cc = (c2 * nOrb + xi) * bs + (c1 * nOrb + mu);
+    const size_t b1 = (size_t)(i * nA + j) * 2 * MM + (size_t)rr * Mpad + cc;
+    // term 2: conj(T(j,i)[r'][c'])
+    const int rr2 = (a2 * nOrb + ga) * bs + (a1 * nOrb + nu);
+    const int cc2 = (c1 * nOrb + mu) * bs + (c2 * nOrb + xi);
+    const size_t b2 = (size_t)(j * nA + i) * 2 * MM + (size_t)rr2 * Mpad + cc2;
+    double re = AI[b1] + AI[b2] + AR[b1];
+    double im = AI[b1 + MM] - AI[b2 + MM] + AR[b1 + MM];
+    const double inv2pi = 0.15915494309189533577;   // 1/(2 pi)
+    chi0[e] = make_double2(re * inv2pi, im * inv2pi);
+    (void)nA2;
+}
+
+template <typename P>
+int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int axis)
+{
+    const int ntile = h->Mpad / TH_TILE;
+    const int splits = h->splits;
+    long kper = (np + splits - 1) / splits;
+    kper = ((kper + TH_KC - 1) / TH_KC) * TH_KC;
+    const int nwg = ntile * ntile * splits * h->nA2;
+    hipLaunchKernelGGL((contract_kernel<P>), dim3(nwg), dim3(NTH), 0, h->stream, (const P *)pA, (const P *)pB,
+                       np, h->Mpad, h->nA2, ntile, splits, kper, h->d_part);
+    TH_HIP(hipGetLastError());
+    const size_t n = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
+    double *acc = h->d_acc + (size_t)axis * n;
+    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                       (const double *)h->d_part, acc, splits, n);
+    TH_HIP(hipGetLastError());
+    h->n_launch[1] += 1;
+    h->n_launch[2] += 1;
+    return TH_OK;
+}
+
+}  // namespace
+
+int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis)
+{
+    if (np <= 0) return TH_OK;
+    if (prec == TH_PREC_FP64) return launch_contract_t<double>(h, pA, pB, np, axis);
+    return launch_contract_t<float>(h, pA, pB, np, axis);
+}
+
+int th_launch_zero(th_handle *h, double *p, size_t n)
+{
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, p, n);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+int th_launch_finalize(th_handle *h, double2 *chi0)
+{
+    const size_t n = (size_t)h->D * h->D;
+    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                       (const double *)h->d_acc, chi0, h->nA, h->nOrb, h->Mpad, h->D);
+    TH_HIP(hipGetLastError());
+    h->n_launch[2] += 1;
+    return TH_OK;
+}

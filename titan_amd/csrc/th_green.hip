@@ -1,0 +1,263 @@
+// th_green.hip -- batched Green functions G(k,z) = (z - H(k))^-1 for gfx950.
+//
+// One workgroup per (k,E) pair.  The workgroup assembles H(k) from h0 and the
+// hop table in LDS (mod_hamiltonian.F90:35-87 and 183-243), inverts (z - H)
+// in place by Gauss-Jordan elimination with partial (row) pivoting
+// (the result of LAPACK zgetrf+zgetri as called by invers,
+// mod_tools.F90:796-829), and scatters the atom blocks the contraction needs
+// into a GEMM panel (the gf(:,:,i,j) scatter of mod_greenfunction.F90:55-62,
+// plus the weight and the real-axis combinations of eintshechi.F90:143-165).
+//
+// Mixed precision: the elimination runs in FP32 on a copy of (z - H), then
+// one refinement step G <- G + G (I - (z - H) G) with the residual formed
+// against the FP64 matrix.
+#include "th_internal.h"
+
+namespace {
+
+template <typename R> struct cplx { R x, y; };
+
+template <typename R> __device__ inline cplx<R> cmul(cplx<R> a, cplx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R> __device__ inline cplx<R> csub(cplx<R> a, cplx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R> __device__ inline cplx<R> cinv(cplx<R> a)
+{
+    R n = a.x * a.x + a.y * a.y;
+    return {a.x / n, -a.y / n};
+}
+template <typename R> __device__ inline R cabs1(cplx<R> a) { return fabs(a.x) + fabs(a.y); }
+
+struct green_params {
+    int mode, prec;
+    int nA, nOrb, bs, d, nhop, Mpad, nA2;
+    long p0, nk_re;
+    double q0, q1, q2, omega, Ef, eta, etap;
+    const double *R;
+    const int *jb, *ib;
+    const double2 *t, *h0;
+    const double *kim, *yim, *wim, *kre, *wre, *x2, *p2;
+    void *panel;
+    int *status;
+};
+
+// In-place Gauss-Jordan inverse of the d x d row-major matrix M in LDS.
+// Row interchanges are recorded; the final column permutation is returned
+// in idx (G[r][c] = M[r][idx[c]]).  Returns through *status the 1-based
+// step of the first exactly-zero pivot (0 if none).
+template <typename R, int NT>
+__device__ void gj_invert(cplx<R> *M, int d, cplx<R> *colk, cplx<R> *prow, int *perm, int *idx,
+                          int *s_piv, cplx<R> *s_inv, int *status)
+{
+    const int tid = threadIdx.x;
+    for (int k = 0; k < d; ++k) {
+        // pivot: largest |re|+|im| in column k at or below the diagonal,
+        // first index on ties (izamax rule); rows < 64 live in wave 0
+        if (tid < 64) {
+            R v = (tid >= k && tid < d) ? cabs1(M[tid * d + k]) : R(-1);
+            int ix = tid;
+            for (int off = 32; off > 0; off >>= 1) {
+                R ov = __shfl_xor(v, off);
+                int oi = __shfl_xor(ix, off);
+                if (ov > v || (ov == v && oi < ix)) { v = ov; ix = oi; }
+            }
+            if (tid == 0) { *s_piv = ix; perm[k] = ix; }
+        }
+        __syncthreads();
+        const int p = *s_piv;
+        cplx<R> rk, rp, ck, piv;
+        if (tid < d) {
+            rk = M[k * d + tid];
+            rp = M[p * d + tid];
+            ck = (tid == k) ? M[p * d + k] : (tid == p ? M[k * d + k] : M[tid * d + k]);
+            piv = M[p * d + k];
+        }
+        __syncthreads();
+        if (tid < d) {
+            if (piv.x == R(0) && piv.y == R(0)) {
+                if (tid == 0) atomicMin(status, k + 1);
+                piv.x = R(1);
+            }
+            cplx<R> inv = cinv(piv);
+            if (p != k) M[p * d + tid] = rk;
+            cplx<R> pr = (tid == k) ? inv : cmul(rp, inv);
+            M[k * d + tid] = pr;
+            prow[tid] = pr;
+            colk[tid] = ck;
+            if (tid == 0) *s_inv = inv;
+        }
+        __syncthreads();
+        const cplx<R> inv = *s_inv;
+        for (int e = tid; e < d * d; e += NT) {
+            int r = e / d, c = e - r * d;
+            if (r == k) continue;
+            cplx<R> f = colk[r];
+            if (c == k) {
+                cplx<R> v = cmul(f, inv);
+                M[e] = {-v.x, -v.y};
+            } else {
+                M[e] = csub(M[e], cmul(f, prow[c]));
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int c = 0; c < d; ++c) idx[c] = c;
+        for (int k = d - 1; k >= 0; --k) {
+            int pk = perm[k];
+            if (pk != k) { int tmp = idx[k]; idx[k] = idx[pk]; idx[pk] = tmp; }
+        }
+    }
+    __syncthreads();
+}
+
+template <typename P, int NT>
+__global__ __launch_bounds__(NT) void green_kernel(green_params g)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int d = g.d, tid = threadIdx.x;
+    const long pl = blockIdx.x;              // pair within the batch
+    const long p = g.p0 + pl;                // global pair index
+    // ---- pair -> (k, z, weight) ----
+    double k0, k1, k2, er, ei, w = 1.0;
+    if (g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B) {
+        k0 = g.kim[3 * p]; k1 = g.kim[3 * p + 1]; k2 = g.kim[3 * p + 2];
+        if (g.mode == TH_G_IMAG_A) {
+            k0 += g.q0; k1 += g.q1; k2 += g.q2;
+            er = g.Ef + g.omega; ei = g.yim[p] + g.eta; w = g.wim[p];
+        } else {
+            er = g.Ef; ei = g.yim[p] + g.etap;
+        }
+    } else {
+        long ie = p / g.nk_re, ik = p - ie * g.nk_re;
+        k0 = g.kre[3 * ik]; k1 = g.kre[3 * ik + 1]; k2 = g.kre[3 * ik + 2];
+        if (g.mode == TH_G_REAL_A) {
+            k0 += g.q0; k1 += g.q1; k2 += g.q2;
+            er = g.x2[ie] + g.omega; ei = g.eta; w = g.p2[ie] * g.wre[ik];
+        } else {
+            er = g.x2[ie]; ei = g.etap;
+        }
+    }
+    // ---- LDS layout ----
+    cplx<double> *M = reinterpret_cast<cplx<double> *>(smem);          // d*d, FP64 (z - H)
+    size_t off = sizeof(cplx<double>) * d * d;
+    cplx<double> *ph = reinterpret_cast<cplx<double> *>(smem + off);    // nhop phases
+    off += sizeof(cplx<double>) * g.nhop;
+    cplx<double> *colk = reinterpret_cast<cplx<double> *>(smem + off);
+    off += sizeof(cplx<double>) * d;
+    cplx<double> *prow = reinterpret_cast<cplx<double> *>(smem + off);
+    off += sizeof(cplx<double>) * d;
+    int *perm = reinterpret_cast<int *>(smem + off);
+    off += sizeof(int) * d;
+    int *idx = reinterpret_cast<int *>(smem + off);
+    off += sizeof(int) * d;
+    int *s_piv = reinterpret_cast<int *>(smem + off);
+    off += 16;
+    cplx<double> *s_inv = reinterpret_cast<cplx<double> *>(smem + off);
+    off += 16;
+
+    // ---- z - H(k) ----
+    for (int e = tid; e < d * d; e += NT) {
+        int r = e / d, c = e - r * d;
+        double2 hv = g.h0[r + (size_t)c * d];
+        M[e] = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
+    }
+    for (int n = tid; n < g.nhop; n += NT) {
+        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
+        double s, c;
+        sincos(a, &s, &c);
+        ph[n] = {c, s};
+    }
+    __syncthreads();
+    {
+        // element (nu, mu) of every hop block is always handled by thread
+        // e % NT, so hops that land on the same entry never race
+        const int nO = g.nOrb, nO2 = nO * nO;
+        for (int e = tid; e < nO2; e += NT) {
+            const int nu = e % nO, mu = e / nO;   // t(nu, mu), rows = orbitals of jb
+            for (int n = 0; n < g.nhop; ++n) {
+                double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                cplx<double> v = cmul(cplx<double>{tv.x, tv.y}, ph[n]);
+                int j = g.jb[n], i = g.ib[n];
+                for (int s = 0; s < 2; ++s) {
+                    int row = (2 * j + s) * nO + nu, col = (2 * i + s) * nO + mu;
+                    M[row * d + col] = csub(M[row * d + col], v);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- inverse ----
+    gj_invert<double, NT>(M, d, colk, prow, perm, idx, s_piv, s_inv, g.status);
+    // ---- panel ----
+    const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
+    P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+    for (int ij = 0; ij < g.nA2; ++ij) {
+        const int i = ij / nA, j = ij - (ij / nA) * nA;
+        P *o = out + (size_t)ij * 2 * Mpad;
+        for (int e = tid; e < Mpad; e += NT) {
+            double vr = 0.0, vi = 0.0;
+            if (e < Mb) {
+                int rr = e / bs, cc = e - rr * bs;
+                if (g.mode == TH_G_IMAG_A) {          // w * Gq(i,j)
+                    cplx<double> v = M[(i * bs + rr) * d + idx[j * bs + cc]];
+                    vr = w * v.x; vi = w * v.y;
+                } else if (g.mode == TH_G_IMAG_B) {   // G(j,i)
+                    cplx<double> v = M[(j * bs + rr) * d + idx[i * bs + cc]];
+                    vr = v.x; vi = v.y;
+                } else if (g.mode == TH_G_REAL_A) {   // -i w (Gq(i,j) - Gq^dagger(i,j))
+                    cplx<double> a = M[(i * bs + rr) * d + idx[j * bs + cc]];
+                    cplx<double> b = M[(j * bs + cc) * d + idx[i * bs + rr]];
+                    double dr = a.x - b.x, di = a.y + b.y;      // a - conj(b)
+                    vr = w * di; vi = -w * dr;                  // -i w (dr + i di)
+                } else {                                        // G^dagger(j,i) = conj(G(i,j))^T
+                    cplx<double> v = M[(i * bs + cc) * d + idx[j * bs + rr]];
+                    vr = v.x; vi = -v.y;
+                }
+            }
+            o[e] = (P)vr;
+            o[Mpad + e] = (P)vi;
+        }
+    }
+}
+
+template <typename P, int NT>
+int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    size_t lds = sizeof(double2) * (size_t)h->d * h->d + sizeof(double2) * h->nhop +
+                 2 * sizeof(double2) * h->d + 2 * sizeof(int) * h->d + 32;
+    hipLaunchKernelGGL((green_kernel<P, NT>), dim3((unsigned)a.np), dim3(NT), lds, h->stream, g);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+}  // namespace
+
+int *th_status_word(th_handle *h);
+
+int th_launch_green(th_handle *h, const th_green_args &a)
+{
+    if (h->d > TH_MAX_D_LDS) {
+        th_set_error("d = 2*nOrb*nA larger than the LDS-resident inversion supports");
+        return TH_E_UNSUPPORTED;
+    }
+    green_params g;
+    g.mode = a.mode; g.prec = a.prec;
+    g.nA = h->nA; g.nOrb = h->nOrb; g.bs = h->bs; g.d = h->d; g.nhop = h->nhop;
+    g.Mpad = h->Mpad; g.nA2 = h->nA2; g.p0 = a.p0; g.nk_re = h->nk_re;
+    g.q0 = a.q[0]; g.q1 = a.q[1]; g.q2 = a.q[2];
+    g.omega = a.omega; g.Ef = a.Ef; g.eta = a.eta; g.etap = a.etap;
+    g.R = h->d_R; g.jb = h->d_jb; g.ib = h->d_ib; g.t = h->d_t; g.h0 = h->d_h0;
+    g.kim = h->d_kim; g.yim = h->d_yim; g.wim = h->d_wim;
+    g.kre = h->d_kre; g.wre = h->d_wre; g.x2 = h->d_x2; g.p2 = h->d_p2;
+    g.panel = a.panel;
+    g.status = th_status_word(h);
+    h->n_launch[0] += 1;
+    if (a.prec == TH_PREC_FP64) {
+        if (h->d <= 24) return launch_green_t<double, 64>(h, a, g);
+        return launch_green_t<double, 256>(h, a, g);
+    }
+    if (h->d <= 24) return launch_green_t<float, 64>(h, a, g);
+    return launch_green_t<float, 256>(h, a, g);
+}

@@ -1,0 +1,102 @@
+// th_internal.h -- shared definitions of the gfx950 kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/titan_hip.h"
+
+// Panel geometry: the G*G contraction runs one complex GEMM per atom pair
+// (i,j) whose rows and columns are the flattened (2 nOrb) x (2 nOrb) atom
+// blocks, M = (2 nOrb)^2 (324 for s,p,d), padded to MPAD, a multiple of the
+// 112-wide output tile.
+constexpr int TH_TILE = 112;        // output tile edge (7 MFMA blocks of 16)
+constexpr int TH_KC = 8;            // pairs per LDS stage of the GEMM
+constexpr int TH_GEMM_WAVES = 7;    // one wave per 16-row block of the tile
+constexpr int TH_MAX_NORB = 9;
+constexpr int TH_MAX_D_LDS = 64;    // largest d inverted with the LDS-resident kernel
+constexpr int TH_MAX_NA = 32;
+
+inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
+
+struct th_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nA = 0, nOrb = 0, bs = 0, d = 0, D = 0, nhop = 0, M = 0, Mpad = 0, nA2 = 0;
+    // model
+    double *d_R = nullptr;       // 3*nhop
+    int *d_jb = nullptr, *d_ib = nullptr;
+    double2 *d_t = nullptr;      // nhop * nOrb*nOrb (column-major blocks)
+    double2 *d_h0 = nullptr;     // d*d column-major
+    // work lists
+    long n_im = 0, nk_re = 0;
+    double *d_kim = nullptr, *d_yim = nullptr, *d_wim = nullptr;
+    double *d_kre = nullptr, *d_wre = nullptr;
+    double *d_x2 = nullptr, *d_p2 = nullptr;  // real-axis nodes of the current omega
+    int nx_cap = 0;
+    // work space
+    long batch = 0;              // pairs per batch
+    void *d_pA = nullptr, *d_pB = nullptr;    // panels [batch][nA2][2][Mpad]
+    size_t panel_bytes = 0;
+    int splits = 0;
+    double *d_part = nullptr;    // [splits][nA2][2][Mpad][Mpad]
+    double *d_acc = nullptr;     // [2 axes][nA2][2][Mpad][Mpad]
+    double2 *d_chi0 = nullptr;   // D*D
+    double2 *d_chi = nullptr;    // D*D
+    // omega-independent imaginary-axis B panels (G(k, E_F + i(y+etap)))
+    void *d_bcache = nullptr;
+    int bcache_prec = -1;
+    double bcache_key[2] = {0, 0};   // (Ef, etap)
+    // Dyson work space
+    int *d_sup = nullptr;        // support of U
+    double2 *d_U = nullptr, *d_W = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_X = nullptr;
+    int sup_cap = 0;
+    // timing
+    hipEvent_t ev[8] = {};
+    double t_ms[5] = {0, 0, 0, 0, 0};
+    long n_launch[4] = {0, 0, 0, 0};
+};
+
+// error plumbing
+void th_set_error(const std::string &msg);
+#define TH_HIP(call)                                                               \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            th_set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+            return TH_E_HIP;                                                       \
+        }                                                                          \
+    } while (0)
+
+// ---- kernel launchers (th_green.hip) ----
+// Modes of the green-function kernel: which matrix, and how it is written
+// into the GEMM panel.
+enum th_green_mode {
+    TH_G_IMAG_A = 0,   // A = w * Gq,                 Gq = G(k+q, Ef+omega + i(y+eta))
+    TH_G_IMAG_B = 1,   // B = G(j,i) blocks,          G  = G(k, Ef + i(y+etap))
+    TH_G_REAL_A = 2,   // A = -i w (Gq - Gq^dagger),  Gq = G(k+q, x+omega + i eta)
+    TH_G_REAL_B = 3,   // B = G^dagger(j,i) blocks,   G  = G(k, x + i etap)
+};
+
+struct th_green_args {
+    int mode;
+    long p0, np;          // pair range of this batch (global pair index p0 .. p0+np-1)
+    double q[3];
+    double omega, Ef, eta, etap;
+    int prec;             // TH_PREC_*
+    void *panel;          // output panel, [np][nA2][2][Mpad] of double or float
+};
+int th_launch_green(th_handle *h, const th_green_args &a);
+
+// ---- contraction (th_gemm.hip) ----
+// acc[axis] += sum over the batch of A (x) B, one complex GEMM per atom pair.
+int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis);
+int th_launch_zero(th_handle *h, double *p, size_t n);
+// chi0 = (T_imag + T_imag^dagger + T_real) / 2pi in the chi basis.
+int th_launch_finalize(th_handle *h, double2 *chi0);
+
+// ---- Dyson (th_dyson.hip) ----
+int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,
+                    double2 *chi, double *cond);
+int th_launch_trace(th_handle *h, const double2 *chi, double2 *schi_dev);

@@ -1,0 +1,299 @@
+"""Host-side set-up of the tight-binding model (numpy), mirroring the
+reference's Fortran set-up modules so that the C ABI receives exactly the
+arrays TITAN hands to calchk / hamilt_local:
+
+  neighbour shells      Lattice.F90:40-232 (generateNeighbors, sortNeighbors)
+  two-centre hoppings   TightBinding.F90:93-219 (get_SK_parameter,
+                        set_hopping_matrix), 655-783 (intd)
+  spin-orbit term       mod_SOC.F90:13-74 (updateLS at theta = phi = 0),
+                        L matrices of mod_magnet.F90:295-358
+  mean-field exchange   mod_U_matrix.F90:29-55 (update_Umatrix)
+  interaction matrix    mod_susceptibilities.F90:106-159
+  index maps            mod_System.F90:101-223 ('ia', 'sigmaimunu2i')
+
+The system is a plain dict (same keys the reference reads from its input,
+element, basis and self-consistency files; energies in Ry):
+  sysdim, a (3 lattice vectors), pos (basis positions), type (per atom),
+  orbs (selected orbital ids 1..9 = s,px,py,pz,dxy,dyz,dzx,dx2,dz2),
+  nStages, relTol, types[{onsite[9], sk[nStages][10], lambda[2], U[2]}],
+  rho/mx/my/mz [nA][nOrb], optional rho0, rhod0, mzd0, mpd0, socscale, Ef.
+SK integral order: sss, pps, ppp, dds, ddp, ddd, sps, sds, pds, pdp.
+"""
+import numpy as np
+
+R3 = np.sqrt(3.0)
+SK_NAMES = ("sss", "pps", "ppp", "dds", "ddp", "ddd", "sps", "sds", "pds", "pdp")
+_ANG = np.array([0, 1, 1, 1, 2, 2, 2, 2, 2])
+
+
+def sk_table(p, w):
+    """9x9 Slater-Koster two-centre table b[a][c] for direction cosines w
+    (Slater & Koster 1954, Table I; orbital order and parity convention of
+    TightBinding.F90:655-783).  Linear in the ten integrals p."""
+    sss, pps, ppp, dds, ddp, ddd, sps, sds, pds, pdp = p
+    l, m, n = w
+    l2, m2, n2 = l * l, m * m, n * n
+    f2 = l2 - m2
+    qz = n2 - 0.5 * (l2 + m2)
+    E = np.zeros((9, 9))
+    E[0, 0] = sss
+    E[0, 1:4] = np.array([l, m, n]) * sps
+    E[0, 4:9] = [R3 * l * m * sds, R3 * m * n * sds, R3 * n * l * sds, 0.5 * R3 * f2 * sds, qz * sds]
+    E[1, 1] = l2 * pps + (1 - l2) * ppp
+    E[2, 2] = m2 * pps + (1 - m2) * ppp
+    E[3, 3] = n2 * pps + (1 - n2) * ppp
+    E[1, 2] = l * m * (pps - ppp)
+    E[1, 3] = l * n * (pps - ppp)
+    E[2, 3] = m * n * (pps - ppp)
+    lmn = l * m * n
+    E[1, 4:9] = [R3 * l2 * m * pds + m * (1 - 2 * l2) * pdp, R3 * lmn * pds - 2 * lmn * pdp,
+                 R3 * l2 * n * pds + n * (1 - 2 * l2) * pdp, 0.5 * R3 * l * f2 * pds + l * (1 - f2) * pdp,
+                 l * qz * pds - R3 * l * n2 * pdp]
+    E[2, 4:9] = [R3 * m2 * l * pds + l * (1 - 2 * m2) * pdp, R3 * m2 * n * pds + n * (1 - 2 * m2) * pdp,
+                 R3 * lmn * pds - 2 * lmn * pdp, 0.5 * R3 * m * f2 * pds - m * (1 + f2) * pdp,
+                 m * qz * pds - R3 * m * n2 * pdp]
+    E[3, 4:9] = [R3 * lmn * pds - 2 * lmn * pdp, R3 * n2 * m * pds + m * (1 - 2 * n2) * pdp,
+                 R3 * n2 * l * pds + l * (1 - 2 * n2) * pdp, 0.5 * R3 * n * f2 * pds - n * f2 * pdp,
+                 n * qz * pds + R3 * n * (l2 + m2) * pdp]
+    E[4, 4] = 3 * l2 * m2 * dds + (l2 + m2 - 4 * l2 * m2) * ddp + (n2 + l2 * m2) * ddd
+    E[5, 5] = 3 * m2 * n2 * dds + (m2 + n2 - 4 * m2 * n2) * ddp + (l2 + m2 * n2) * ddd
+    E[6, 6] = 3 * n2 * l2 * dds + (n2 + l2 - 4 * n2 * l2) * ddp + (m2 + n2 * l2) * ddd
+    E[4, 5] = 3 * l * m2 * n * dds + l * n * (1 - 4 * m2) * ddp + l * n * (m2 - 1) * ddd
+    E[4, 6] = 3 * l2 * m * n * dds + m * n * (1 - 4 * l2) * ddp + m * n * (l2 - 1) * ddd
+    E[5, 6] = 3 * m * n2 * l * dds + m * l * (1 - 4 * n2) * ddp + m * l * (n2 - 1) * ddd
+    E[4, 7] = 1.5 * l * m * f2 * dds - 2 * l * m * f2 * ddp + 0.5 * l * m * f2 * ddd
+    E[5, 7] = 1.5 * m * n * f2 * dds - m * n * (1 + 2 * f2) * ddp + m * n * (1 + 0.5 * f2) * ddd
+    E[6, 7] = 1.5 * n * l * f2 * dds + n * l * (1 - 2 * f2) * ddp - n * l * (1 - 0.5 * f2) * ddd
+    E[4, 8] = R3 * l * m * qz * dds - 2 * R3 * l * m * n2 * ddp + 0.5 * R3 * l * m * (1 + n2) * ddd
+    E[5, 8] = R3 * m * n * qz * dds + R3 * m * n * (l2 + m2 - n2) * ddp - 0.5 * R3 * m * n * (l2 + m2) * ddd
+    E[6, 8] = R3 * l * n * qz * dds + R3 * l * n * (l2 + m2 - n2) * ddp - 0.5 * R3 * l * n * (l2 + m2) * ddd
+    E[7, 7] = 0.75 * f2 * f2 * dds + (l2 + m2 - f2 * f2) * ddp + (n2 + 0.25 * f2 * f2) * ddd
+    E[7, 8] = 0.5 * R3 * f2 * qz * dds - R3 * n2 * f2 * ddp + 0.25 * R3 * (1 + n2) * f2 * ddd
+    E[8, 8] = qz * qz * dds + 3 * n2 * (l2 + m2) * ddp + 0.75 * (l2 + m2) ** 2 * ddd
+    sign = np.where((_ANG[:, None] + _ANG[None, :]) % 2 == 1, -1.0, 1.0)
+    iu = np.triu_indices(9, 1)
+    E[(iu[1], iu[0])] = (sign * E)[iu]
+    return E
+
+
+def l_matrices():
+    """L_x, L_y, L_z (9x9 complex, orbital ids 1..9 -> 0..8) of L = -i r x grad
+    on the real cubic harmonics, with the reference's phase conventions."""
+    Lz = np.zeros((9, 9), dtype=complex)
+    Lp = np.zeros((9, 9), dtype=complex)
+    Lz[1, 2], Lz[2, 1] = -1j, 1j
+    Lz[4, 7], Lz[7, 4] = 2j, -2j
+    Lz[5, 6], Lz[6, 5] = 1j, -1j
+    Lp[1, 3], Lp[2, 3], Lp[3, 1], Lp[3, 2] = -1, -1j, 1, 1j
+    Lp[4, 5], Lp[4, 6] = -1, -1j
+    Lp[5, 4], Lp[5, 7], Lp[5, 8] = 1, -1j, -R3 * 1j
+    Lp[6, 4], Lp[6, 7], Lp[6, 8] = 1j, 1, -R3
+    Lp[7, 5], Lp[7, 6] = 1j, -1
+    Lp[8, 5], Lp[8, 6] = R3 * 1j, R3
+    Lm = Lp.conj().T
+    return np.array([0.5 * (Lp + Lm), -0.5j * (Lp - Lm), Lz])
+
+
+def reciprocal(a, sysdim):
+    a = np.asarray(a, dtype=float)
+    b = np.zeros((3, 3))
+    if sysdim == 3:
+        vol = 2 * np.pi / np.dot(a[0], np.cross(a[1], a[2]))
+        b[0] = vol * np.cross(a[1], a[2])
+        b[1] = vol * np.cross(a[2], a[0])
+        b[2] = vol * np.cross(a[0], a[1])
+    elif sysdim == 2:
+        z = np.cross(a[0], a[1])
+        z /= np.linalg.norm(z)
+        c1, c2 = np.cross(a[1], z), np.cross(z, a[0])
+        b[0] = 2 * np.pi * c1 / np.dot(a[0], c1)
+        b[1] = 2 * np.pi * c2 / np.dot(a[1], c2)
+    else:
+        b[0] = 2 * np.pi * a[0] / np.dot(a[0], a[0])
+    return b
+
+
+class TBModel:
+    """Hop table, h0 and interaction matrix of one cell."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.sysdim = int(spec["sysdim"])
+        self.a = np.asarray(spec["a"], dtype=float)
+        self.pos = np.asarray(spec["pos"], dtype=float).reshape(-1, 3)
+        self.nA = len(self.pos)
+        self.type = np.asarray(spec["type"], dtype=int)
+        self.orbs = np.asarray(spec["orbs"], dtype=int)
+        self.nOrb = len(self.orbs)
+        self.nStages = int(spec["nStages"])
+        self.relTol = float(spec.get("relTol", 0.05))
+        self.Ef = float(spec["Ef"])
+        self.d = 2 * self.nOrb * self.nA
+        self.D = 4 * self.nOrb ** 2 * self.nA
+        self.b = reciprocal(self.a, self.sysdim)
+        self._neighbours()
+        self.build()
+
+    # --- Lattice.F90 -------------------------------------------------------
+    def _neighbours(self):
+        nS, nA = self.nStages, self.nA
+        rng = np.arange(-nS, nS + 1)
+        grids = [rng if k < self.sysdim else np.zeros(1, dtype=int) for k in range(3)]
+        # cell order as the reference: first index fastest
+        cells = np.array([(i, j, k) for k in grids[2] for j in grids[1] for i in grids[0]], dtype=float)
+        cellv = cells @ self.a
+        sites_R = np.repeat(cellv, nA, axis=0)
+        sites_j = np.tile(np.arange(nA), len(cellv))
+        sites_pos = sites_R + self.pos[sites_j]
+        dist = np.linalg.norm(sites_pos[:, None, :] - self.pos[None, :, :], axis=2)   # [site, k]
+        stages = np.full((nS, nA), 1e12)
+        for k in range(nA):
+            dk = np.sort(dist[:, k][dist[:, k] > 1e-9])
+            stages[0, k] = dk[0]
+            l = 0
+            for v in dk[1:]:
+                if abs(v - stages[l, k]) < stages[0, k] * self.relTol:
+                    continue
+                l += 1
+                if l >= nS:
+                    break
+                stages[l, k] = v
+        hops = []
+        for s in range(len(sites_j)):
+            j = sites_j[s]
+            for i in range(nA):
+                dd = dist[s, i]
+                if dd <= 1e-9:
+                    continue
+                st = np.nonzero(np.abs(dd - stages[:, i]) < stages[0, i] * self.relTol)[0]
+                if len(st) == 0:
+                    continue
+                hops.append((sites_R[s], j, i, int(st[0]), (sites_pos[s] - self.pos[i]) / dd))
+        self._hops = hops
+        self.stages = stages
+
+    # --- TightBinding.F90, mod_SOC.F90, mod_U_matrix.F90 ---------------------
+    def build(self):
+        spec, nO, nA, d = self.spec, self.nOrb, self.nA, self.d
+        types = spec["types"]
+        o = self.orbs - 1
+        n = len(self._hops)
+        self.R = np.zeros((n, 3))
+        self.jb = np.zeros(n, dtype=np.int32)
+        self.ib = np.zeros(n, dtype=np.int32)
+        self.stage_of = np.zeros(n, dtype=np.int32)
+        self.dircos = np.zeros((n, 3))
+        self.t = np.zeros((n, nO, nO))
+        for h, (R, j, i, st, dc) in enumerate(self._hops):
+            t1 = np.asarray(types[self.type[i]]["sk"][st], dtype=float)
+            t2 = np.asarray(types[self.type[j]]["sk"][st], dtype=float)
+            g = np.sqrt(np.abs(t1) * np.abs(t2))
+            mix = np.where(t1 + t2 >= 0, g, -g)
+            b = sk_table(mix, dc)
+            self.R[h], self.jb[h], self.ib[h], self.stage_of[h], self.dircos[h] = R, j, i, st, dc
+            self.t[h] = b[np.ix_(o, o)]          # t[nu (orbital of j), mu (orbital of i)]
+        self.h0 = self.local_hamiltonian()
+        self.Umat = self.interaction_matrix()
+
+    def hidx(self, i, s, mu):
+        return (2 * i + s) * self.nOrb + mu
+
+    def cidx(self, sigma, i, mu, nu):
+        return 4 * ((i * self.nOrb + mu) * self.nOrb + nu) + sigma
+
+    def soc_block(self, lam_p, lam_d):
+        """(1/2) lambda L.sigma on one atom, (2 nOrb)^2, up block then down block."""
+        L = l_matrices()
+        o = self.orbs - 1
+        nO = self.nOrb
+        blk = np.zeros((2 * nO, 2 * nO), dtype=complex)
+        for a in range(nO):
+            for c in range(nO):
+                oa, oc = o[a], o[c]
+                if 1 <= oa <= 3 and 1 <= oc <= 3:
+                    lam = lam_p
+                elif oa >= 4 and oc >= 4:
+                    lam = lam_d
+                else:
+                    continue
+                lx, ly, lz = L[0][oa, oc], L[1][oa, oc], L[2][oa, oc]
+                blk[a, c] += 0.5 * lam * lz
+                blk[nO + a, nO + c] += -0.5 * lam * lz
+                blk[a, nO + c] += 0.5 * lam * (lx - 1j * ly)
+                blk[nO + a, c] += 0.5 * lam * (lx + 1j * ly)
+        return blk
+
+    def exchange_block(self, i, Un, Um):
+        """Mean-field term hee of atom i (mod_U_matrix.F90:43-52)."""
+        spec, nO = self.spec, self.nOrb
+        z = np.zeros(nO)
+
+        def arr(key, default):
+            return np.asarray(spec.get(key, default), dtype=float).reshape(self.nA, nO)[i]
+        rho, mx, my, mz = (arr(k, [z] * self.nA) for k in ("rho", "mx", "my", "mz"))
+        rho0 = arr("rho0", [z] * self.nA)
+        dm = self.orbs >= 5
+        rhod, mxd, myd, mzd = rho[dm].sum(), mx[dm].sum(), my[dm].sum(), mz[dm].sum()
+        rhod0 = float(np.asarray(spec.get("rhod0", np.zeros(self.nA)))[i])
+        mzd0 = float(np.asarray(spec.get("mzd0", np.zeros(self.nA)))[i])
+        mpd0 = complex(np.asarray(spec.get("mpd0", np.zeros(self.nA)), dtype=complex)[i])
+        mpd = mxd + 1j * myd
+        blk = np.zeros((2 * nO, 2 * nO), dtype=complex)
+        for mu in np.nonzero(dm)[0]:
+            dn = Un * (rho[mu] - rho0[mu]) + 0.5 * Un * (-rhod + rhod0)
+            blk[mu, mu] += dn + 0.5 * Um * (-mzd + mzd0)
+            blk[nO + mu, nO + mu] += dn + 0.5 * Um * (mzd - mzd0)
+            blk[mu, nO + mu] += -0.5 * Um * np.conj(mpd - mpd0)
+            blk[nO + mu, mu] += -0.5 * Um * (mpd - mpd0)
+        return blk
+
+    def local_hamiltonian(self):
+        spec, nO, d = self.spec, self.nOrb, self.d
+        types = spec["types"]
+        scale = float(spec.get("socscale", 1.0))
+        h0 = np.zeros((d, d), dtype=complex)
+        for i in range(self.nA):
+            ty = types[self.type[i]]
+            on = np.asarray(ty["onsite"], dtype=float)[self.orbs - 1]
+            blk = np.diag(np.concatenate([on, on])).astype(complex)
+            blk += self.soc_block(scale * ty["lambda"][0], scale * ty["lambda"][1])
+            blk += self.exchange_block(i, ty["U"][0], ty["U"][1])
+            s = slice(2 * nO * i, 2 * nO * (i + 1))
+            h0[s, s] += blk
+        return h0
+
+    def interaction_matrix(self):
+        D, nO = self.D, self.nOrb
+        U = np.zeros((D, D), dtype=complex)
+        dorb = np.nonzero(self.orbs >= 5)[0]
+        for i in range(self.nA):
+            Un, Um = self.spec["types"][self.type[i]]["U"]
+            for mu in dorb:
+                for ga in dorb:
+                    U[self.cidx(0, i, mu, mu), self.cidx(0, i, ga, ga)] = Um
+                    U[self.cidx(1, i, mu, mu), self.cidx(1, i, ga, ga)] = Un
+                    U[self.cidx(2, i, mu, mu), self.cidx(2, i, ga, ga)] = Un
+                    U[self.cidx(3, i, mu, mu), self.cidx(3, i, ga, ga)] = Um
+            for mu in dorb:
+                for nu in dorb:
+                    for s1 in (1, 2):
+                        for s2 in (1, 2):
+                            U[self.cidx(s1, i, mu, nu), self.cidx(s2, i, nu, mu)] -= Un
+        return U
+
+    def hk(self, k):
+        """H(k) = h0 + sum_n exp(i k.R_n) t_n (x) 1_spin (host reference use only)."""
+        H = self.h0.copy()
+        nO = self.nOrb
+        ph = np.exp(1j * (self.R @ np.asarray(k, dtype=float)))
+        for n in range(len(ph)):
+            j, i = self.jb[n], self.ib[n]
+            for s in range(2):
+                r0, c0 = (2 * j + s) * nO, (2 * i + s) * nO
+                H[r0:r0 + nO, c0:c0 + nO] += self.t[n] * ph[n]
+        return H
+
+    def handle(self):
+        """Uploads the model to the device (titan_amd.binding.Handle)."""
+        from titan_amd.binding import Handle
+        return Handle(self.nA, self.nOrb, self.R, self.jb, self.ib, self.t.astype(complex), self.h0)
