@@ -1,0 +1,173 @@
+"""Throughput of the chi(omega) path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
+
+A step is one frequency of the chi(omega) sweep: the full k-mesh times all
+imaginary- and real-axis energies ((k,E)-points), the G*G contraction into
+chi0, and the FP64 Dyson step.  The headline value is the mixed-precision
+path (FP32 G panels and FP32 MFMA contraction, FP64 contour sums and Dyson);
+the plain FP64 path runs the same steps for comparison, and the relative
+error of the mixed result against it is reported.  Inputs are seeded
+synthetic systems (titan_amd.configs), resident on the device before the
+timed region starts.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK = {"fp64": 78.6, "mixed": 157.3}     # TFLOP/s, MI355X dense MFMA FP64 / FP32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(spec, run, omega, target_s=12.0):
+    """The FP64 CPU oracle (the reference's eintshechi loop restated in C,
+    OpenMP over (k,E) pairs) on a bounded sample of the same workload:
+    a subset of k-points with every imaginary and real energy at one omega."""
+    from oracle import oracle as O
+    from titan_amd import contour as Ct
+    from titan_amd.system import TBModel
+    s = O.OracleSystem(spec)
+    m = TBModel(spec)
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    kp, wk = Ct.uniform_kmesh(m.b, *run["nk"])
+    x2, p2 = Ct.real_epoints(m.Ef, omega, run["parts3"], run["n3gl"])
+    threads = os.cpu_count() or 1
+    rng = np.random.default_rng(7)
+
+    def go(nks):
+        sel = rng.choice(len(wk), nks, replace=False)
+        ks, ws = kp[sel], wk[sel]
+        K = np.tile(ks, (len(y), 1))
+        t0 = time.perf_counter()
+        s.chi0(run["q"], omega, run["eta"], run["etap"], (K, np.repeat(y, nks), np.outer(wy, ws).ravel()),
+               (ks, ws), (x2, p2), nthreads=threads)
+        return time.perf_counter() - t0, nks * (len(y) + len(x2))
+
+    dt, pairs = go(2)
+    nks = int(max(2, min(len(wk), 2 * target_s / max(dt, 1e-3))))
+    dt, pairs = go(nks)
+    return {"value": pairs / dt, "unit": "(k,E)-points/s", "cores": threads, "kind": "port",
+            "sample": "%d random k-points x %d energies (%d imag + %d real) at omega=%.4g Ry, %.1f s" %
+                      (nks, len(y) + len(x2), len(y), len(x2), omega, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the chi(omega) path is built for a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from titan_amd import binding as B
+    from titan_amd import configs as CF
+    from titan_amd.chi import ChiWorkload
+
+    spec = CF.system(args.config)
+    run = CF.run_params(args.config)
+    om_all = np.asarray(run["omega"] if run["omega"] is not None else CF.omega_grid(args.config, spec), dtype=float)
+    need = args.warmup + args.steps
+    om_all = np.resize(om_all, max(need, len(om_all)))
+    om_warm, om_time = om_all[:args.warmup], om_all[args.warmup:args.warmup + args.steps]
+    log("config", args.config, spec["name"], "nk", run["nk"], "omega timed", om_time)
+    t0 = time.perf_counter()
+    wl = ChiWorkload(spec, run)
+    log("set-up %.1f s, n_imag %d, n_k %d" % (time.perf_counter() - t0, wl.nk * len(wl.y), wl.nk))
+    m = wl.model
+    P = wl.pairs(om_time)
+    res = {}
+    for name, prec in (("mixed", B.PREC_MIXED), ("fp64", B.PREC_FP64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        if args.warmup:
+            wl.sweep(om_warm, precision=prec)
+        t0 = time.perf_counter()
+        s0, s, cond, chi = wl.sweep(om_time, precision=prec, full=True)
+        dt = time.perf_counter() - t0
+        tm = wl.h.last_timing()
+        res[name] = {"dt": dt, "timing": tm, "schi": s, "schi0": s0, "chi": chi, "cond": cond}
+        log(name, "%.3f s, %.4g pairs/s" % (dt, P / dt), tm)
+    mx = res["mixed"]
+    out = {
+        "metric": "(k,E)-points/s for chi(omega) at <=1e-6 rel err vs FP64; mixed vs plain FP64",
+        "value": P / mx["dt"],
+        "unit": "(k,E)-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mx["dt"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 G panels + f32 MFMA contraction, f64 contour sums and Dyson",
+        "data": "synthetic (seeded Slater-Koster model, seed %d)" % spec.get("seed", 0),
+        "config": {"workload": "cfg%d: %s, H %dx%d, chi0 %dx%d, k-mesh %s, %d imag + %d real energies, "
+                               "one omega per step" % (args.config, spec["name"], m.d, m.d, m.D, m.D,
+                                                       "x".join(map(str, run["nk"])), len(wl.y), wl.n_x),
+                   "pairs_per_step": P // max(1, args.steps), "omega_timed_Ry": om_time.tolist()},
+    }
+    # dominant kernel: the MFMA contraction (algorithmic FLOPs of the reference's contraction per pair)
+    alg = wl.contraction_flops(om_time)
+    ctime = mx["timing"]["contract_ms"] * 1e-3
+    n_imag = wl.nk * len(wl.y) * len(om_time)
+    n_real = wl.nk * wl.n_x * int(np.sum(np.abs(om_time) >= 1e-15))
+    Mpad = ((4 * m.nOrb * m.nOrb + 111) // 112) * 112
+    executed = (n_imag + n_real) * m.nA ** 2 * 8.0 * Mpad * Mpad
+    out["roofline"] = {"bound": "mfma", "achieved": alg / ctime / 1e12, "peak": PEAK["mixed"], "unit": "TFLOP/s",
+                       "frac": alg / ctime / 1e12 / PEAK["mixed"], "traffic": None,
+                       "kernel": "contract_kernel<float> (FP32 MFMA 16x16x4)",
+                       "launches": mx["timing"]["launches"][1], "avg_launch_ms": 1e3 * ctime / max(1, mx["timing"]["launches"][1]),
+                       "executed_tflops": executed / ctime / 1e12,
+                       "executed_frac": executed / ctime / 1e12 / PEAK["mixed"]}
+    prof = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(prof):
+        try:
+            tr = json.load(open(prof)).get("cfg%d" % args.config)
+            if tr:
+                out["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        except Exception:
+            pass
+    out["algorithmic_tflops_per_s"] = wl.flops(om_time) / mx["dt"] / 1e12
+    out["stage_ms"] = {k: mx["timing"][k] for k in ("green_ms", "contract_ms", "finalize_ms", "dyson_ms", "total_ms")}
+    if "fp64" in res:
+        f = res["fp64"]
+        out["fp64_value"] = P / f["dt"]
+        out["mixed_speedup_vs_fp64"] = f["dt"] / mx["dt"]
+        out["fp64_contract_tflops_alg"] = alg / (f["timing"]["contract_ms"] * 1e-3) / 1e12
+        ref = f["schi"]
+        out["rel_err_schi_vs_fp64"] = float(np.linalg.norm(mx["schi"] - ref) / np.linalg.norm(ref))
+        out["rel_err_chi_last_omega_vs_fp64"] = float(np.linalg.norm(mx["chi"] - f["chi"]) / np.linalg.norm(f["chi"]))
+        out["rel_err_schi0_vs_fp64"] = float(np.linalg.norm(mx["schi0"] - f["schi0"]) / np.linalg.norm(f["schi0"]))
+        im_m = np.abs(mx["schi"][:, 0, 0].imag)
+        im_f = np.abs(ref[:, 0, 0].imag)
+        out["peak_bin_match"] = bool(np.argmax(im_m) == np.argmax(im_f))
+    out["dyson_cond_max"] = float(np.max(mx["cond"]))
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(spec, run, float(om_time[-1]))
+        except Exception as e:   # the baseline is reported, never required
+            log("cpu baseline failed:", e)
+            out["cpu_baseline"] = None
+    wl.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

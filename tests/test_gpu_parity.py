@@ -229,3 +229,45 @@ def test_gpu_reproduces_fe_bulk_susc_fixture(fe, fe_fixture):
         bigr = np.abs(rr) > 1e-3
         assert np.max(np.abs(sc[w][bigr] - rr[bigr]) / np.abs(rr[bigr])) < 2e-8
     h.close()
+
+
+def test_other_orbital_counts_use_generic_kernel():
+    """p_x,p_y chain (nOrb = 2, d = 8): the generic LDS inversion and the
+    small-block panels, against the oracle."""
+    spec = CF.chain_p(2)
+    m, s = TBModel(spec), O.OracleSystem(spec)
+    assert m.d == 8
+    y, wy = Ct.imag_epoints(2, 8)
+    kp, wk = Ct.uniform_kmesh(m.b, 12, 1, 1)
+    K = np.tile(kp, (len(y), 1)); Y = np.repeat(y, len(wk)); W = np.outer(wy, wk).ravel()
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    for om in (0.0, 3e-3):
+        x2, p2 = Ct.real_epoints(m.Ef, om, 1, 4)
+        g = h.chi0([0.1, 0, 0], om, m.Ef, 1e-3, 1e-3, x2, p2)
+        o = (s.chi0([0.1, 0, 0], om, 1e-3, 1e-3, (K, Y, W), (kp, wk), (x2, p2)) if om
+             else s.chi0([0.1, 0, 0], om, 1e-3, 1e-3, (K, Y, W)))
+        assert rel(g, o) < 1e-10
+    h.close()
+
+
+@pytest.mark.parametrize("which", ["cfg0", "cfg2"])
+def test_mixed_precision_error(which, cfg0, cfg2):
+    """Mixed path (FP32 panels, FP32 MFMA, FP64 sums and Dyson) against the
+    FP64 oracle: relative Frobenius error of chi0 and chi <= 1e-6."""
+    m, s = {"cfg0": cfg0, "cfg2": cfg2}[which]
+    nk = 6 if which == "cfg0" else 3
+    (K, Y, W), (kp, wk) = _work(m, nk, n1gl=16)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    om = 4e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 8)
+    gm = h.chi0([0, 0, 0], om, m.Ef, 5e-4, 5e-4, x2, p2, precision=B.PREC_MIXED)
+    o = s.chi0([0, 0, 0], om, 5e-4, 5e-4, (K, Y, W), (kp, wk), (x2, p2))
+    assert rel(gm, o) < 1e-6, rel(gm, o)
+    chim, _ = h.dyson(gm, m.Umat)
+    oc = O.dyson(o, s.umat())
+    assert rel(chim, oc) < 1e-6, rel(chim, oc)
+    h.close()

@@ -146,3 +146,32 @@ def system(cfg):
     if cfg == 2:
         return bcc_supercell(3)
     raise ValueError("unknown configuration %r" % cfg)
+
+
+def omega_grid(cfg, spec=None):
+    """Frequencies of a configuration (Ry).  cfg2 uses 64 nonzero
+    frequencies up to 0.01 Ry."""
+    r = run_params(cfg)
+    if r["omega"] is not None:
+        return np.asarray(r["omega"], dtype=float)
+    return np.linspace(0.01 / 64, 0.01, 64)
+
+
+def chain_p(nA=2, seed=5, a0=4.0):
+    """Small 1D chain with p_x, p_y orbitals (no d orbitals: U = 0), for
+    checks of other orbital counts and of the 1D k-mesh."""
+    rng = np.random.default_rng(seed)
+    types = []
+    for i in range(nA):
+        ty = seeded_type(rng, 1)
+        ty["lambda"] = [float(_ry(0.2)), 0.0]
+        ty["U"] = [0.0, 0.0]
+        types.append(ty)
+    pos = [[0.0, (a0 / nA) * i * 0.5, 0.0] for i in range(nA)]
+    spec = {
+        "name": "p chain seeded", "seed": seed, "sysdim": 1, "a": [[a0, 0, 0], [0, 0, 0], [0, 0, 0]],
+        "pos": pos, "type": list(range(nA)), "orbs": [2, 3], "nStages": 1, "relTol": 0.05, "types": types,
+        "rho": [[0.0] * 2] * nA, "mx": [[0.0] * 2] * nA, "my": [[0.0] * 2] * nA, "mz": [[0.0] * 2] * nA,
+        "socscale": 1.0,
+    }
+    return _finish(spec, filling=1.0 * nA)

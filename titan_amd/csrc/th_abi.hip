@@ -152,7 +152,8 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
     const size_t need = (size_t)h->n_im * h->nA2 * 2 * h->Mpad * prec_size(prec);
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return TH_OK;
-    if (need > freeb / 2 || need > ((size_t)64 << 30)) return TH_OK;   // recompute per batch instead
+    // the cache may take most of HBM (288 GB on MI355X); otherwise recompute per batch
+    if (need > (size_t)(0.7 * (double)freeb)) return TH_OK;
     if (hipMalloc(&h->d_bcache, need) != hipSuccess) {
         h->d_bcache = nullptr;
         (void)hipGetLastError();

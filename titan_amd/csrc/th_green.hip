@@ -112,15 +112,12 @@ __device__ void gj_invert(cplx<R> *M, int d, cplx<R> *colk, cplx<R> *prow, int *
     __syncthreads();
 }
 
-template <typename P, int NT>
-__global__ __launch_bounds__(NT) void green_kernel(green_params g)
+
+// (k, z, weight) of pair p for the kernel mode
+__device__ inline void pair_point(const green_params &g, long p, double &k0, double &k1, double &k2, double &er,
+                                  double &ei, double &w)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int d = g.d, tid = threadIdx.x;
-    const long pl = blockIdx.x;              // pair within the batch
-    const long p = g.p0 + pl;                // global pair index
-    // ---- pair -> (k, z, weight) ----
-    double k0, k1, k2, er, ei, w = 1.0;
+    w = 1.0;
     if (g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B) {
         k0 = g.kim[3 * p]; k1 = g.kim[3 * p + 1]; k2 = g.kim[3 * p + 2];
         if (g.mode == TH_G_IMAG_A) {
@@ -139,6 +136,171 @@ __global__ __launch_bounds__(NT) void green_kernel(green_params g)
             er = g.x2[ie]; ei = g.etap;
         }
     }
+}
+
+// Writes the panel row of pair pl from the inverse held row-major in S with
+// column permutation idx (G[r][c] = S[r][idx[c]]).
+template <typename P, int NT>
+__device__ void write_panel(const green_params &g, long pl, const cplx<double> *S, const int *idx, int d, double w)
+{
+    const int tid = threadIdx.x;
+    const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
+    P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+    for (int ij = 0; ij < g.nA2; ++ij) {
+        const int i = ij / nA, j = ij - (ij / nA) * nA;
+        P *o = out + (size_t)ij * 2 * Mpad;
+        for (int e = tid; e < Mpad; e += NT) {
+            double vr = 0.0, vi = 0.0;
+            if (e < Mb) {
+                int rr = e / bs, cc = e - rr * bs;
+                if (g.mode == TH_G_IMAG_A) {          // w * Gq(i,j)
+                    cplx<double> v = S[(i * bs + rr) * d + idx[j * bs + cc]];
+                    vr = w * v.x; vi = w * v.y;
+                } else if (g.mode == TH_G_IMAG_B) {   // G(j,i)
+                    cplx<double> v = S[(j * bs + rr) * d + idx[i * bs + cc]];
+                    vr = v.x; vi = v.y;
+                } else if (g.mode == TH_G_REAL_A) {   // -i w (Gq(i,j) - Gq^dagger(i,j))
+                    cplx<double> a = S[(i * bs + rr) * d + idx[j * bs + cc]];
+                    cplx<double> b = S[(j * bs + cc) * d + idx[i * bs + rr]];
+                    double dr = a.x - b.x, di = a.y + b.y;      // a - conj(b)
+                    vr = w * di; vi = -w * dr;                  // -i w (dr + i di)
+                } else {                                        // G^dagger(j,i) = conj(G(i,j))^T
+                    cplx<double> v = S[(i * bs + cc) * d + idx[j * bs + rr]];
+                    vr = v.x; vi = -v.y;
+                }
+            }
+            o[e] = (P)vr;
+            o[Mpad + e] = (P)vi;
+        }
+    }
+}
+
+// Register-resident Gauss-Jordan for a compile-time d = DD.  Thread t owns
+// the entries e = t + i*NT (row-major) of (z - H) in registers; per
+// elimination step only the pivot column and the two swapped rows travel
+// through LDS (double-buffered by step parity, two barriers per step).
+// The pivot search is repeated by every wave on the published column, so
+// no third barrier is needed.
+template <typename P, int DD, int NT>
+__global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
+{
+    constexpr int NE = DD * DD;
+    constexpr int EPT = (NE + NT - 1) / NT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);          // DD*DD staging of the result
+    cplx<double> *ph = S + NE;                                          // nhop phases
+    __shared__ cplx<double> colk[2][DD], rowk[2][DD], rowp[2][DD];
+    __shared__ int perm[DD], idx[DD];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const long pl = blockIdx.x;
+    const long p = g.p0 + pl;
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, p, k0, k1, k2, er, ei, w);
+    for (int n = tid; n < g.nhop; n += NT) {
+        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[n] = {cs, sn};
+    }
+    __syncthreads();
+    // owned entries of z - H(k)
+    const int nO = g.nOrb, nO2 = nO * nO;
+    cplx<double> v[EPT];
+    int rr[EPT], cc[EPT];
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+        const int e = tid + u * NT;
+        rr[u] = (e < NE) ? e / DD : -1;
+        cc[u] = (e < NE) ? e - (e / DD) * DD : -1;
+        v[u] = {0.0, 0.0};
+        if (e < NE) {
+            const int r = rr[u], c = cc[u];
+            double2 hv = g.h0[r + (size_t)c * DD];
+            cplx<double> x = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
+            const int ja = r / (2 * nO), rs = (r / nO) & 1, nu = r - (r / nO) * nO;
+            const int ia = c / (2 * nO), cs = (c / nO) & 1, mu = c - (c / nO) * nO;
+            if (rs == cs)
+                for (int n = 0; n < g.nhop; ++n)
+                    if (g.jb[n] == ja && g.ib[n] == ia) {
+                        double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                        x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
+                    }
+            v[u] = x;
+        }
+    }
+    for (int k = 0; k < DD; ++k) {
+        const int bb = k & 1;
+        // publish column k
+#pragma unroll
+        for (int u = 0; u < EPT; ++u)
+            if (cc[u] == k) colk[bb][rr[u]] = v[u];
+        __syncthreads();
+        // pivot search (every wave): largest |re|+|im| at or below the diagonal
+        double pv = -1.0;
+        int pi = lane;
+        if (lane >= k && lane < DD) pv = cabs1(colk[bb][lane]);
+        for (int off = 32; off > 0; off >>= 1) {
+            double ov = __shfl_xor(pv, off);
+            int oi = __shfl_xor(pi, off);
+            if (ov > pv || (ov == pv && oi < pi)) { pv = ov; pi = oi; }
+        }
+        const int pr = pi;
+        if (tid == 0) {
+            perm[k] = pr;
+            if (pv == 0.0) atomicMin(g.status, k + 1);
+        }
+        // publish rows k and pr
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            if (rr[u] == k) rowk[bb][cc[u]] = v[u];
+            if (rr[u] == pr) rowp[bb][cc[u]] = v[u];
+        }
+        __syncthreads();
+        cplx<double> piv = colk[bb][pr];
+        if (piv.x == 0.0 && piv.y == 0.0) piv.x = 1.0;
+        const cplx<double> inv = cinv(piv);
+        const cplx<double> fk = colk[bb][k];   // multiplier of the row that moves to pr
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int r = rr[u], c = cc[u];
+            if (r < 0) continue;
+            if (r == k) {
+                v[u] = (c == k) ? inv : cmul(rowp[bb][c], inv);
+            } else {
+                const cplx<double> src = (r == pr) ? rowk[bb][c] : v[u];
+                const cplx<double> f = (r == pr) ? fk : colk[bb][r];
+                if (c == k) {
+                    cplx<double> t = cmul(f, inv);
+                    v[u] = {-t.x, -t.y};
+                } else {
+                    v[u] = csub(src, cmul(f, cmul(rowp[bb][c], inv)));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; ++u)
+        if (rr[u] >= 0) S[rr[u] * DD + cc[u]] = v[u];
+    if (tid == 0) {
+        for (int c = 0; c < DD; ++c) idx[c] = c;
+        for (int k = DD - 1; k >= 0; --k) {
+            int pk = perm[k];
+            if (pk != k) { int tmp = idx[k]; idx[k] = idx[pk]; idx[pk] = tmp; }
+        }
+    }
+    __syncthreads();
+    write_panel<P, NT>(g, pl, S, idx, DD, w);
+}
+
+template <typename P, int NT>
+__global__ __launch_bounds__(NT) void green_kernel(green_params g)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int d = g.d, tid = threadIdx.x;
+    const long pl = blockIdx.x;              // pair within the batch
+    const long p = g.p0 + pl;                // global pair index
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, p, k0, k1, k2, er, ei, w);
     // ---- LDS layout ----
     cplx<double> *M = reinterpret_cast<cplx<double> *>(smem);          // d*d, FP64 (z - H)
     size_t off = sizeof(cplx<double>) * d * d;
@@ -190,36 +352,33 @@ __global__ __launch_bounds__(NT) void green_kernel(green_params g)
     __syncthreads();
     // ---- inverse ----
     gj_invert<double, NT>(M, d, colk, prow, perm, idx, s_piv, s_inv, g.status);
-    // ---- panel ----
-    const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
-    P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
-    for (int ij = 0; ij < g.nA2; ++ij) {
-        const int i = ij / nA, j = ij - (ij / nA) * nA;
-        P *o = out + (size_t)ij * 2 * Mpad;
-        for (int e = tid; e < Mpad; e += NT) {
-            double vr = 0.0, vi = 0.0;
-            if (e < Mb) {
-                int rr = e / bs, cc = e - rr * bs;
-                if (g.mode == TH_G_IMAG_A) {          // w * Gq(i,j)
-                    cplx<double> v = M[(i * bs + rr) * d + idx[j * bs + cc]];
-                    vr = w * v.x; vi = w * v.y;
-                } else if (g.mode == TH_G_IMAG_B) {   // G(j,i)
-                    cplx<double> v = M[(j * bs + rr) * d + idx[i * bs + cc]];
-                    vr = v.x; vi = v.y;
-                } else if (g.mode == TH_G_REAL_A) {   // -i w (Gq(i,j) - Gq^dagger(i,j))
-                    cplx<double> a = M[(i * bs + rr) * d + idx[j * bs + cc]];
-                    cplx<double> b = M[(j * bs + cc) * d + idx[i * bs + rr]];
-                    double dr = a.x - b.x, di = a.y + b.y;      // a - conj(b)
-                    vr = w * di; vi = -w * dr;                  // -i w (dr + i di)
-                } else {                                        // G^dagger(j,i) = conj(G(i,j))^T
-                    cplx<double> v = M[(i * bs + cc) * d + idx[j * bs + rr]];
-                    vr = v.x; vi = -v.y;
-                }
-            }
-            o[e] = (P)vr;
-            o[Mpad + e] = (P)vi;
-        }
+    write_panel<P, NT>(g, pl, M, idx, d, w);
+}
+
+template <typename P, int NT>
+int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g);
+
+template <typename P, int DD, int NT>
+int launch_green_reg(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    size_t lds = sizeof(double2) * ((size_t)DD * DD + h->nhop);
+    hipLaunchKernelGGL((green_reg_kernel<P, DD, NT>), dim3((unsigned)a.np), dim3(NT), lds, h->stream, g);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+template <typename P>
+int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    switch (h->d) {
+    case 18: return launch_green_reg<P, 18, 64>(h, a, g);
+    case 36: return launch_green_reg<P, 36, 128>(h, a, g);
+    case 40: return launch_green_reg<P, 40, 128>(h, a, g);
+    case 54: return launch_green_reg<P, 54, 256>(h, a, g);
+    default: break;
     }
+    if (h->d <= 24) return launch_green_t<P, 64>(h, a, g);
+    return launch_green_t<P, 256>(h, a, g);
 }
 
 template <typename P, int NT>
@@ -254,10 +413,6 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.panel = a.panel;
     g.status = th_status_word(h);
     h->n_launch[0] += 1;
-    if (a.prec == TH_PREC_FP64) {
-        if (h->d <= 24) return launch_green_t<double, 64>(h, a, g);
-        return launch_green_t<double, 256>(h, a, g);
-    }
-    if (h->d <= 24) return launch_green_t<float, 64>(h, a, g);
-    return launch_green_t<float, 256>(h, a, g);
+    if (a.prec == TH_PREC_FP64) return launch_green_p<double>(h, a, g);
+    return launch_green_p<float>(h, a, g);
 }
