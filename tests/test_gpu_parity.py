@@ -271,3 +271,27 @@ def test_mixed_precision_error(which, cfg0, cfg2):
     oc = O.dyson(o, s.umat())
     assert rel(chim, oc) < 1e-6, rel(chim, oc)
     h.close()
+
+
+def test_gpu_reproduces_nanoribbon_fixture(ribbon_fixture):
+    """examples/nanoribbon on the GPU (d = 40, ten atoms, 100 atom pairs)."""
+    from tests.conftest import ribbon_blocks
+    d = ribbon_fixture
+    m = TBModel(d["system"])
+    run = d["run"]
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    kp, wk = Ct.uniform_kmesh(m.b, run["nkpt"], 1, 1)
+    h = m.handle()
+    h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+    h.set_real_kmesh(kp, wk)
+    s0, sc, cond = h.sweep(run["q"], [run["emin"]], m.Ef, run["eta"], run["etap"], run["parts3"], run["n3gl"],
+                           m.Umat)
+    n = 0
+    for kind, ours in (("chihf", s0[0]), ("chi", sc[0])):
+        for blk, ref in ribbon_blocks(ours, d, kind):
+            big = np.abs(ref) > 1e-9
+            if big.any():
+                assert np.max(np.abs(blk[big] - ref[big]) / np.abs(ref[big])) < 1e-9
+                n += 1
+    assert n == 200
+    h.close()

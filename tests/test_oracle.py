@@ -311,3 +311,31 @@ def test_oracle_reproduces_fe_bulk_susc_fixture(fe_sys, fe_fixture, iw):
     bigr = np.abs(ref) > 1e-3
     relr = np.abs(s[bigr] - ref[bigr]) / np.abs(ref[bigr])
     assert relr.max() < 2e-8, relr
+
+
+def test_oracle_reproduces_nanoribbon_fixture(ribbon_fixture):
+    """examples/nanoribbon: 10-site 1D ribbon, two element types, p_x/p_y,
+    SOC on p, U = 0; all 100 chihf_i_j and chi_i_j blocks at omega = 1e-6."""
+    from tests.conftest import ribbon_blocks
+    d = ribbon_fixture
+    s = O.OracleSystem(d["system"])
+    run = d["run"]
+    R = s.tables()[0]
+    assert s.nhop == 28 and s.d == 40
+    y, wy = O.imag_epoints(run["parts"], run["n1gl"])
+    kp, wk = s.kmesh_uniform(run["nkpt"], 1, 1)
+    K = np.tile(kp, (len(y), 1))
+    om = run["emin"]
+    x2, p2 = O.real_epoints(s.Ef, om, run["parts3"], run["n3gl"])
+    c0 = s.chi0(run["q"], om, run["eta"], run["etap"], (K, np.repeat(y, len(wk)), np.outer(wy, wk).ravel()),
+                (kp, wk), (x2, p2))
+    shf = O.trace(10, 2, c0)
+    sc = O.trace(10, 2, O.dyson(c0, s.umat()))
+    n = 0
+    for kind, ours in (("chihf", shf), ("chi", sc)):
+        for blk, ref in ribbon_blocks(ours, d, kind):
+            big = np.abs(ref) > 1e-9
+            if big.any():
+                assert np.max(np.abs(blk[big] - ref[big]) / np.abs(ref[big])) < 1e-9
+                n += 1
+    assert n == 200
