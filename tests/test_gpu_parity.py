@@ -295,3 +295,24 @@ def test_gpu_reproduces_nanoribbon_fixture(ribbon_fixture):
                 n += 1
     assert n == 200
     h.close()
+
+
+@pytest.mark.parametrize("which", ["bcc4", "slab8"])
+def test_large_blocks_match_oracle(which):
+    """d = 72 (4-site bcc supercell) and d = 144 (8-layer fcc(001) slab of
+    cfg1): register-resident inversion with block-wise panel staging."""
+    spec = CF.bcc_supercell(4) if which == "bcc4" else CF.fcc001_slab(8)
+    m, s = TBModel(spec), O.OracleSystem(spec)
+    dims = (2, 1, 1) if which == "bcc4" else (2, 2, 1)
+    y, wy = Ct.imag_epoints(1, 3)
+    kp, wk = Ct.uniform_kmesh(m.b, *dims)
+    K = np.tile(kp, (len(y), 1)); Y = np.repeat(y, len(wk)); W = np.outer(wy, wk).ravel()
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    om = 2e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 2)
+    g = h.chi0([0.02, 0, 0], om, m.Ef, 1e-3, 1e-3, x2, p2)
+    o = s.chi0([0.02, 0, 0], om, 1e-3, 1e-3, (K, Y, W), (kp, wk), (x2, p2))
+    assert rel(g, o) < 1e-10, rel(g, o)
+    h.close()

@@ -278,7 +278,7 @@ const char *th_last_error(void) { return g_err.c_str(); }
 int th_limits(int *max_nOrb, int *max_d, int *max_nA)
 {
     if (max_nOrb) *max_nOrb = TH_MAX_NORB;
-    if (max_d) *max_d = TH_MAX_D_LDS;
+    if (max_d) *max_d = TH_MAX_D;
     if (max_nA) *max_nA = TH_MAX_NA;
     return TH_OK;
 }
@@ -299,7 +299,7 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
             return TH_E_ARG;
         }
     const int d = 2 * nOrb * nA;
-    if (d > TH_MAX_D_LDS) {
+    if (d > TH_MAX_D_LDS && d != 72 && d != 144) {
         th_set_error("th_create: d = 2*nOrb*nA = " + std::to_string(d) + " exceeds " + std::to_string(TH_MAX_D_LDS));
         return TH_E_UNSUPPORTED;
     }
@@ -325,15 +325,36 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
     int *st = nullptr;
     TH_HIP(hipMalloc(&st, sizeof(int)));
     g_status.push_back({h, st});
+    // hops grouped by atom pair (jb, ib) so an entry of H(k) visits only its own hops
+    std::vector<int> order(nhop), hoff(nA * nA + 1, 0);
+    for (int n = 0; n < nhop; ++n) {
+        order[n] = n;
+        hoff[jb[n] * nA + ib[n] + 1] += 1;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return jb[a] * nA + ib[a] < jb[b] * nA + ib[b]; });
+    for (int k = 0; k < nA * nA; ++k) hoff[k + 1] += hoff[k];
+    TH_HIP(hipMalloc(&h->d_hoff, sizeof(int) * (nA * nA + 1)));
+    TH_HIP(hipMemcpy(h->d_hoff, hoff.data(), sizeof(int) * (nA * nA + 1), hipMemcpyHostToDevice));
     if (nhop > 0) {
+        const size_t nO2 = (size_t)nOrb * nOrb;
+        std::vector<double> Rs(3 * (size_t)nhop), ts(2 * nO2 * nhop);
+        std::vector<int> jbs(nhop), ibs(nhop);
+        for (int n = 0; n < nhop; ++n) {
+            const int o = order[n];
+            for (int x = 0; x < 3; ++x) Rs[3 * n + x] = R[3 * o + x];
+            jbs[n] = jb[o];
+            ibs[n] = ib[o];
+            std::memcpy(&ts[2 * nO2 * n], &t[2 * nO2 * o], sizeof(double) * 2 * nO2);
+        }
         TH_HIP(hipMalloc(&h->d_R, sizeof(double) * 3 * nhop));
         TH_HIP(hipMalloc(&h->d_jb, sizeof(int) * nhop));
         TH_HIP(hipMalloc(&h->d_ib, sizeof(int) * nhop));
         TH_HIP(hipMalloc(&h->d_t, sizeof(double2) * (size_t)nhop * nOrb * nOrb));
-        TH_HIP(hipMemcpy(h->d_R, R, sizeof(double) * 3 * nhop, hipMemcpyHostToDevice));
-        TH_HIP(hipMemcpy(h->d_jb, jb, sizeof(int) * nhop, hipMemcpyHostToDevice));
-        TH_HIP(hipMemcpy(h->d_ib, ib, sizeof(int) * nhop, hipMemcpyHostToDevice));
-        TH_HIP(hipMemcpy(h->d_t, t, sizeof(double2) * (size_t)nhop * nOrb * nOrb, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_R, Rs.data(), sizeof(double) * 3 * nhop, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_jb, jbs.data(), sizeof(int) * nhop, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_ib, ibs.data(), sizeof(int) * nhop, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(h->d_t, ts.data(), sizeof(double2) * (size_t)nhop * nOrb * nOrb, hipMemcpyHostToDevice));
     }
     TH_HIP(hipMalloc(&h->d_h0, sizeof(double2) * (size_t)d * d));
     TH_HIP(hipMemcpy(h->d_h0, h0, sizeof(double2) * (size_t)d * d, hipMemcpyHostToDevice));
@@ -483,7 +504,7 @@ int th_destroy(th_handle *h)
 {
     if (!h) return TH_OK;
     hipStreamSynchronize(h->stream);
-    void *bufs[] = {h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
+    void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
                     h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X};
     for (void *b : bufs)

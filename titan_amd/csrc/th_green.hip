@@ -37,6 +37,7 @@ struct green_params {
     const double *R;
     const int *jb, *ib;
     const double2 *t, *h0;
+    const int *hoff;
     const double *kim, *yim, *wim, *kre, *wre, *x2, *p2;
     void *panel;
     int *status;
@@ -186,9 +187,10 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
 {
     constexpr int NE = DD * DD;
     constexpr int EPT = (NE + NT - 1) / NT;
+    constexpr bool FULL_STAGE = (NE * 16) <= 49152;
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);          // DD*DD staging of the result
-    cplx<double> *ph = S + NE;                                          // nhop phases
+    cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);          // staging of the result
+    cplx<double> *ph = S + (FULL_STAGE ? NE : 2 * g.bs * g.bs);        // nhop phases
     __shared__ cplx<double> colk[2][DD], rowk[2][DD], rowp[2][DD];
     __shared__ int perm[DD], idx[DD];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -205,26 +207,27 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
     __syncthreads();
     // owned entries of z - H(k)
     const int nO = g.nOrb, nO2 = nO * nO;
+    // entry u of this thread: e = tid + u*NT, row e / DD, column e % DD
+#define RR(u) ((tid + (u) * NT) < NE ? (tid + (u) * NT) / DD : -1)
+#define CC(u) ((tid + (u) * NT) % DD)
     cplx<double> v[EPT];
-    int rr[EPT], cc[EPT];
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
         const int e = tid + u * NT;
-        rr[u] = (e < NE) ? e / DD : -1;
-        cc[u] = (e < NE) ? e - (e / DD) * DD : -1;
         v[u] = {0.0, 0.0};
         if (e < NE) {
-            const int r = rr[u], c = cc[u];
+            const int r = e / DD, c = e - (e / DD) * DD;
             double2 hv = g.h0[r + (size_t)c * DD];
             cplx<double> x = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
             const int ja = r / (2 * nO), rs = (r / nO) & 1, nu = r - (r / nO) * nO;
             const int ia = c / (2 * nO), cs = (c / nO) & 1, mu = c - (c / nO) * nO;
-            if (rs == cs)
-                for (int n = 0; n < g.nhop; ++n)
-                    if (g.jb[n] == ja && g.ib[n] == ia) {
-                        double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
-                        x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
-                    }
+            if (rs == cs) {
+                const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
+                for (int n = h1; n < h2; ++n) {
+                    double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                    x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
+                }
+            }
             v[u] = x;
         }
     }
@@ -233,12 +236,16 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
         // publish column k
 #pragma unroll
         for (int u = 0; u < EPT; ++u)
-            if (cc[u] == k) colk[bb][rr[u]] = v[u];
+            if (RR(u) >= 0 && CC(u) == k) colk[bb][RR(u)] = v[u];
         __syncthreads();
         // pivot search (every wave): largest |re|+|im| at or below the diagonal
         double pv = -1.0;
-        int pi = lane;
-        if (lane >= k && lane < DD) pv = cabs1(colk[bb][lane]);
+        int pi = DD;
+        for (int r = lane; r < DD; r += 64)
+            if (r >= k) {
+                double a = cabs1(colk[bb][r]);
+                if (a > pv) { pv = a; pi = r; }
+            }
         for (int off = 32; off > 0; off >>= 1) {
             double ov = __shfl_xor(pv, off);
             int oi = __shfl_xor(pi, off);
@@ -252,8 +259,8 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
         // publish rows k and pr
 #pragma unroll
         for (int u = 0; u < EPT; ++u) {
-            if (rr[u] == k) rowk[bb][cc[u]] = v[u];
-            if (rr[u] == pr) rowp[bb][cc[u]] = v[u];
+            if (RR(u) == k) rowk[bb][CC(u)] = v[u];
+            if (RR(u) == pr) rowp[bb][CC(u)] = v[u];
         }
         __syncthreads();
         cplx<double> piv = colk[bb][pr];
@@ -262,7 +269,7 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
         const cplx<double> fk = colk[bb][k];   // multiplier of the row that moves to pr
 #pragma unroll
         for (int u = 0; u < EPT; ++u) {
-            const int r = rr[u], c = cc[u];
+            const int r = RR(u), c = CC(u);
             if (r < 0) continue;
             if (r == k) {
                 v[u] = (c == k) ? inv : cmul(rowp[bb][c], inv);
@@ -278,9 +285,6 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
             }
         }
     }
-#pragma unroll
-    for (int u = 0; u < EPT; ++u)
-        if (rr[u] >= 0) S[rr[u] * DD + cc[u]] = v[u];
     if (tid == 0) {
         for (int c = 0; c < DD; ++c) idx[c] = c;
         for (int k = DD - 1; k >= 0; --k) {
@@ -288,9 +292,63 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
             if (pk != k) { int tmp = idx[k]; idx[k] = idx[pk]; idx[pk] = tmp; }
         }
     }
-    __syncthreads();
-    write_panel<P, NT>(g, pl, S, idx, DD, w);
+    if constexpr (FULL_STAGE) {
+#pragma unroll
+        for (int u = 0; u < EPT; ++u)
+            if (RR(u) >= 0) S[RR(u) * DD + CC(u)] = v[u];
+        __syncthreads();
+        write_panel<P, NT>(g, pl, S, idx, DD, w);
+    } else {
+        // the inverse does not fit LDS: stage one atom-block pair at a time.
+        // Entry (r, c') of the registers is G[r][pos[c']] with pos = idx^-1.
+        int *pos = perm;   // perm is no longer needed
+        __syncthreads();
+        if (tid < DD) pos[idx[tid]] = tid;
+        __syncthreads();
+        const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
+        cplx<double> *Sij = S, *Sji = S + Mb;          // blocks (i,j) and (j,i), row-major bs x bs
+        P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+        for (int ij = 0; ij < g.nA2; ++ij) {
+            const int i = ij / nA, j = ij - (ij / nA) * nA;
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) {
+                const int r = RR(u);
+                if (r < 0) continue;
+                const int fcu = pos[CC(u)];
+                const int ra = r / bs, ca = fcu / bs;
+                if (ra == i && ca == j) Sij[(r - i * bs) * bs + fcu - j * bs] = v[u];
+                if (ra == j && ca == i) Sji[(r - j * bs) * bs + fcu - i * bs] = v[u];
+            }
+            __syncthreads();
+            P *o = out + (size_t)ij * 2 * Mpad;
+            for (int e = tid; e < Mpad; e += NT) {
+                double vr = 0.0, vi = 0.0;
+                if (e < Mb) {
+                    const int a = e / bs, b = e - a * bs;
+                    if (g.mode == TH_G_IMAG_A) {
+                        cplx<double> x = Sij[a * bs + b];
+                        vr = w * x.x; vi = w * x.y;
+                    } else if (g.mode == TH_G_IMAG_B) {       // G(j,i)
+                        cplx<double> x = Sji[a * bs + b];
+                        vr = x.x; vi = x.y;
+                    } else if (g.mode == TH_G_REAL_A) {       // -i w (G(i,j) - G(j,i)^dagger)
+                        cplx<double> x = Sij[a * bs + b], y = Sji[b * bs + a];
+                        double dr = x.x - y.x, di = x.y + y.y;
+                        vr = w * di; vi = -w * dr;
+                    } else {                                  // G^dagger(j,i) = conj(G(i,j))^T
+                        cplx<double> x = Sij[b * bs + a];
+                        vr = x.x; vi = -x.y;
+                    }
+                }
+                o[e] = (P)vr;
+                o[Mpad + e] = (P)vi;
+            }
+            __syncthreads();
+        }
+    }
 }
+#undef RR
+#undef CC
 
 template <typename P, int NT>
 __global__ __launch_bounds__(NT) void green_kernel(green_params g)
@@ -361,7 +419,8 @@ int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g);
 template <typename P, int DD, int NT>
 int launch_green_reg(th_handle *h, const th_green_args &a, const green_params &g)
 {
-    size_t lds = sizeof(double2) * ((size_t)DD * DD + h->nhop);
+    const bool full = (DD * DD * 16) <= 49152;
+    size_t lds = sizeof(double2) * ((full ? (size_t)DD * DD : (size_t)2 * h->bs * h->bs) + h->nhop);
     hipLaunchKernelGGL((green_reg_kernel<P, DD, NT>), dim3((unsigned)a.np), dim3(NT), lds, h->stream, g);
     TH_HIP(hipGetLastError());
     return TH_OK;
@@ -375,6 +434,8 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
     case 36: return launch_green_reg<P, 36, 128>(h, a, g);
     case 40: return launch_green_reg<P, 40, 128>(h, a, g);
     case 54: return launch_green_reg<P, 54, 256>(h, a, g);
+    case 72: return launch_green_reg<P, 72, 256>(h, a, g);
+    case 144: return launch_green_reg<P, 144, 1024>(h, a, g);
     default: break;
     }
     if (h->d <= 24) return launch_green_t<P, 64>(h, a, g);
@@ -397,8 +458,8 @@ int *th_status_word(th_handle *h);
 
 int th_launch_green(th_handle *h, const th_green_args &a)
 {
-    if (h->d > TH_MAX_D_LDS) {
-        th_set_error("d = 2*nOrb*nA larger than the LDS-resident inversion supports");
+    if (h->d > TH_MAX_D_LDS && h->d != 72 && h->d != 144) {
+        th_set_error("d = 2*nOrb*nA = " + std::to_string(h->d) + " has no inversion kernel");
         return TH_E_UNSUPPORTED;
     }
     green_params g;
@@ -407,7 +468,7 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.Mpad = h->Mpad; g.nA2 = h->nA2; g.p0 = a.p0; g.nk_re = h->nk_re;
     g.q0 = a.q[0]; g.q1 = a.q[1]; g.q2 = a.q[2];
     g.omega = a.omega; g.Ef = a.Ef; g.eta = a.eta; g.etap = a.etap;
-    g.R = h->d_R; g.jb = h->d_jb; g.ib = h->d_ib; g.t = h->d_t; g.h0 = h->d_h0;
+    g.R = h->d_R; g.jb = h->d_jb; g.ib = h->d_ib; g.t = h->d_t; g.h0 = h->d_h0; g.hoff = h->d_hoff;
     g.kim = h->d_kim; g.yim = h->d_yim; g.wim = h->d_wim;
     g.kre = h->d_kre; g.wre = h->d_wre; g.x2 = h->d_x2; g.p2 = h->d_p2;
     g.panel = a.panel;

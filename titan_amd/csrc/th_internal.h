@@ -15,7 +15,8 @@ constexpr int TH_TILE = 112;        // output tile edge (7 MFMA blocks of 16)
 constexpr int TH_KC = 8;            // pairs per LDS stage of the GEMM
 constexpr int TH_GEMM_WAVES = 7;    // one wave per 16-row block of the tile
 constexpr int TH_MAX_NORB = 9;
-constexpr int TH_MAX_D_LDS = 64;    // largest d inverted with the LDS-resident kernel
+constexpr int TH_MAX_D_LDS = 64;    // largest d of the generic LDS-resident inversion
+constexpr int TH_MAX_D = 144;       // largest d of any inversion kernel
 constexpr int TH_MAX_NA = 32;
 
 inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
@@ -27,7 +28,8 @@ struct th_handle {
     // model
     double *d_R = nullptr;       // 3*nhop
     int *d_jb = nullptr, *d_ib = nullptr;
-    double2 *d_t = nullptr;      // nhop * nOrb*nOrb (column-major blocks)
+    double2 *d_t = nullptr;      // nhop * nOrb*nOrb (column-major blocks), sorted by (jb, ib)
+    int *d_hoff = nullptr;       // nA*nA+1 offsets: hops of atom pair (jb, ib) are hoff[jb*nA+ib] ..
     double2 *d_h0 = nullptr;     // d*d column-major
     // work lists
     long n_im = 0, nk_re = 0;
