@@ -119,6 +119,15 @@ int th_limits(int *max_nOrb, int *max_d, int *max_nA);
 int th_last_timing(th_handle *h, double *t, long *n);
 
 
+/* Tuning knobs of the kernels (no counterpart in the reference; used by the
+ * autotuner in titan_amd/autotune.py):
+ *   "green_variant"  0 = 2-D register-blocked inversion (default), 1 = 1-D
+ *   "splits"         K-splits of the contraction (workgroups per tile)
+ *   "batch_mb"       panel memory per batch in MB (pairs per launch)
+ * th_get_tuning reads the current values. */
+int th_tune(th_handle *h, const char *key, long value);
+int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_mb);
+
 /* Kernel-level test hooks (no counterpart in the reference; used by the
  * GPU unit tests to check one kernel at a time).
  * thx_green_panel: runs the green-function kernel in 'mode' (0 imag A,

@@ -20,7 +20,8 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_destroy", "th_last_error", "th_limits", "th_last_timing")
+           "th_chi_sweep", "th_destroy", "th_last_error", "th_limits", "th_last_timing", "th_tune",
+           "th_get_tuning")
 
 
 class TitanHipError(RuntimeError):
@@ -47,6 +48,8 @@ def load():
     L.th_last_error.restype = C.c_char_p
     L.th_limits.argtypes = [_ip, _ip, _ip]
     L.th_last_timing.argtypes = [C.c_void_p, _dp, _lp]
+    L.th_tune.argtypes = [C.c_void_p, C.c_char_p, C.c_long]
+    L.th_get_tuning.argtypes = [C.c_void_p, _lp, _lp, _lp]
     _lib = L
     return L
 
@@ -159,6 +162,15 @@ class Handle:
         s0 = s0.transpose(0, 2, 1).copy()
         s = s.transpose(0, 2, 1).copy()
         return (s0, s, cond, chi) if full else (s0, s, cond)
+
+    def tune(self, **kw):
+        for k, v in kw.items():
+            _check(load().th_tune(self._h, k.encode(), int(v)), "th_tune(%s)" % k)
+
+    def tuning(self):
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        load().th_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return {"green_variant": a.value, "splits": b.value, "batch_mb": c.value}
 
     def last_timing(self):
         t = np.zeros(5)

@@ -132,7 +132,7 @@ int ensure_panels(th_handle *h, long pairs)
 long batch_pairs(th_handle *h, long total)
 {
     const size_t per = (size_t)h->nA2 * 2 * h->Mpad * sizeof(double);
-    const size_t budget = (size_t)512 << 20;
+    const size_t budget = (size_t)h->batch_mb << 20;
     long b = (long)std::max<size_t>(TH_KC, budget / per);
     return std::min(b, std::max<long>(total, 1));
 }
@@ -497,6 +497,42 @@ int th_last_timing(th_handle *h, double *t, long *n)
         for (int s = 0; s < 5; ++s) t[s] = h->t_ms[s];
     if (n)
         for (int s = 0; s < 4; ++s) n[s] = h->n_launch[s];
+    return TH_OK;
+}
+
+
+int th_tune(th_handle *h, const char *key, long value)
+{
+    if (!h || !key) { th_set_error("th_tune: bad arguments"); return TH_E_ARG; }
+    const std::string k(key);
+    if (k == "green_variant") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: green_variant is 0 or 1"); return TH_E_ARG; }
+        h->kernel_variant = (int)value;
+    } else if (k == "splits") {
+        if (value < 1 || value > 4096) { th_set_error("th_tune: splits in 1..4096"); return TH_E_ARG; }
+        if (value > h->splits) {
+            const size_t MM = (size_t)h->Mpad * h->Mpad;
+            hipFree(h->d_part);
+            h->d_part = nullptr;
+            TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)value * h->nA2 * 2 * MM));
+        }
+        h->splits = (int)value;
+    } else if (k == "batch_mb") {
+        if (value < 16 || value > 65536) { th_set_error("th_tune: batch_mb in 16..65536"); return TH_E_ARG; }
+        h->batch_mb = (int)value;
+    } else {
+        th_set_error("th_tune: unknown key " + k);
+        return TH_E_ARG;
+    }
+    return TH_OK;
+}
+
+int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_mb)
+{
+    if (!h) return TH_E_ARG;
+    if (green_variant) *green_variant = h->kernel_variant;
+    if (splits) *splits = h->splits;
+    if (batch_mb) *batch_mb = h->batch_mb;
     return TH_OK;
 }
 

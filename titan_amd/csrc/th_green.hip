@@ -350,6 +350,216 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
 #undef RR
 #undef CC
 
+
+// 2-D register-blocked Gauss-Jordan for compile-time d = DD.  Thread (ty,tx)
+// of a TY x TX grid owns rows ty + i*TY and columns tx + j*TX of (z - H).
+// Pivoting is implicit: rows are never moved; step k picks the unused row
+// p_k with the largest |re|+|im| in column k (one packed-key DPP wavefront
+// max, repeated by every wave) and the result is unscrambled at the end,
+// G[k][p_c] = V[p_k][c].  The published scaled pivot row carries 1 + 1/piv
+// in column k, so every non-pivot entry is updated by one complex FMA,
+// v -= f * row, with no selects.  Per step: the pivot column and the scaled
+// pivot row pass through LDS (double-buffered by step parity), two barriers.
+extern "C" __device__ unsigned __ockl_wfred_max_u32(unsigned);
+
+template <typename P, int DD, int TY, int TX>
+__global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
+{
+    constexpr int NT = TY * TX;
+    constexpr int RI = (DD + TY - 1) / TY, CJ = (DD + TX - 1) / TX;
+    constexpr int NE = DD * DD;
+    constexpr bool FULL_STAGE = (NE * 16) <= 49152;
+    static_assert(DD <= 255, "row index packed in 8 bits");
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);
+    cplx<double> *ph = S + (FULL_STAGE ? NE : 2 * g.bs * g.bs);
+    __shared__ cplx<double> colk[2][DD], prs[2][DD];
+    __shared__ int perm[DD], qinv[DD], ident[DD];
+    __shared__ unsigned char used[DD];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int ty = tid / TX, tx = tid - (tid / TX) * TX;
+    const long pl = blockIdx.x;
+    const long p = g.p0 + pl;
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, p, k0, k1, k2, er, ei, w);
+    for (int n = tid; n < g.nhop; n += NT) {
+        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[n] = {cs, sn};
+    }
+    for (int r = tid; r < DD; r += NT) {
+        used[r] = 0;
+        ident[r] = r;
+    }
+    __syncthreads();
+    const int nO = g.nOrb, nO2 = nO * nO;
+    cplx<double> v[RI][CJ];
+#pragma unroll
+    for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < CJ; ++j) {
+            const int r = ty + i * TY, c = tx + j * TX;
+            v[i][j] = {0.0, 0.0};
+            if (r < DD && c < DD) {
+                double2 hv = g.h0[r + (size_t)c * DD];
+                cplx<double> x = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
+                const int ja = r / (2 * nO), rs = (r / nO) & 1, nu = r - (r / nO) * nO;
+                const int ia = c / (2 * nO), csp = (c / nO) & 1, mu = c - (c / nO) * nO;
+                if (rs == csp) {
+                    const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
+                    for (int n = h1; n < h2; ++n) {
+                        double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                        x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
+                    }
+                }
+                v[i][j] = x;
+            }
+        }
+    for (int k = 0; k < DD; ++k) {
+        const int bb = k & 1;
+        if (tx == k % TX) {
+            const int jk = k / TX;
+#pragma unroll
+            for (int j = 0; j < CJ; ++j)
+                if (j == jk)
+#pragma unroll
+                    for (int i = 0; i < RI; ++i) {
+                        const int r = ty + i * TY;
+                        if (r < DD) colk[bb][r] = v[i][j];
+                    }
+        }
+        __syncthreads();
+        // pivot: key = magnitude (float, low 8 mantissa bits dropped) | 255 - row,
+        // so the maximum key is the largest |re|+|im|, lowest row on ties
+        unsigned key = 0u;
+        for (int r = lane; r < DD; r += 64)
+            if (!used[r]) {
+                const float a = (float)cabs1(colk[bb][r]);
+                const unsigned kk = (__float_as_uint(a) & 0xFFFFFF00u) | (unsigned)(255 - r);
+                key = kk > key ? kk : key;
+            }
+        key = __ockl_wfred_max_u32(key);
+        const int pr = 255 - (int)(key & 0xFFu);
+        cplx<double> piv = colk[bb][pr];
+        if (piv.x == 0.0 && piv.y == 0.0) {
+            if (tid == 0) atomicMin(g.status, k + 1);
+            piv.x = 1.0;
+        }
+        const cplx<double> inv = cinv(piv);
+        if (ty == pr % TY) {
+            const int ip = pr / TY;
+#pragma unroll
+            for (int i = 0; i < RI; ++i)
+                if (i == ip)
+#pragma unroll
+                    for (int j = 0; j < CJ; ++j) {
+                        const int c = tx + j * TX;
+                        if (c < DD) prs[bb][c] = (c == k) ? cplx<double>{1.0 + inv.x, inv.y} : cmul(v[i][j], inv);
+                    }
+        }
+        __syncthreads();
+        // after the barrier: every wave has finished this step's pivot search
+        if (tid == 0) {
+            perm[k] = pr;
+            used[pr] = 1;
+        }
+        cplx<double> pj[CJ];
+#pragma unroll
+        for (int j = 0; j < CJ; ++j) {
+            const int c = tx + j * TX;
+            pj[j] = (c < DD) ? prs[bb][c] : cplx<double>{0.0, 0.0};
+        }
+#pragma unroll
+        for (int i = 0; i < RI; ++i) {
+            const int r = ty + i * TY;
+            if (r >= DD) continue;
+            if (r == pr) {
+#pragma unroll
+                for (int j = 0; j < CJ; ++j) {
+                    const int c = tx + j * TX;
+                    v[i][j] = (c == k) ? inv : cmul(v[i][j], inv);
+                }
+            } else {
+                const cplx<double> f = colk[bb][r];
+#pragma unroll
+                for (int j = 0; j < CJ; ++j) {
+                    // v -= f * pj
+                    v[i][j] = {fma(-f.x, pj[j].x, fma(f.y, pj[j].y, v[i][j].x)),
+                               fma(-f.x, pj[j].y, fma(-f.y, pj[j].x, v[i][j].y))};
+                }
+            }
+        }
+    }
+    // unscramble: G[k][p_c] = V[p_k][c], i.e. entry (rho, c) goes to (q[rho], perm[c])
+    __syncthreads();
+    if (tid < DD) qinv[perm[tid]] = tid;
+    __syncthreads();
+    int dcol[CJ];
+#pragma unroll
+    for (int j = 0; j < CJ; ++j) dcol[j] = (tx + j * TX < DD) ? perm[tx + j * TX] : -1;
+    if constexpr (FULL_STAGE) {
+#pragma unroll
+        for (int i = 0; i < RI; ++i) {
+            const int r = ty + i * TY;
+            if (r >= DD) continue;
+            const int dr = qinv[r];
+#pragma unroll
+            for (int j = 0; j < CJ; ++j)
+                if (dcol[j] >= 0) S[dr * DD + dcol[j]] = v[i][j];
+        }
+        __syncthreads();
+        write_panel<P, NT>(g, pl, S, ident, DD, w);
+    } else {
+        const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
+        cplx<double> *Sij = S, *Sji = S + Mb;
+        P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+        int drow[RI];
+#pragma unroll
+        for (int i = 0; i < RI; ++i) drow[i] = (ty + i * TY < DD) ? qinv[ty + i * TY] : -1;
+        for (int ij = 0; ij < g.nA2; ++ij) {
+            const int i0 = ij / nA, j0 = ij - (ij / nA) * nA;
+#pragma unroll
+            for (int i = 0; i < RI; ++i) {
+                if (drow[i] < 0) continue;
+                const int ra = drow[i] / bs;
+#pragma unroll
+                for (int j = 0; j < CJ; ++j) {
+                    if (dcol[j] < 0) continue;
+                    const int ca = dcol[j] / bs;
+                    if (ra == i0 && ca == j0) Sij[(drow[i] - i0 * bs) * bs + dcol[j] - j0 * bs] = v[i][j];
+                    if (ra == j0 && ca == i0) Sji[(drow[i] - j0 * bs) * bs + dcol[j] - i0 * bs] = v[i][j];
+                }
+            }
+            __syncthreads();
+            P *o = out + (size_t)ij * 2 * Mpad;
+            for (int e = tid; e < Mpad; e += NT) {
+                double vr = 0.0, vi = 0.0;
+                if (e < Mb) {
+                    const int a = e / bs, b = e - a * bs;
+                    if (g.mode == TH_G_IMAG_A) {
+                        cplx<double> x = Sij[a * bs + b];
+                        vr = w * x.x; vi = w * x.y;
+                    } else if (g.mode == TH_G_IMAG_B) {
+                        cplx<double> x = Sji[a * bs + b];
+                        vr = x.x; vi = x.y;
+                    } else if (g.mode == TH_G_REAL_A) {
+                        cplx<double> x = Sij[a * bs + b], y = Sji[b * bs + a];
+                        double dr = x.x - y.x, di = x.y + y.y;
+                        vr = w * di; vi = -w * dr;
+                    } else {
+                        cplx<double> x = Sij[b * bs + a];
+                        vr = x.x; vi = -x.y;
+                    }
+                }
+                o[e] = (P)vr;
+                o[Mpad + e] = (P)vi;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <typename P, int NT>
 __global__ __launch_bounds__(NT) void green_kernel(green_params g)
 {
@@ -426,16 +636,33 @@ int launch_green_reg(th_handle *h, const th_green_args &a, const green_params &g
     return TH_OK;
 }
 
+template <typename P, int DD, int TY, int TX>
+int launch_green_2d(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    const bool full = (DD * DD * 16) <= 49152;
+    size_t lds = sizeof(double2) * ((full ? (size_t)DD * DD : (size_t)2 * h->bs * h->bs) + h->nhop);
+    hipLaunchKernelGGL((green_2d_kernel<P, DD, TY, TX>), dim3((unsigned)a.np), dim3(TY * TX), lds, h->stream, g);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
+    if (h->kernel_variant == 1) {   // one-dimensional ownership (kept for comparison)
+        switch (h->d) {
+        case 18: return launch_green_reg<P, 18, 64>(h, a, g);
+        case 54: return launch_green_reg<P, 54, 256>(h, a, g);
+        default: break;
+        }
+    }
     switch (h->d) {
-    case 18: return launch_green_reg<P, 18, 64>(h, a, g);
-    case 36: return launch_green_reg<P, 36, 128>(h, a, g);
-    case 40: return launch_green_reg<P, 40, 128>(h, a, g);
-    case 54: return launch_green_reg<P, 54, 256>(h, a, g);
-    case 72: return launch_green_reg<P, 72, 256>(h, a, g);
-    case 144: return launch_green_reg<P, 144, 1024>(h, a, g);
+    case 18: return launch_green_2d<P, 18, 8, 8>(h, a, g);
+    case 36: return launch_green_2d<P, 36, 8, 16>(h, a, g);
+    case 40: return launch_green_2d<P, 40, 8, 16>(h, a, g);
+    case 54: return launch_green_2d<P, 54, 16, 16>(h, a, g);
+    case 72: return launch_green_2d<P, 72, 16, 16>(h, a, g);
+    case 144: return launch_green_2d<P, 144, 32, 16>(h, a, g);
     default: break;
     }
     if (h->d <= 24) return launch_green_t<P, 64>(h, a, g);

@@ -58,6 +58,8 @@ struct th_handle {
     hipEvent_t ev[8] = {};
     double t_ms[5] = {0, 0, 0, 0, 0};
     long n_launch[4] = {0, 0, 0, 0};
+    int kernel_variant = 0;      // 0 default kernels, 1 alternates (tuning / comparison)
+    int batch_mb = 512;          // panel budget per batch (MB)
 };
 
 // error plumbing
