@@ -66,10 +66,12 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
 
     // XCD-aware order: consecutive logical ids (the tiles of one split) land
     // on the same XCD, whose L2 then serves their shared panel rows.
+    // (workgroups are dispatched round-robin over the 8 XCDs: b % 8 is the XCD;
+    // the tail beyond a multiple of 8 keeps its own index)
     const int nwg = gridDim.x;
     const int b = blockIdx.x;
-    int lid = b;
-    if ((nwg & 7) == 0) lid = (b & 7) * (nwg >> 3) + (b >> 3);
+    const int n8 = nwg & ~7;
+    const int lid = (b < n8) ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
     const int tile = lid % (ntile * ntile);
     const int rest = lid / (ntile * ntile);
     const int split = rest % splits;
