@@ -44,9 +44,9 @@ typedef struct th_handle th_handle;
 
 /* Precision modes of th_chi0 / th_chi_sweep. */
 #define TH_PREC_FP64   0     /* G, chi0 contraction and sums in FP64 */
-#define TH_PREC_MIXED  1     /* G in FP32 + one FP64-residual refinement step, FP32 MFMA
-                                contraction with FP32 in-tile accumulation, FP64 contour
-                                sums and FP64 Dyson */
+#define TH_PREC_MIXED  1     /* G inverted in FP64 and rounded once to FP32 GEMM panels,
+                                FP32 MFMA contraction with FP32 in-tile accumulation,
+                                FP64 contour sums across tiles and splits, FP64 Dyson */
 
 /* Creates a handle on the current HIP device and uploads the tight-binding
  * model: the local Hamiltonian h0 (d x d complex, d = 2*nOrb*nA) built by
@@ -100,6 +100,26 @@ int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out,
 int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega, double Ef,
                  double eta, double etap, int parts3, int n3gl, const double *U, int precision,
                  double *schi0_out, double *schi_out, double *cond_out, double *chi_full_out);
+
+/* Adjoint (reverse-mode derivative) of one frequency of the chi path.  No
+ * counterpart in the reference, which has no derivatives; it differentiates
+ * th_chi0 followed by th_dyson with respect to the model th_create uploaded
+ * (hops t, local Hamiltonian h0) and to U.  Given the cotangent chi_hat of
+ * the output (dL/dRe + i dL/dIm of a real loss L, D x D column-major), it
+ * returns
+ *   t_hat   nhop blocks nOrb x nOrb, same layout and hop order as th_create's t,
+ *   h0_hat  d x d,
+ *   U_hat   D x D (may be NULL),
+ * in the same convention.  chi0 and chi are the forward results at this
+ * omega (from th_chi0 and th_dyson with the same arguments).  With U == NULL
+ * the Dyson step is skipped: chi_hat is then the cotangent of chi0 and chi is
+ * not read.  The Green functions are recomputed, never stored; 'precision'
+ * selects the panel precision of the two adjoint contractions as in
+ * th_chi0 (the Green functions, the adjoint of H and every sum are FP64). */
+int th_chi_adjoint(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
+                   int n_x, const double *x2, const double *p2, int precision,
+                   const double *chi0, const double *U, const double *chi, const double *chi_hat,
+                   double *t_hat, double *h0_hat, double *U_hat);
 
 /* Releases every device buffer of the handle. */
 int th_destroy(th_handle *h);

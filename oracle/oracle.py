@@ -57,6 +57,8 @@ def lib():
         L.or_lmat.argtypes = [C.c_void_p]
         L.or_build_system.argtypes = [C.POINTER(_SysIn), C.POINTER(_Sys)]
         L.or_free_system.argtypes = [C.POINTER(_Sys)]
+        L.or_sys_from_arrays.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp, C.c_void_p, _dp,
+                                         C.POINTER(_Sys)]
         L.or_sys_export.argtypes = [C.POINTER(_Sys), _dp, _ip, _ip, _dp, C.c_void_p, C.c_void_p, _dp]
         L.or_hk.argtypes = [C.POINTER(_Sys), _dp, C.c_void_p]
         L.or_zinv.argtypes = [C.c_int, C.c_void_p]
@@ -170,6 +172,31 @@ class OracleSystem:
         self.nA, self.nOrb, self.d, self.D = nA, nO, self._s.d, self._s.D
         self.nhop = self._s.nhop
         self.Ef = float(spec["Ef"])
+
+    @classmethod
+    def from_arrays(cls, nA, nOrb, R, jb, ib, t, h0, b, Ef):
+        """A system given by its tables: t[nhop, nO, nO] real (rows = orbitals
+        of jb), h0[d, d] complex, b = reciprocal basis rows.  For derivative
+        checks that perturb the tables directly."""
+        self = cls.__new__(cls)
+        self._keep = []
+        R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+        nhop = R.shape[0]
+        jb = np.ascontiguousarray(jb, dtype=np.int32)
+        ib = np.ascontiguousarray(ib, dtype=np.int32)
+        tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).transpose(0, 2, 1))
+        hh = np.asfortranarray(h0, dtype=np.complex128)
+        bb = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        self._keep += [R, jb, ib, tt, hh, bb]
+        self._s = _Sys()
+        rc = lib().or_sys_from_arrays(nA, nOrb, nhop, _d(R), _i(jb), _i(ib), _d(tt),
+                                      hh.ctypes.data_as(C.c_void_p), _d(bb), C.byref(self._s))
+        if rc != 0:
+            raise RuntimeError("or_sys_from_arrays failed")
+        self.nA, self.nOrb, self.d, self.D = nA, nOrb, self._s.d, self._s.D
+        self.nhop = nhop
+        self.Ef = float(Ef)
+        return self
 
     def __del__(self):
         try:

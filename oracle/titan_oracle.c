@@ -503,6 +503,32 @@ void or_free_system(or_sys *s)
     memset(s, 0, sizeof(*s));
 }
 
+/* A system from explicit tables (hops t real, nhop*nOrb*nOrb column-major,
+ * h0 d*d), for derivative checks that perturb the tables themselves.  The
+ * interaction matrix is left zero (or_dyson takes U separately). */
+int or_sys_from_arrays(int nA, int nOrb, int nhop, const double *R, const int *jb, const int *ib,
+                       const double *t, const zc *h0, const double *b, or_sys *out)
+{
+    memset(out, 0, sizeof(*out));
+    out->nA = nA; out->nOrb = nOrb; out->d = 2 * nOrb * nA; out->nhop = nhop;
+    out->D = 4 * nOrb * nOrb * nA;
+    const int d = out->d;
+    out->R = malloc(sizeof(double) * 3 * (nhop > 0 ? nhop : 1));
+    out->jb = malloc(sizeof(int) * (nhop > 0 ? nhop : 1));
+    out->ib = malloc(sizeof(int) * (nhop > 0 ? nhop : 1));
+    out->t = malloc(sizeof(double) * (nhop > 0 ? nhop : 1) * nOrb * nOrb);
+    out->h0 = malloc(sizeof(zc) * d * d);
+    out->Umat = calloc((size_t)out->D * out->D, sizeof(zc));
+    if (!out->R || !out->jb || !out->ib || !out->t || !out->h0 || !out->Umat) return -1;
+    memcpy(out->R, R, sizeof(double) * 3 * nhop);
+    memcpy(out->jb, jb, sizeof(int) * nhop);
+    memcpy(out->ib, ib, sizeof(int) * nhop);
+    memcpy(out->t, t, sizeof(double) * nhop * nOrb * nOrb);
+    memcpy(out->h0, h0, sizeof(zc) * d * d);
+    memcpy(out->b, b, sizeof(double) * 9);
+    return 0;
+}
+
 /* Copies of the built tables for inspection from Python. */
 int or_sys_nhop(const or_sys *s) { return s->nhop; }
 void or_sys_export(const or_sys *s, double *R, int *jb, int *ib, double *t, zc *h0, zc *Umat, double *b)

@@ -20,8 +20,8 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_destroy", "th_last_error", "th_limits", "th_last_timing", "th_tune",
-           "th_get_tuning")
+           "th_chi_sweep", "th_chi_adjoint", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
+           "th_tune", "th_get_tuning")
 
 
 class TitanHipError(RuntimeError):
@@ -44,6 +44,8 @@ def load():
     L.th_dyson.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.th_chi_sweep.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]
+    L.th_chi_adjoint.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                 _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.th_destroy.argtypes = [C.c_void_p]
     L.th_last_error.restype = C.c_char_p
     L.th_limits.argtypes = [_ip, _ip, _ip]
@@ -90,6 +92,7 @@ class Handle:
         tt = np.ascontiguousarray(np.asarray(t, dtype=np.complex128).transpose(0, 2, 1))
         h0f = _c128(h0)
         self.nA, self.nOrb = nA, nOrb
+        self.nhop = nhop
         self.d = 2 * nOrb * nA
         self.D = 4 * nOrb * nOrb * nA
         self._h = C.c_void_p()
@@ -162,6 +165,37 @@ class Handle:
         s0 = s0.transpose(0, 2, 1).copy()
         s = s.transpose(0, 2, 1).copy()
         return (s0, s, cond, chi) if full else (s0, s, cond)
+
+    def adjoint(self, q, omega, Ef, eta, etap, chi0, chi_hat, U=None, chi=None, x2=None, p2=None,
+                precision=PREC_FP64, want_U=True):
+        """Reverse-mode derivative of chi0 (U is None) or of chi = Dyson(chi0, U)
+        at one omega.  chi_hat = dL/dRe chi + i dL/dIm chi (D x D).  Returns
+        (t_hat [nhop, nOrb, nOrb] with rows = orbitals of jb, h0_hat [d, d],
+        U_hat [D, D] or None) in the same convention."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if x2 is None:
+            x2 = np.zeros(1)
+            p2 = np.zeros(1)
+            nx = 0
+        else:
+            x2 = np.ascontiguousarray(x2, dtype=np.float64)
+            p2 = np.ascontiguousarray(p2, dtype=np.float64)
+            nx = len(x2)
+        c0 = _c128(chi0)
+        ch = _c128(chi_hat)
+        u = _c128(U) if U is not None else None
+        cf = _c128(chi) if U is not None else None
+        nh = self.nhop
+        th = np.zeros(nh * self.nOrb * self.nOrb, dtype=np.complex128)
+        hh = np.zeros((self.d, self.d), dtype=np.complex128, order="F")
+        uh = np.zeros((self.D, self.D), dtype=np.complex128, order="F") if (U is not None and want_U) else None
+        ptr = lambda a: a.ctypes.data_as(_dp) if a is not None else None
+        rc = load().th_chi_adjoint(self._h, _d(q), omega, Ef, eta, etap, nx, _d(x2), _d(p2), precision,
+                                   ptr(c0), ptr(u), ptr(cf), ptr(ch), ptr(th), ptr(hh), ptr(uh))
+        _check(rc, "th_chi_adjoint")
+        # each hop block is column-major (rows = orbitals of jb)
+        t_hat = th.reshape(nh, self.nOrb, self.nOrb).transpose(0, 2, 1).copy()
+        return t_hat, np.ascontiguousarray(hh), (np.ascontiguousarray(uh) if uh is not None else None)
 
     def tune(self, **kw):
         for k, v in kw.items():

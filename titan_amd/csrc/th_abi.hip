@@ -58,7 +58,7 @@ int *th_status_word(th_handle *h)
     return nullptr;
 }
 
-static int reset_status(th_handle *h)
+int th_reset_status(th_handle *h)
 {
     const int v = INT_MAX;
     TH_HIP(hipMemcpyAsync(th_status_word(h), &v, sizeof(int), hipMemcpyHostToDevice, h->stream));
@@ -79,40 +79,6 @@ int th_check_status(th_handle *h)
 
 namespace {
 
-// stage timing with HIP events on the handle's stream
-struct stage_timer {
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
-    th_handle *h;
-    explicit stage_timer(th_handle *hh) : h(hh) {}
-    int begin(int stage)
-    {
-        hipEvent_t a, b;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
-        hipEventRecord(a, h->stream);
-        ev.push_back({stage, {a, b}});
-        return (int)ev.size() - 1;
-    }
-    void end(int id)
-    {
-        if (id >= 0) hipEventRecord(ev[id].second.second, h->stream);
-    }
-    void collect()
-    {
-        for (int s = 0; s < 5; ++s) h->t_ms[s] = 0.0;
-        for (auto &e : ev) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) h->t_ms[e.first] += ms;
-        }
-    }
-    ~stage_timer()
-    {
-        for (auto &e : ev) {
-            hipEventDestroy(e.second.first);
-            hipEventDestroy(e.second.second);
-        }
-    }
-};
-
 size_t prec_size(int prec) { return prec == TH_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
 int ensure_panels(th_handle *h, long pairs)
@@ -128,6 +94,29 @@ int ensure_panels(th_handle *h, long pairs)
     h->panel_bytes = need;
     return TH_OK;
 }
+
+}  // namespace
+
+int th_ensure_panels(th_handle *h, long pairs) { return ensure_panels(h, pairs); }
+
+// real-axis energy nodes of the current omega to the device
+int th_upload_nodes(th_handle *h, int n_x, const double *x2, const double *p2)
+{
+    if (n_x > h->nx_cap) {
+        hipFree(h->d_x2);
+        hipFree(h->d_p2);
+        h->d_x2 = h->d_p2 = nullptr;
+        h->nx_cap = 0;
+        TH_HIP(hipMalloc(&h->d_x2, sizeof(double) * n_x));
+        TH_HIP(hipMalloc(&h->d_p2, sizeof(double) * n_x));
+        h->nx_cap = n_x;
+    }
+    TH_HIP(hipMemcpyAsync(h->d_x2, x2, sizeof(double) * n_x, hipMemcpyHostToDevice, h->stream));
+    TH_HIP(hipMemcpyAsync(h->d_p2, p2, sizeof(double) * n_x, hipMemcpyHostToDevice, h->stream));
+    return TH_OK;
+}
+
+namespace {
 
 long batch_pairs(th_handle *h, long total)
 {
@@ -195,15 +184,8 @@ int chi0_device(th_handle *h, const double *q, double omega, double Ef, double e
         return TH_E_STATE;
     }
     if (real_axis) {
-        if (n_x > h->nx_cap) {
-            hipFree(h->d_x2);
-            hipFree(h->d_p2);
-            TH_HIP(hipMalloc(&h->d_x2, sizeof(double) * n_x));
-            TH_HIP(hipMalloc(&h->d_p2, sizeof(double) * n_x));
-            h->nx_cap = n_x;
-        }
-        TH_HIP(hipMemcpyAsync(h->d_x2, x2, sizeof(double) * n_x, hipMemcpyHostToDevice, h->stream));
-        TH_HIP(hipMemcpyAsync(h->d_p2, p2, sizeof(double) * n_x, hipMemcpyHostToDevice, h->stream));
+        int rc0 = th_upload_nodes(h, n_x, x2, p2);
+        if (rc0) return rc0;
     }
     const size_t nacc = (size_t)2 * h->nA2 * 2 * h->Mpad * h->Mpad;
     int rc = th_launch_zero(h, h->d_acc, nacc);
@@ -334,6 +316,7 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return jb[a] * nA + ib[a] < jb[b] * nA + ib[b]; });
     for (int k = 0; k < nA * nA; ++k) hoff[k + 1] += hoff[k];
+    h->hop_order = order;
     TH_HIP(hipMalloc(&h->d_hoff, sizeof(int) * (nA * nA + 1)));
     TH_HIP(hipMemcpy(h->d_hoff, hoff.data(), sizeof(int) * (nA * nA + 1), hipMemcpyHostToDevice));
     if (nhop > 0) {
@@ -408,7 +391,7 @@ int th_chi0(th_handle *h, const double *q, double omega, double Ef, double eta, 
     stage_timer tm(h);
     int id = tm.begin(4);
     for (auto &c : h->n_launch) c = 0;
-    int rc = reset_status(h);
+    int rc = th_reset_status(h);
     if (!rc) rc = chi0_device(h, q, omega, Ef, eta, etap, n_x, x2, p2, precision, tm);
     tm.end(id);
     if (rc) return rc;
@@ -423,7 +406,7 @@ int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out,
 {
     if (!h || bad(chi0) || bad(U) || bad(chi_out)) { th_set_error("th_dyson: bad arguments"); return TH_E_ARG; }
     const size_t n = (size_t)h->D * h->D;
-    int rc = reset_status(h);
+    int rc = th_reset_status(h);
     if (rc) return rc;
     TH_HIP(hipMemcpyAsync(h->d_chi0, chi0, sizeof(double2) * n, hipMemcpyHostToDevice, h->stream));
     stage_timer tm(h);
@@ -453,7 +436,7 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
     stage_timer tm(h);
     int idall = tm.begin(4);
     for (auto &c : h->n_launch) c = 0;
-    int rc = reset_status(h);
+    int rc = th_reset_status(h);
     for (int w = 0; w < n_omega && !rc; ++w) {
         const double e = omega[w];
         if (std::fabs(e) >= 1e-15)
@@ -542,7 +525,7 @@ int th_destroy(th_handle *h)
     hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
-                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X};
+                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (size_t i = 0; i < g_status.size(); ++i)
@@ -565,18 +548,9 @@ int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, d
     if (!h || np < 1 || bad(q) || bad(out)) { th_set_error("thx_green_panel: bad arguments"); return TH_E_ARG; }
     long total = (mode <= TH_G_IMAG_B) ? h->n_im : (long)n_x * h->nk_re;
     if (p0 < 0 || p0 + np > total) { th_set_error("thx_green_panel: pair range"); return TH_E_ARG; }
-    int rc = reset_status(h);
+    int rc = th_reset_status(h);
     if (rc) return rc;
-    if (mode >= TH_G_REAL_A) {
-        if (n_x > h->nx_cap) {
-            hipFree(h->d_x2); hipFree(h->d_p2);
-            TH_HIP(hipMalloc(&h->d_x2, sizeof(double) * n_x));
-            TH_HIP(hipMalloc(&h->d_p2, sizeof(double) * n_x));
-            h->nx_cap = n_x;
-        }
-        TH_HIP(hipMemcpy(h->d_x2, x2, sizeof(double) * n_x, hipMemcpyHostToDevice));
-        TH_HIP(hipMemcpy(h->d_p2, p2, sizeof(double) * n_x, hipMemcpyHostToDevice));
-    }
+    if (mode >= TH_G_REAL_A && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
     if ((rc = ensure_panels(h, np))) return rc;
     th_green_args a{};
     a.mode = mode; a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];

@@ -1,0 +1,725 @@
+// th_adjoint.hip -- reverse-mode derivative of the chi(omega) path on gfx950.
+//
+// Forward, per frequency: H(k) from the hops t_n and h0, G = (z - H)^-1 for
+// every (k,E) pair, the contraction T = sum_p A_p B_p^T of the G panels,
+// chi0 = finalize(T), and chi = (I + chi0 U)^-1 chi0.  The reference has no
+// derivatives; this file runs the chain backwards for a real loss L with
+// every cotangent written as xhat = dL/dRe x + i dL/dIm x:
+//   Dyson      chi0hat = M^-H chihat N^H, Uhat = -chi0^H M^-H chihat chi^H,
+//              M = I + chi0 U, N = I - U chi, with M^-H applied through the
+//              support S of U (the push-through identity of the forward);
+//   finalize   That[b1(X,Y)] = chi0hat[X,Y]/2pi, That[b2(X,Y)] += conj(..)/2pi
+//              (imaginary axis), Fhat[b1] = chi0hat/2pi (real axis);
+//   contract   Ahat_p = That conj(B_p), Bhat_p = That^T conj(A_p) (MFMA,
+//              th_gemm.hip), then Ghat of each Green function from the panel
+//              formulas (weights, the real-axis Gq - Gq^H and G^H);
+//   inversion  Hhat = G^H Ghat G^H for both Green functions of the pair;
+//   assembly   that_n = sum_p exp(-i k_p.R_n) sum_s Hhat_p[(jb,s,nu),(ib,s,mu)],
+//              h0hat = sum_p Hhat_p.
+// Green functions are recomputed batch by batch (never stored across
+// batches).  All sums are FP64 and run in a fixed order, so the result is
+// bitwise reproducible.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "th_internal.h"
+
+namespace {
+
+__device__ inline double2 zmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+
+inline unsigned nblk(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// ---------------------------------------------------------------------------
+// Dyson adjoint: dense complex products, column-major.
+// C = alpha op(A) op(B) + beta C, op 0 = as is, 1 = conjugate transpose.
+// A 16 x 16 tile per 256-thread workgroup, K staged through LDS by 16; the
+// loads follow the memory order of each operand.
+__global__ __launch_bounds__(256) void zgemm_kernel(int opA, int opB, int m, int n, int k, double2 alpha,
+                                                    const double2 *__restrict__ A, int lda,
+                                                    const double2 *__restrict__ B, int ldb, double2 beta,
+                                                    double2 *__restrict__ C, int ldc)
+{
+    __shared__ double2 As[16][17], Bs[16][17];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+    const int r = t & 15, c = t >> 4;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < k; k0 += 16) {
+        {   // As[row][kk] = op(A)[i0+row, k0+kk]
+            int row, kk;
+            if (opA == 0) { row = t & 15; kk = t >> 4; } else { kk = t & 15; row = t >> 4; }
+            const int gi = i0 + row, gk = k0 + kk;
+            double2 v = make_double2(0.0, 0.0);
+            if (gi < m && gk < k) v = (opA == 0) ? A[gi + (size_t)lda * gk] : zconj(A[gk + (size_t)lda * gi]);
+            As[row][kk] = v;
+        }
+        {   // Bs[kk][col] = op(B)[k0+kk, j0+col]
+            int kk, col;
+            if (opB == 0) { kk = t & 15; col = t >> 4; } else { col = t & 15; kk = t >> 4; }
+            const int gk = k0 + kk, gj = j0 + col;
+            double2 v = make_double2(0.0, 0.0);
+            if (gj < n && gk < k) v = (opB == 0) ? B[gk + (size_t)ldb * gj] : zconj(B[gj + (size_t)ldb * gk]);
+            Bs[kk][col] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) acc = zadd(acc, zmul(As[r][kk], Bs[kk][c]));
+        __syncthreads();
+    }
+    const int gi = i0 + r, gj = j0 + c;
+    if (gi < m && gj < n) {
+        double2 *o = C + gi + (size_t)ldc * gj;
+        double2 v = zmul(alpha, acc);
+        if (beta.x != 0.0 || beta.y != 0.0) v = zadd(v, zmul(beta, *o));
+        *o = v;
+    }
+}
+
+// out[x + D a] = M[x + D S[a]]   (D x r)
+__global__ void gather_cols(const double2 *M, int D, const int *S, int r, double2 *out)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * r) return;
+    size_t x = e % D, a = e / D;
+    out[e] = M[x + (size_t)D * S[a]];
+}
+
+// out[a + r y] = M[S[a] + D y]   (r x D)
+__global__ void gather_rows(const double2 *M, int D, const int *S, int r, double2 *out)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * r) return;
+    size_t a = e % r, y = e / r;
+    out[e] = M[S[a] + (size_t)D * y];
+}
+
+// X[S[a] + D y] -= Z[a + r y]   (rows of S are distinct: no two threads share an entry)
+__global__ void scatter_sub_rows(double2 *X, int D, const int *S, int r, const double2 *Z)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * r) return;
+    size_t a = e % r, y = e / r;
+    double2 *o = X + S[a] + (size_t)D * y;
+    *o = make_double2(o->x - Z[e].x, o->y - Z[e].y);
+}
+
+// X[x + D S[b]] -= Q[x + D b]
+__global__ void scatter_sub_cols(double2 *X, int D, const int *S, int r, const double2 *Q)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * r) return;
+    size_t x = e % D, b = e / D;
+    double2 *o = X + x + (size_t)D * S[b];
+    *o = make_double2(o->x - Q[e].x, o->y - Q[e].y);
+}
+
+// W = I + chi0[S,S] U_S  (r x r, column-major) and its inverse by
+// Gauss-Jordan with partial pivoting in one workgroup (r is a few hundred)
+__global__ void form_w_adj(const double2 *chi0, int D, const int *S, const double2 *US, int r, double2 *W)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= r * r) return;
+    int a = e % r, b = e / r;
+    double2 s = make_double2(a == b ? 1.0 : 0.0, 0.0);
+    for (int c = 0; c < r; ++c) s = zadd(s, zmul(chi0[S[a] + (size_t)D * S[c]], US[c + (size_t)r * b]));
+    W[e] = s;
+}
+
+__global__ __launch_bounds__(1024) void gj_inverse(double2 *W, int r, int *perm, int *status)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2 *colk = reinterpret_cast<double2 *>(smem);
+    double2 *prow = colk + r;
+    __shared__ double redv[1024];
+    __shared__ int redi[1024];
+    __shared__ int s_p;
+    const int tid = threadIdx.x, NT = blockDim.x;
+#define Wm(a, b) W[(a) + (size_t)r * (b)]
+    for (int k = 0; k < r; ++k) {
+        double v = -1.0;
+        int ix = r;
+        for (int t = k + tid; t < r; t += NT) {
+            double a = fabs(Wm(t, k).x) + fabs(Wm(t, k).y);
+            if (a > v) { v = a; ix = t; }
+        }
+        redv[tid] = v;
+        redi[tid] = ix;
+        __syncthreads();
+        for (int off = NT / 2; off > 0; off >>= 1) {
+            if (tid < off) {
+                double ov = redv[tid + off];
+                int oi = redi[tid + off];
+                if (ov > redv[tid] || (ov == redv[tid] && oi < redi[tid])) { redv[tid] = ov; redi[tid] = oi; }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) { s_p = redi[0]; perm[k] = redi[0]; }
+        __syncthreads();
+        const int p = s_p;
+        double2 piv = Wm(p, k);
+        __syncthreads();
+        for (int t = tid; t < r; t += NT) {
+            double2 rk = Wm(k, t), rp = Wm(p, t);
+            colk[t] = (t == k) ? Wm(p, k) : (t == p ? Wm(k, k) : Wm(t, k));
+            prow[t] = rp;
+            if (p != k) Wm(p, t) = rk;
+        }
+        __syncthreads();
+        if (piv.x == 0.0 && piv.y == 0.0) {
+            if (tid == 0) atomicMin(status, k + 1);
+            piv = make_double2(1.0, 0.0);
+        }
+        const double nn = piv.x * piv.x + piv.y * piv.y;
+        const double2 inv = make_double2(piv.x / nn, -piv.y / nn);
+        for (int t = tid; t < r; t += NT) {
+            double2 pr = (t == k) ? inv : zmul(prow[t], inv);
+            prow[t] = pr;
+            Wm(k, t) = pr;
+        }
+        __syncthreads();
+        for (size_t e = tid; e < (size_t)r * r; e += NT) {
+            int a = (int)(e % r), c = (int)(e / r);
+            if (a == k) continue;
+            double2 f = colk[a];
+            if (c == k) {
+                double2 q = zmul(f, inv);
+                Wm(a, c) = make_double2(-q.x, -q.y);
+            } else {
+                double2 q = zmul(f, prow[c]);
+                Wm(a, c) = make_double2(Wm(a, c).x - q.x, Wm(a, c).y - q.y);
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = r - 1; k >= 0; --k) {
+        int pk = perm[k];
+        if (pk != k)
+            for (int a = tid; a < r; a += NT) {
+                double2 t0 = Wm(a, k);
+                Wm(a, k) = Wm(a, pk);
+                Wm(a, pk) = t0;
+            }
+        __syncthreads();
+    }
+#undef Wm
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of finalize_kernel (th_gemm.hip): chi0 = (T[b1] + conj(T[b2]) +
+// F[b1]) / 2pi with the index maps below; b1 and b2 are bijections of the
+// chi index pairs onto the unpadded accumulator entries, so each pass
+// writes every entry once.
+__device__ inline void chi_maps(int X, int Y, int nA, int nOrb, int Mpad, size_t &b1, size_t &b2)
+{
+    const int bs = 2 * nOrb;
+    int s1 = X & 3, r = X >> 2;
+    const int nu = r % nOrb; r /= nOrb;
+    const int mu = r % nOrb;
+    const int i = r / nOrb;
+    int s2 = Y & 3;
+    r = Y >> 2;
+    const int xi = r % nOrb; r /= nOrb;
+    const int ga = r % nOrb;
+    const int j = r / nOrb;
+    const int a1 = (s1 == 0 || s1 == 2) ? 1 : 0, c1 = (s1 >= 2) ? 1 : 0;
+    const int a2 = (s2 == 0 || s2 == 2) ? 1 : 0, c2 = (s2 >= 2) ? 1 : 0;
+    const size_t MM = (size_t)Mpad * Mpad;
+    const int rr = (a1 * nOrb + nu) * bs + (a2 * nOrb + ga);
+    const int cc = (c2 * nOrb + xi) * bs + (c1 * nOrb + mu);
+    b1 = (size_t)(i * nA + j) * 2 * MM + (size_t)rr * Mpad + cc;
+    const int rr2 = (a2 * nOrb + ga) * bs + (a1 * nOrb + nu);
+    const int cc2 = (c1 * nOrb + mu) * bs + (c2 * nOrb + xi);
+    b2 = (size_t)(j * nA + i) * 2 * MM + (size_t)rr2 * Mpad + cc2;
+}
+
+__global__ void unfinalize_1(const double2 *__restrict__ chat, double *__restrict__ acc, int nA, int nOrb, int Mpad,
+                             int D, int real_axis)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * D) return;
+    size_t b1, b2;
+    chi_maps((int)(e % D), (int)(e / D), nA, nOrb, Mpad, b1, b2);
+    const double inv2pi = 0.15915494309189533577;
+    const size_t MM = (size_t)Mpad * Mpad, axis = (size_t)nA * nA * 2 * MM;
+    const double2 v = chat[e];
+    acc[b1] = v.x * inv2pi;
+    acc[b1 + MM] = v.y * inv2pi;
+    if (real_axis) {
+        acc[axis + b1] = v.x * inv2pi;
+        acc[axis + b1 + MM] = v.y * inv2pi;
+    }
+}
+
+__global__ void unfinalize_2(const double2 *__restrict__ chat, double *__restrict__ acc, int nA, int nOrb, int Mpad,
+                             int D)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)D * D) return;
+    size_t b1, b2;
+    chi_maps((int)(e % D), (int)(e / D), nA, nOrb, Mpad, b1, b2);
+    const double inv2pi = 0.15915494309189533577;
+    const size_t MM = (size_t)Mpad * Mpad;
+    const double2 v = chat[e];
+    acc[b2] += v.x * inv2pi;
+    acc[b2 + MM] -= v.y * inv2pi;
+}
+
+// ---------------------------------------------------------------------------
+// Ghat (row-major d x d per pair) from the panel cotangent of one side.
+struct assemble_params {
+    int mode, nA, bs, d, Mpad, nA2;
+    long p0, np, nk_re;
+    const double *wim, *p2, *wre;
+    const double *hat;       // [np][nA2][2][Mpad]
+    double2 *out;            // [np][d][d]
+};
+
+__global__ void assemble_kernel(assemble_params a)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t dd = (size_t)a.d * a.d;
+    if (e >= (size_t)a.np * dd) return;
+    const long pl = (long)(e / dd);
+    const int rc = (int)(e - pl * dd);
+    const int r = rc / a.d, c = rc - (rc / a.d) * a.d;
+    const int bs = a.bs, nA = a.nA, Mpad = a.Mpad;
+    const int ra = r / bs, ca = c / bs, rr = r - ra * bs, cc = c - ca * bs;
+    const double *H = a.hat + (size_t)pl * a.nA2 * 2 * Mpad;
+    auto hat = [&](int blk, int idx) { return make_double2(H[(size_t)blk * 2 * Mpad + idx], H[(size_t)blk * 2 * Mpad + Mpad + idx]); };
+    const long p = a.p0 + pl;
+    double2 v;
+    if (a.mode == TH_G_IMAG_A) {            // A = w Gq(i,j):  Gqhat = w Ahat
+        const double w = a.wim[p];
+        const double2 x = hat(ra * nA + ca, rr * bs + cc);
+        v = make_double2(w * x.x, w * x.y);
+    } else if (a.mode == TH_G_IMAG_B) {     // B(i,j) = G(j,i)
+        v = hat(ca * nA + ra, rr * bs + cc);
+    } else if (a.mode == TH_G_REAL_A) {     // A = -i w (Gq - Gq^H):  Gqhat = i w (Ahat + Ahat^H)
+        const long ie = p / a.nk_re, ik = p - ie * a.nk_re;
+        const double w = a.p2[ie] * a.wre[ik];
+        const double2 x = hat(ra * nA + ca, rr * bs + cc), y = hat(ca * nA + ra, cc * bs + rr);
+        const double sr = x.x + y.x, si = x.y - y.y;
+        v = make_double2(-w * si, w * sr);
+    } else {                                // B(i,j) = G^H(j,i):  Ghat(i,j)[cc][rr] = conj(Bhat(i,j)[rr][cc])
+        v = zconj(hat(ra * nA + ca, cc * bs + rr));
+    }
+    a.out[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Batched C_p = op(A_p) op(B_p), d x d row-major FP64 complex per pair,
+// op 1 = conjugate transpose.  A 32 x 32 output tile per 256-thread
+// workgroup (2 x 2 entries per thread), K staged through LDS by 16.
+__global__ __launch_bounds__(256) void bmm_kernel(const double2 *__restrict__ A, int opA,
+                                                  const double2 *__restrict__ B, int opB, double2 *__restrict__ C,
+                                                  int d, int ntile)
+{
+    __shared__ double2 As[32][17], Bs[16][33];
+    const int t = threadIdx.x;
+    const long pl = blockIdx.x / (ntile * ntile);
+    const int tile = (int)(blockIdx.x - pl * ntile * ntile);
+    const int i0 = (tile / ntile) * 32, j0 = (tile % ntile) * 32;
+    const size_t dd = (size_t)d * d;
+    const double2 *Ap = A + pl * dd, *Bp = B + pl * dd;
+    const int tx = t & 15, ty = t >> 4;
+    double2 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < d; k0 += 16) {
+        for (int u = 0; u < 2; ++u) {
+            const int v = t + 256 * u;
+            int row, kk;
+            if (opA == 0) { kk = v & 15; row = v >> 4; } else { row = v & 31; kk = v >> 5; }
+            const int gi = i0 + row, gk = k0 + kk;
+            double2 x = make_double2(0.0, 0.0);
+            if (gi < d && gk < d) x = (opA == 0) ? Ap[(size_t)gi * d + gk] : zconj(Ap[(size_t)gk * d + gi]);
+            As[row][kk] = x;
+        }
+        for (int u = 0; u < 2; ++u) {
+            const int v = t + 256 * u;
+            int kk, col;
+            if (opB == 0) { col = v & 31; kk = v >> 5; } else { kk = v & 15; col = v >> 4; }
+            const int gk = k0 + kk, gj = j0 + col;
+            double2 x = make_double2(0.0, 0.0);
+            if (gj < d && gk < d) x = (opB == 0) ? Bp[(size_t)gk * d + gj] : zconj(Bp[(size_t)gj * d + gk]);
+            Bs[kk][col] = x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const double2 a0 = As[ty][kk], a1 = As[ty + 16][kk];
+            const double2 b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+            acc[0][0] = zadd(acc[0][0], zmul(a0, b0));
+            acc[0][1] = zadd(acc[0][1], zmul(a0, b1));
+            acc[1][0] = zadd(acc[1][0], zmul(a1, b0));
+            acc[1][1] = zadd(acc[1][1], zmul(a1, b1));
+        }
+        __syncthreads();
+    }
+    double2 *Cp = C + pl * dd;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int gi = i0 + ty + 16 * a, gj = j0 + tx + 16 * b;
+            if (gi < d && gj < d) Cp[(size_t)gi * d + gj] = acc[a][b];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Partial sums over a chunk of pairs of the hop and h0 cotangents:
+//   that[n][nu + nOrb mu] = sum_p exp(-i k_p.R_n) sum_s Hhat_p[(2jb+s)nO+nu][(2ib+s)nO+mu]
+//   h0hat[r + d c]        = sum_p Hhat_p[r][c]
+// Output e < nhop nO^2 is a hop entry, the rest the h0 entries.
+struct hreduce_params {
+    int side_q;              // 1: k + q (the Gq side)
+    int imag;                // 1: imaginary-axis pair list
+    long p0, np, nk_re;
+    int chunk, nout, nhop, nOrb, d;
+    double q0, q1, q2;
+    const double *kim, *kre, *R;
+    const int *jb, *ib;
+    const double2 *Hh;       // [np][d][d]
+    double2 *partial;        // [nchunk][nout]
+};
+
+__global__ void hreduce_kernel(hreduce_params a)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nout) return;
+    const long c0 = (long)blockIdx.y * a.chunk, c1 = min(a.np, c0 + a.chunk);
+    const int nO = a.nOrb, nO2 = nO * nO, d = a.d;
+    const size_t dd = (size_t)d * d;
+    double2 s = make_double2(0.0, 0.0);
+    if (e < a.nhop * nO2) {
+        const int n = e / nO2, loc = e - n * nO2;
+        const int nu = loc % nO, mu = loc / nO;
+        const int j = a.jb[n], i = a.ib[n];
+        const double R0 = a.R[3 * n], R1 = a.R[3 * n + 1], R2 = a.R[3 * n + 2];
+        const size_t o0 = (size_t)(2 * j * nO + nu) * d + 2 * i * nO + mu;
+        const size_t o1 = (size_t)((2 * j + 1) * nO + nu) * d + (2 * i + 1) * nO + mu;
+        for (long pl = c0; pl < c1; ++pl) {
+            const long p = a.p0 + pl;
+            const double *k = a.imag ? a.kim + 3 * p : a.kre + 3 * (p % a.nk_re);
+            double k0 = k[0], k1 = k[1], k2 = k[2];
+            if (a.side_q) { k0 += a.q0; k1 += a.q1; k2 += a.q2; }
+            double sn, cs;
+            sincos(k0 * R0 + k1 * R1 + k2 * R2, &sn, &cs);
+            const double2 *Hp = a.Hh + pl * dd;
+            const double2 h = zadd(Hp[o0], Hp[o1]);
+            s = zadd(s, zmul(h, make_double2(cs, -sn)));
+        }
+    } else {
+        const int f = e - a.nhop * nO2;
+        const int r = f % d, c = f / d;
+        for (long pl = c0; pl < c1; ++pl) s = zadd(s, a.Hh[pl * dd + (size_t)r * d + c]);
+    }
+    a.partial[(size_t)blockIdx.y * a.nout + e] = s;
+}
+
+// acc[e] += sum_ch partial[ch][e], fixed order
+__global__ void hsum_kernel(const double2 *partial, int nch, int nout, double2 *acc)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nout) return;
+    double2 s = acc[e];
+    for (int ch = 0; ch < nch; ++ch) s = zadd(s, partial[(size_t)ch * nout + e]);
+    acc[e] = s;
+}
+
+__global__ void zero2_kernel(double2 *p, size_t n)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = make_double2(0.0, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// bump allocator over the handle's adjoint work space
+struct arena {
+    char *base;
+    size_t off = 0;
+    template <typename T> T *take(size_t n)
+    {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += (n * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+};
+
+int zgemm(th_handle *h, int opA, int opB, int m, int n, int k, double2 alpha, const double2 *A, int lda,
+          const double2 *B, int ldb, double2 beta, double2 *C, int ldc)
+{
+    if (m <= 0 || n <= 0) return TH_OK;
+    dim3 grid((unsigned)((m + 15) / 16), (unsigned)((n + 15) / 16));
+    hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, h->stream, opA, opB, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+int zero2(th_handle *h, double2 *p, size_t n)
+{
+    if (n == 0) return TH_OK;
+    hipLaunchKernelGGL(zero2_kernel, dim3(nblk(n, 256)), dim3(256), 0, h->stream, p, n);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+// chi0hat (and Uhat) from chihat through the Dyson step; all device pointers
+int dyson_adjoint(th_handle *h, const double2 *d_chi0, const double2 *d_chi, const double2 *d_chat,
+                  const double *U_host, double2 *d_c0hat, double2 *d_Uhat, arena &ar)
+{
+    const int D = h->D;
+    const size_t DD = (size_t)D * D;
+    const double2 *U = reinterpret_cast<const double2 *>(U_host);
+    std::vector<char> on(D, 0);
+    for (int c = 0; c < D; ++c)
+        for (int a = 0; a < D; ++a) {
+            double2 u = U[a + (size_t)D * c];
+            if (u.x != 0.0 || u.y != 0.0) { on[a] = 1; on[c] = 1; }
+        }
+    std::vector<int> S;
+    for (int a = 0; a < D; ++a)
+        if (on[a]) S.push_back(a);
+    const int r = (int)S.size();
+    const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
+    TH_HIP(hipMemcpyAsync(d_c0hat, d_chat, sizeof(double2) * DD, hipMemcpyDeviceToDevice, h->stream));
+    if (r == 0) {
+        if (d_Uhat) {
+            // Uhat = -chi0^H chihat chi^H with chi = chi0
+            double2 *R = ar.take<double2>(DD);
+            int rc = zgemm(h, 0, 1, D, D, D, one, d_chat, D, d_chi, D, zero, R, D);
+            if (!rc) rc = zgemm(h, 1, 0, D, D, D, mone, d_chi0, D, R, D, zero, d_Uhat, D);
+            return rc;
+        }
+        return TH_OK;
+    }
+    std::vector<double2> US((size_t)r * r);
+    for (int b = 0; b < r; ++b)
+        for (int a = 0; a < r; ++a) US[a + (size_t)r * b] = U[S[a] + (size_t)D * S[b]];
+    int *d_S = ar.take<int>(r);
+    int *d_perm = ar.take<int>(r);
+    double2 *d_US = ar.take<double2>((size_t)r * r);
+    double2 *d_W = ar.take<double2>((size_t)r * r);
+    double2 *c0S = ar.take<double2>((size_t)D * r);   // chi0[:,S]
+    double2 *chS = ar.take<double2>((size_t)D * r);   // chi[S,:]
+    double2 *Y = ar.take<double2>((size_t)D * r);
+    double2 *Y2 = ar.take<double2>((size_t)D * r);
+    double2 *Q = ar.take<double2>((size_t)D * r);
+    TH_HIP(hipMemcpyAsync(d_S, S.data(), sizeof(int) * r, hipMemcpyHostToDevice, h->stream));
+    TH_HIP(hipMemcpyAsync(d_US, US.data(), sizeof(double2) * r * r, hipMemcpyHostToDevice, h->stream));
+    const unsigned g = nblk((size_t)D * r, 256);
+    hipLaunchKernelGGL(gather_cols, dim3(g), dim3(256), 0, h->stream, d_chi0, D, d_S, r, c0S);
+    hipLaunchKernelGGL(gather_rows, dim3(g), dim3(256), 0, h->stream, d_chi, D, d_S, r, chS);
+    hipLaunchKernelGGL(form_w_adj, dim3(nblk((size_t)r * r, 256)), dim3(256), 0, h->stream, d_chi0, D, d_S, d_US, r,
+                       d_W);
+    hipLaunchKernelGGL(gj_inverse, dim3(1), dim3(1024), sizeof(double2) * 2 * r, h->stream, d_W, r, d_perm,
+                       th_status_word(h));
+    TH_HIP(hipGetLastError());
+    int rc;
+    // X = M^-H chihat = chihat - P W^-H U_S^H chi0[:,S]^H chihat   (X in d_c0hat)
+    if ((rc = zgemm(h, 1, 0, r, D, D, one, c0S, D, d_chat, D, zero, Y, r))) return rc;
+    if ((rc = zgemm(h, 1, 0, r, D, r, one, d_US, r, Y, r, zero, Y2, r))) return rc;
+    if ((rc = zgemm(h, 1, 0, r, D, r, one, d_W, r, Y2, r, zero, Y, r))) return rc;
+    hipLaunchKernelGGL(scatter_sub_rows, dim3(g), dim3(256), 0, h->stream, d_c0hat, D, d_S, r, Y);
+    TH_HIP(hipGetLastError());
+    // Q = X chi[S,:]^H  (D x r)
+    if ((rc = zgemm(h, 0, 1, D, r, D, one, d_c0hat, D, chS, r, zero, Q, D))) return rc;
+    if (d_Uhat) {
+        // Uhat = -chi0^H X chi^H
+        double2 *R = ar.take<double2>(DD);
+        if ((rc = zgemm(h, 0, 1, D, D, D, one, d_c0hat, D, d_chi, D, zero, R, D))) return rc;
+        if ((rc = zgemm(h, 1, 0, D, D, D, mone, d_chi0, D, R, D, zero, d_Uhat, D))) return rc;
+    }
+    // chi0hat = X N^H = X - (X chi[S,:]^H) U_S^H in the columns S
+    if ((rc = zgemm(h, 0, 1, D, r, r, one, Q, D, d_US, r, zero, Y2, D))) return rc;
+    hipLaunchKernelGGL(scatter_sub_cols, dim3(g), dim3(256), 0, h->stream, d_c0hat, D, d_S, r, Y2);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+bool bad(const void *p) { return p == nullptr; }
+
+}  // namespace
+
+extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
+                              int n_x, const double *x2, const double *p2, int precision, const double *chi0,
+                              const double *U, const double *chi, const double *chi_hat, double *t_hat,
+                              double *h0_hat, double *U_hat)
+{
+    if (!h || bad(q) || bad(chi0) || bad(chi_hat) || bad(t_hat) || bad(h0_hat) || (U && bad(chi)) ||
+        (U_hat && !U) || (precision != TH_PREC_FP64 && precision != TH_PREC_MIXED) || n_x < 0 ||
+        (std::fabs(omega) >= 1e-15 && n_x > 0 && (bad(x2) || bad(p2)))) {
+        th_set_error("th_chi_adjoint: bad arguments");
+        return TH_E_ARG;
+    }
+    if (!h->d_kim || h->n_im <= 0) {
+        th_set_error("th_set_imag_points must be called before th_chi_adjoint");
+        return TH_E_STATE;
+    }
+    const bool real_axis = std::fabs(omega) >= 1e-15 && n_x > 0;
+    if (real_axis && (!h->d_kre || h->nk_re <= 0)) {
+        th_set_error("th_set_real_kmesh must be called before th_chi_adjoint at omega != 0");
+        return TH_E_STATE;
+    }
+    const int D = h->D, d = h->d, nO = h->nOrb;
+    const size_t DD = (size_t)D * D, dd = (size_t)d * d;
+    const int nout = h->nhop * nO * nO + (int)dd;
+    constexpr int CHUNK = 64;
+    // pairs per batch: adjoint buffers within twice the panel budget
+    const size_t per = (size_t)2 * h->nA2 * 2 * h->Mpad * sizeof(double) + 5 * dd * sizeof(double2) +
+                       (size_t)nout * sizeof(double2) / CHUNK + 64;
+    const long n_re = real_axis ? (long)n_x * h->nk_re : 0;
+    const long total = std::max(h->n_im, n_re);
+    long bp = (long)std::max<size_t>(TH_KC, ((size_t)h->batch_mb << 21) / per);
+    bp = std::min(bp, total);
+    const long nch_max = (bp + CHUNK - 1) / CHUNK;
+    // work space
+    size_t need = 0;
+    need += 4 * ((DD * sizeof(double2) + 255) & ~(size_t)255);                  // chat, c0hat, chi0, chi
+    need += U_hat ? ((DD * sizeof(double2) + 255) & ~(size_t)255) * 2 : 0;       // Uhat, R
+    {
+        size_t r_max = (size_t)D;                                               // support size bound
+        need += 2 * ((r_max * sizeof(int) + 255) & ~(size_t)255) + 2 * ((r_max * r_max * sizeof(double2) + 255) & ~(size_t)255) +
+                5 * (((size_t)D * r_max * sizeof(double2) + 255) & ~(size_t)255);
+    }
+    need += ((size_t)nout * sizeof(double2) + 255) & ~(size_t)255;              // gradient accumulator
+    need += (((size_t)nch_max * nout * sizeof(double2)) + 255) & ~(size_t)255;  // partial sums
+    need += 2 * (((size_t)bp * h->nA2 * 2 * h->Mpad * sizeof(double) + 255) & ~(size_t)255);
+    need += 5 * (((size_t)bp * dd * sizeof(double2) + 255) & ~(size_t)255);
+    need += 4096;
+    if (need > h->adj_bytes) {
+        hipFree(h->d_adj);
+        h->d_adj = nullptr;
+        h->adj_bytes = 0;
+        TH_HIP(hipMalloc(&h->d_adj, need));
+        h->adj_bytes = need;
+    }
+    arena ar{reinterpret_cast<char *>(h->d_adj)};
+    double2 *d_chat = ar.take<double2>(DD);
+    double2 *d_c0hat = ar.take<double2>(DD);
+    double2 *d_x0 = ar.take<double2>(DD);
+    double2 *d_x = ar.take<double2>(DD);
+    double2 *d_grad = ar.take<double2>(nout);
+    double2 *d_part = ar.take<double2>((size_t)nch_max * nout);
+    double *hatA = ar.take<double>((size_t)bp * h->nA2 * 2 * h->Mpad);
+    double *hatB = ar.take<double>((size_t)bp * h->nA2 * 2 * h->Mpad);
+    double2 *rawA = ar.take<double2>((size_t)bp * dd);
+    double2 *rawB = ar.take<double2>((size_t)bp * dd);
+    double2 *gA = ar.take<double2>((size_t)bp * dd);
+    double2 *gB = ar.take<double2>((size_t)bp * dd);
+    double2 *tmp = ar.take<double2>((size_t)bp * dd);
+    double2 *d_Uhat = U_hat ? ar.take<double2>(DD) : nullptr;
+
+    stage_timer tm(h);
+    const int idall = tm.begin(4);
+    for (auto &c : h->n_launch) c = 0;
+    int rc = th_reset_status(h);
+    if (rc) return rc;
+    if ((rc = th_ensure_panels(h, bp))) return rc;
+    if (real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
+    TH_HIP(hipMemcpyAsync(d_chat, chi_hat, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));
+    TH_HIP(hipMemcpyAsync(d_x0, chi0, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));
+    if (U) TH_HIP(hipMemcpyAsync(d_x, chi, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));
+    // 1. Dyson
+    int id = tm.begin(3);
+    if (U) {
+        if ((rc = dyson_adjoint(h, d_x0, d_x, d_chat, U, d_c0hat, d_Uhat, ar))) return rc;
+    } else {
+        TH_HIP(hipMemcpyAsync(d_c0hat, d_chat, sizeof(double2) * DD, hipMemcpyDeviceToDevice, h->stream));
+    }
+    tm.end(id);
+    // 2. accumulator cotangents (axis 0 imaginary, axis 1 real) in h->d_acc
+    id = tm.begin(2);
+    const size_t nacc = (size_t)2 * h->nA2 * 2 * h->Mpad * h->Mpad;
+    if ((rc = th_launch_zero(h, h->d_acc, nacc))) return rc;
+    hipLaunchKernelGGL(unfinalize_1, dim3(nblk(DD, 256)), dim3(256), 0, h->stream, d_c0hat, h->d_acc, h->nA, nO,
+                       h->Mpad, D, real_axis ? 1 : 0);
+    hipLaunchKernelGGL(unfinalize_2, dim3(nblk(DD, 256)), dim3(256), 0, h->stream, d_c0hat, h->d_acc, h->nA, nO,
+                       h->Mpad, D);
+    TH_HIP(hipGetLastError());
+    if ((rc = zero2(h, d_grad, nout))) return rc;
+    tm.end(id);
+    const size_t accsz = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
+    const int ntile = (d + 31) / 32;
+    // 3. pair batches of each axis
+    for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
+        const long npairs = axis == 0 ? h->n_im : n_re;
+        const double *That = h->d_acc + (size_t)axis * accsz;
+        for (long p0 = 0; p0 < npairs; p0 += bp) {
+            const long np = std::min(bp, npairs - p0);
+            th_green_args a{};
+            a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
+            a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = precision;
+            id = tm.begin(0);
+            a.mode = axis == 0 ? TH_G_IMAG_A : TH_G_REAL_A;
+            a.panel = h->d_pA;
+            a.raw = rawA;
+            if ((rc = th_launch_green(h, a))) return rc;
+            a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
+            a.panel = h->d_pB;
+            a.raw = rawB;
+            if ((rc = th_launch_green(h, a))) return rc;
+            tm.end(id);
+            id = tm.begin(1);
+            if ((rc = th_launch_panel_adjoint(h, h->d_pB, precision, That, np, 1, hatA))) return rc;
+            if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB))) return rc;
+            h->n_launch[1] += 2;
+            tm.end(id);
+            id = tm.begin(2);
+            for (int side = 0; side < 2; ++side) {
+                assemble_params ap{};
+                ap.mode = axis == 0 ? (side == 0 ? TH_G_IMAG_A : TH_G_IMAG_B) : (side == 0 ? TH_G_REAL_A : TH_G_REAL_B);
+                ap.nA = h->nA; ap.bs = h->bs; ap.d = d; ap.Mpad = h->Mpad; ap.nA2 = h->nA2;
+                ap.p0 = p0; ap.np = np; ap.nk_re = h->nk_re;
+                ap.wim = h->d_wim; ap.p2 = h->d_p2; ap.wre = h->d_wre;
+                ap.hat = side == 0 ? hatA : hatB;
+                double2 *gh = side == 0 ? gA : gB;
+                const double2 *G = side == 0 ? rawA : rawB;
+                ap.out = gh;
+                hipLaunchKernelGGL(assemble_kernel, dim3(nblk((size_t)np * dd, 256)), dim3(256), 0, h->stream, ap);
+                TH_HIP(hipGetLastError());
+                // Hhat = G^H (Ghat G^H)
+                const unsigned nwg = (unsigned)(np * ntile * ntile);
+                hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, (const double2 *)gh, 0, G, 1, tmp, d,
+                                   ntile);
+                hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, G, 1, (const double2 *)tmp, 0, gh, d,
+                                   ntile);
+                TH_HIP(hipGetLastError());
+                hreduce_params hp{};
+                hp.side_q = side == 0 ? 1 : 0;
+                hp.imag = axis == 0 ? 1 : 0;
+                hp.p0 = p0; hp.np = np; hp.nk_re = h->nk_re;
+                hp.chunk = CHUNK; hp.nout = nout; hp.nhop = h->nhop; hp.nOrb = nO; hp.d = d;
+                hp.q0 = q[0]; hp.q1 = q[1]; hp.q2 = q[2];
+                hp.kim = h->d_kim; hp.kre = h->d_kre; hp.R = h->d_R; hp.jb = h->d_jb; hp.ib = h->d_ib;
+                hp.Hh = gh;
+                hp.partial = d_part;
+                const int nch = (int)((np + CHUNK - 1) / CHUNK);
+                hipLaunchKernelGGL(hreduce_kernel, dim3(nblk(nout, 128), nch), dim3(128), 0, h->stream, hp);
+                hipLaunchKernelGGL(hsum_kernel, dim3(nblk(nout, 256)), dim3(256), 0, h->stream, (const double2 *)d_part,
+                                   nch, nout, d_grad);
+                TH_HIP(hipGetLastError());
+            }
+            tm.end(id);
+        }
+    }
+    tm.end(idall);
+    // 4. results to the host, hops back in the caller's order
+    std::vector<double2> g(nout);
+    TH_HIP(hipMemcpyAsync(g.data(), d_grad, sizeof(double2) * nout, hipMemcpyDeviceToHost, h->stream));
+    if (U_hat) TH_HIP(hipMemcpyAsync(U_hat, d_Uhat, sizeof(double2) * DD, hipMemcpyDeviceToHost, h->stream));
+    TH_HIP(hipStreamSynchronize(h->stream));
+    const int nO2 = nO * nO;
+    double2 *th = reinterpret_cast<double2 *>(t_hat);
+    for (int n = 0; n < h->nhop; ++n)
+        for (int e = 0; e < nO2; ++e) th[(size_t)h->hop_order[n] * nO2 + e] = g[(size_t)n * nO2 + e];
+    std::copy(g.begin() + (size_t)h->nhop * nO2, g.end(), reinterpret_cast<double2 *>(h0_hat));
+    rc = th_check_status(h);
+    tm.collect();
+    return rc;
+}

@@ -159,6 +159,128 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
         }
 }
 
+// Adjoint of the contraction (reverse mode).  With a cotangent That of one
+// axis's accumulator, T = sum_p A_p B_p^T gives
+//     Ahat_p = That conj(B_p),   Bhat_p = That^T conj(A_p)
+// per atom pair: a GEMM whose rows are the pairs of the batch and whose
+// inner dimension is the flattened block (K = Mpad).  out[p][n] =
+// sum_k conj(X[p][k]) Y[k][n] with Y[k][n] = That[n][k] (trans = 1, for
+// Ahat from X = B) or That[k][n] (trans = 0, for Bhat from X = A).  A
+// workgroup owns 112 pairs x 112 columns of one atom pair; the seven waves
+// each hold 16 pairs as 7 MFMA blocks, the same register layout as the
+// forward kernel.  In the mixed mode That is rounded to FP32 in LDS.
+template <typename P>
+__global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict__ X, const double *__restrict__ That,
+                                                            long np, int Mpad, int nA2, int ntile, int trans,
+                                                            double *__restrict__ out)
+{
+    __shared__ __align__(16) P Xs[2][TH_KC][2][TH_TILE];
+    __shared__ __align__(16) P Ys[2][TH_KC][2][TH_TILE];
+    constexpr int STAGE = TH_KC * 2 * TH_TILE;            // elements per matrix per stage
+    constexpr int PER_T = (STAGE + NTH - 1) / NTH;
+    const long nptile = (np + TH_TILE - 1) / TH_TILE;
+    const long b = blockIdx.x;
+    const int tn = (int)(b % ntile);
+    const long rest = b / ntile;
+    const long tp = rest % nptile;
+    const int ij = (int)(rest / nptile);
+    const long p0 = tp * TH_TILE;
+    const int n0 = tn * TH_TILE;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const size_t pstride = (size_t)nA2 * 2 * Mpad;
+    const size_t MM = (size_t)Mpad * Mpad;
+    const P *Xb = X + (size_t)ij * 2 * Mpad;
+    const double *Tb = That + (size_t)ij * 2 * MM;
+
+    typename mfma_t<P>::acc accr[7], acci[7];
+    for (int j = 0; j < 7; ++j)
+        for (int q = 0; q < 4; ++q) { accr[j][q] = 0; acci[j][q] = 0; }
+
+    P rx[PER_T], ry[PER_T];
+    // Stage element v of X: kk fastest (8 consecutive k of one pair row are
+    // one contiguous segment of the panel), then the 112 rows, then re/im.
+    // Y follows its memory order: kk fastest for trans = 1 (rows n of That),
+    // n fastest for trans = 0 (rows k of That).
+    auto xidx = [](int v, int &kk, int &m, int &comp) {
+        kk = v % TH_KC; m = (v / TH_KC) % TH_TILE; comp = v / (TH_KC * TH_TILE);
+    };
+    auto yidx = [trans](int v, int &kk, int &m, int &comp) {
+        comp = v / (TH_KC * TH_TILE);
+        const int r = v - comp * TH_KC * TH_TILE;
+        if (trans) { kk = r % TH_KC; m = r / TH_KC; }
+        else { m = r % TH_TILE; kk = r / TH_TILE; }
+    };
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            const int v = tid + u * NTH;
+            P x = 0, y = 0;
+            if (v < STAGE) {
+                int kk, m, comp;
+                xidx(v, kk, m, comp);
+                const long pp = p0 + m;
+                if (pp < np) x = Xb[pp * pstride + (size_t)comp * Mpad + k0 + kk];
+                yidx(v, kk, m, comp);
+                const int n = n0 + m, k = k0 + kk;
+                y = (P)(trans ? Tb[(size_t)comp * MM + (size_t)n * Mpad + k] : Tb[(size_t)comp * MM + (size_t)k * Mpad + n]);
+            }
+            rx[u] = x;
+            ry[u] = y;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            const int v = tid + u * NTH;
+            if (v < STAGE) {
+                int kk, m, comp;
+                xidx(v, kk, m, comp);
+                Xs[buf][kk][comp][m] = rx[u];
+                yidx(v, kk, m, comp);
+                Ys[buf][kk][comp][m] = ry[u];
+            }
+        }
+    };
+    const int nstage = Mpad / TH_KC;
+    gload(0);
+    for (int s = 0; s < nstage; ++s) {
+        const int buf = s & 1;
+        sstore(buf);
+        __syncthreads();
+        if (s + 1 < nstage) gload((s + 1) * TH_KC);
+#pragma unroll
+        for (int kk = 0; kk < TH_KC; kk += 4) {
+            const int pr = kk + (lane >> 4);
+            const int m = wave * 16 + (lane & 15);
+            const P xr = Xs[buf][pr][0][m], xi = Xs[buf][pr][1][m];
+            const P nxi = -xi;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int n = j * 16 + (lane & 15);
+                const P yr = Ys[buf][pr][0][n], yi = Ys[buf][pr][1][n];
+                // conj(x) * y = (xr yr + xi yi) + i (xr yi - xi yr)
+                accr[j] = mfma_t<P>::mac(xr, yr, accr[j]);
+                accr[j] = mfma_t<P>::mac(xi, yi, accr[j]);
+                acci[j] = mfma_t<P>::mac(xr, yi, acci[j]);
+                acci[j] = mfma_t<P>::mac(nxi, yr, acci[j]);
+            }
+        }
+        // the next iteration's sstore overwrites the other buffer only
+    }
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long pp = p0 + wave * 16 + mfma_t<P>::row(lane >> 4, q);
+            const int col = n0 + j * 16 + (lane & 15);
+            if (pp < np) {
+                double *o = out + (size_t)pp * pstride + (size_t)ij * 2 * Mpad;
+                o[col] = (double)accr[j][q];
+                o[Mpad + col] = (double)acci[j][q];
+            }
+        }
+}
+
 // acc += sum_s part[s], elementwise, fixed order
 __global__ void reduce_kernel(const double *__restrict__ part, double *__restrict__ acc, int splits, size_t n)
 {
@@ -257,5 +379,22 @@ int th_launch_finalize(th_handle *h, double2 *chi0)
                        (const double *)h->d_acc, chi0, h->nA, h->nOrb, h->Mpad, h->D);
     TH_HIP(hipGetLastError());
     h->n_launch[2] += 1;
+    return TH_OK;
+}
+
+int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
+                            double *out)
+{
+    if (np <= 0) return TH_OK;
+    const int ntile = h->Mpad / TH_TILE;
+    const long nptile = (np + TH_TILE - 1) / TH_TILE;
+    const long nwg = nptile * ntile * h->nA2;
+    if (prec == TH_PREC_FP64)
+        hipLaunchKernelGGL((panel_adjoint_kernel<double>), dim3((unsigned)nwg), dim3(NTH), 0, h->stream,
+                           (const double *)panel, That, np, h->Mpad, h->nA2, ntile, trans, out);
+    else
+        hipLaunchKernelGGL((panel_adjoint_kernel<float>), dim3((unsigned)nwg), dim3(NTH), 0, h->stream,
+                           (const float *)panel, That, np, h->Mpad, h->nA2, ntile, trans, out);
+    TH_HIP(hipGetLastError());
     return TH_OK;
 }

@@ -8,9 +8,11 @@
 // into a GEMM panel (the gf(:,:,i,j) scatter of mod_greenfunction.F90:55-62,
 // plus the weight and the real-axis combinations of eintshechi.F90:143-165).
 //
-// Mixed precision: the elimination runs in FP32 on a copy of (z - H), then
-// one refinement step G <- G + G (I - (z - H) G) with the residual formed
-// against the FP64 matrix.
+// Precision: the elimination always runs in FP64 (on CDNA4 unpacked FP32
+// VALU work issues at the FP64 rate, so an FP32 elimination would not be
+// faster); the mixed mode rounds the finished G once to FP32 when it writes
+// the panel.  For the adjoint the kernel can also write the unpermuted G
+// (FP64, row-major d x d per pair) to a second buffer.
 #include "th_internal.h"
 
 namespace {
@@ -40,6 +42,7 @@ struct green_params {
     const int *hoff;
     const double *kim, *yim, *wim, *kre, *wre, *x2, *p2;
     void *panel;
+    double2 *raw;
     int *status;
 };
 
@@ -509,6 +512,10 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
                 if (dcol[j] >= 0) S[dr * DD + dcol[j]] = v[i][j];
         }
         __syncthreads();
+        if (g.raw) {
+            double2 *o = g.raw + (size_t)pl * NE;
+            for (int e = tid; e < NE; e += NT) o[e] = make_double2(S[e].x, S[e].y);
+        }
         write_panel<P, NT>(g, pl, S, ident, DD, w);
     } else {
         const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
@@ -517,6 +524,16 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
         int drow[RI];
 #pragma unroll
         for (int i = 0; i < RI; ++i) drow[i] = (ty + i * TY < DD) ? qinv[ty + i * TY] : -1;
+        if (g.raw) {
+            double2 *o = g.raw + (size_t)pl * NE;
+#pragma unroll
+            for (int i = 0; i < RI; ++i) {
+                if (drow[i] < 0) continue;
+#pragma unroll
+                for (int j = 0; j < CJ; ++j)
+                    if (dcol[j] >= 0) o[drow[i] * DD + dcol[j]] = make_double2(v[i][j].x, v[i][j].y);
+            }
+        }
         for (int ij = 0; ij < g.nA2; ++ij) {
             const int i0 = ij / nA, j0 = ij - (ij / nA) * nA;
 #pragma unroll
@@ -620,6 +637,14 @@ __global__ __launch_bounds__(NT) void green_kernel(green_params g)
     __syncthreads();
     // ---- inverse ----
     gj_invert<double, NT>(M, d, colk, prow, perm, idx, s_piv, s_inv, g.status);
+    if (g.raw) {
+        double2 *o = g.raw + (size_t)pl * d * d;
+        for (int e = tid; e < d * d; e += NT) {
+            const int r = e / d, c = e - (e / d) * d;
+            const cplx<double> x = M[r * d + idx[c]];
+            o[e] = make_double2(x.x, x.y);
+        }
+    }
     write_panel<P, NT>(g, pl, M, idx, d, w);
 }
 
@@ -649,7 +674,7 @@ int launch_green_2d(th_handle *h, const th_green_args &a, const green_params &g)
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
-    if (h->kernel_variant == 1) {   // one-dimensional ownership (kept for comparison)
+    if (h->kernel_variant == 1 && !a.raw) {   // one-dimensional ownership (kept for comparison)
         switch (h->d) {
         case 18: return launch_green_reg<P, 18, 64>(h, a, g);
         case 54: return launch_green_reg<P, 54, 256>(h, a, g);
@@ -699,6 +724,7 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.kim = h->d_kim; g.yim = h->d_yim; g.wim = h->d_wim;
     g.kre = h->d_kre; g.wre = h->d_wre; g.x2 = h->d_x2; g.p2 = h->d_p2;
     g.panel = a.panel;
+    g.raw = a.raw;
     g.status = th_status_word(h);
     h->n_launch[0] += 1;
     if (a.prec == TH_PREC_FP64) return launch_green_p<double>(h, a, g);

@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/titan_hip.h"
 
@@ -60,6 +61,10 @@ struct th_handle {
     long n_launch[4] = {0, 0, 0, 0};
     int kernel_variant = 0;      // 0 default kernels, 1 alternates (tuning / comparison)
     int batch_mb = 512;          // panel budget per batch (MB)
+    // adjoint work space and the caller's hop order (hops are stored sorted)
+    std::vector<int> hop_order;  // stored hop n came from caller's entry hop_order[n]
+    void *d_adj = nullptr;
+    size_t adj_bytes = 0;
 };
 
 // error plumbing
@@ -72,6 +77,47 @@ void th_set_error(const std::string &msg);
             return TH_E_HIP;                                                       \
         }                                                                          \
     } while (0)
+
+// Stage timing with HIP events on the handle's stream
+struct stage_timer {
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    th_handle *h;
+    explicit stage_timer(th_handle *hh) : h(hh) {}
+    int begin(int stage)
+    {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+        hipEventRecord(a, h->stream);
+        ev.push_back({stage, {a, b}});
+        return (int)ev.size() - 1;
+    }
+    void end(int id)
+    {
+        if (id >= 0) hipEventRecord(ev[id].second.second, h->stream);
+    }
+    void collect()
+    {
+        for (int s = 0; s < 5; ++s) h->t_ms[s] = 0.0;
+        for (auto &e : ev) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) h->t_ms[e.first] += ms;
+        }
+    }
+    ~stage_timer()
+    {
+        for (auto &e : ev) {
+            hipEventDestroy(e.second.first);
+            hipEventDestroy(e.second.second);
+        }
+    }
+};
+
+// shared helpers of the C ABI (th_abi.hip)
+int *th_status_word(th_handle *h);
+int th_reset_status(th_handle *h);
+int th_check_status(th_handle *h);
+int th_ensure_panels(th_handle *h, long pairs);
+int th_upload_nodes(th_handle *h, int n_x, const double *x2, const double *p2);
 
 // ---- kernel launchers (th_green.hip) ----
 // Modes of the green-function kernel: which matrix, and how it is written
@@ -90,6 +136,7 @@ struct th_green_args {
     double omega, Ef, eta, etap;
     int prec;             // TH_PREC_*
     void *panel;          // output panel, [np][nA2][2][Mpad] of double or float
+    double2 *raw = nullptr;   // optional: G itself, [np][d][d] row-major FP64
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
 
@@ -99,6 +146,11 @@ int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, in
 int th_launch_zero(th_handle *h, double *p, size_t n);
 // chi0 = (T_imag + T_imag^dagger + T_real) / 2pi in the chi basis.
 int th_launch_finalize(th_handle *h, double2 *chi0);
+// Adjoint of the contraction for one panel: out[p] = conj(panel[p]) . That^T
+// (trans = 1) or conj(panel[p]) . That (trans = 0) per atom pair, FP64 out
+// [np][nA2][2][Mpad]; That is [nA2][2][Mpad][Mpad] FP64.
+int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
+                            double *out);
 
 // ---- Dyson (th_dyson.hip) ----
 int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,

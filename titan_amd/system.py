@@ -262,12 +262,14 @@ class TBModel:
             h0[s, s] += blk
         return h0
 
-    def interaction_matrix(self):
+    def interaction_matrix(self, per_atom=None):
+        """Umatorb (mod_susceptibilities.F90:106-159).  per_atom[i] = (Un, Um)
+        overrides the values of the atoms' types (linear in them)."""
         D, nO = self.D, self.nOrb
         U = np.zeros((D, D), dtype=complex)
         dorb = np.nonzero(self.orbs >= 5)[0]
         for i in range(self.nA):
-            Un, Um = self.spec["types"][self.type[i]]["U"]
+            Un, Um = self.spec["types"][self.type[i]]["U"] if per_atom is None else per_atom[i]
             for mu in dorb:
                 for ga in dorb:
                     U[self.cidx(0, i, mu, mu), self.cidx(0, i, ga, ga)] = Um
