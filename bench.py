@@ -65,6 +65,43 @@ def cpu_baseline(spec, run, omega, target_s=12.0):
                       (nks, len(y) + len(x2), len(y), len(x2), omega, dt)}
 
 
+def adjoint_cost(cfg=3, omega=4e-3, reps=2):
+    """Forward (th_chi0 + th_dyson) and adjoint (th_chi_adjoint) time of one
+    frequency of the adjoint workload (cfg3: the cfg0 system on a 24^3
+    k-mesh, 64 + 32 energies), in both precisions, after a warm-up call of
+    each (the forward's omega-independent G cache is then built, as in a
+    sweep)."""
+    from titan_amd import binding as B
+    from titan_amd import configs as CF
+    from titan_amd import contour as Ct
+    from titan_amd.chi import ChiWorkload
+    spec, run = CF.system(cfg), CF.run_params(cfg)
+    wl = ChiWorkload(spec, run)
+    m, h = wl.model, wl.h
+    x2, p2 = Ct.real_epoints(m.Ef, omega, run["parts3"], run["n3gl"])
+    rng = np.random.default_rng(1)
+    W = rng.standard_normal((m.D, m.D)) + 1j * rng.standard_normal((m.D, m.D))
+    out = {"workload": "cfg%d: %s, k-mesh %s, %d imag + %d real energies, one omega" %
+                       (cfg, spec["name"], "x".join(map(str, run["nk"])), len(wl.y), wl.n_x),
+           "pairs": wl.pairs([omega])}
+    for name, prec in (("fp64", B.PREC_FP64), ("mixed", B.PREC_MIXED)):
+        tf, tb = [], []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            c0 = h.chi0(run["q"], omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=prec)
+            chi, _ = h.dyson(c0, m.Umat)
+            t1 = time.perf_counter()
+            h.adjoint(run["q"], omega, m.Ef, run["eta"], run["etap"], c0, W, U=m.Umat, chi=chi, x2=x2, p2=p2,
+                      precision=prec)
+            t2 = time.perf_counter()
+            if r:
+                tf.append(t1 - t0)
+                tb.append(t2 - t1)
+        out[name] = {"forward_ms": 1e3 * min(tf), "adjoint_ms": 1e3 * min(tb), "ratio": min(tb) / min(tf)}
+    wl.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +110,7 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the chi(omega) path is built for a single MI355X "
@@ -163,6 +201,13 @@ def main():
         im_f = np.abs(ref[:, 0, 0].imag)
         out["peak_bin_match"] = bool(np.argmax(im_m) == np.argmax(im_f))
     out["dyson_cond_max"] = float(np.max(mx["cond"]))
+    if not args.no_adjoint:
+        try:
+            out["adjoint_cost"] = adjoint_cost()
+            log("adjoint", out["adjoint_cost"])
+        except Exception as e:   # reported, never required
+            log("adjoint cost failed:", e)
+            out["adjoint_cost"] = None
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(spec, run, float(om_time[-1]))

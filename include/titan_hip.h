@@ -141,7 +141,14 @@ int th_last_timing(th_handle *h, double *t, long *n);
 
 /* Tuning knobs of the kernels (no counterpart in the reference; used by the
  * autotuner in titan_amd/autotune.py):
- *   "green_variant"  0 = 2-D register-blocked inversion (default), 1 = 1-D
+ *   "green_variant"  inversion kernel shape: 0 = 2-D register-blocked (default
+ *                    tiles), 1 = 1-D ownership, 2 = one wave per matrix,
+ *                    3 = two waves per matrix (2 and 3 where compiled for d)
+ *   "graph"          1 = th_chi_sweep captures one frequency step (chi0, Dyson,
+ *                    traces) as a hipGraph at its first frequency with and
+ *                    without real axis and replays it for the others, omega
+ *                    read from device memory; per-stage times are then not
+ *                    split (only t[4] is set).  0 = plain launches (default)
  *   "splits"         K-splits of the contraction (workgroups per tile)
  *   "batch_mb"       panel memory per batch in MB (pairs per launch)
  * th_get_tuning reads the current values. */

@@ -316,3 +316,54 @@ def test_large_blocks_match_oracle(which):
     o = s.chi0([0.02, 0, 0], om, 1e-3, 1e-3, (K, Y, W), (kp, wk), (x2, p2))
     assert rel(g, o) < 1e-10, rel(g, o)
     h.close()
+
+
+@pytest.mark.parametrize("which,variants", [("cfg2", (2, 3)), ("bcc4", (3,))])
+def test_inversion_variants_are_bitwise_identical(which, variants):
+    """The inversion kernel shapes selectable with th_tune("green_variant")
+    perform the same floating-point operations per matrix entry (same pivot
+    order, same fused updates), so chi0 must agree bit for bit."""
+    spec = {"cfg2": lambda: CF.system(2), "bcc4": lambda: CF.bcc_supercell(4),
+            "slab8": lambda: CF.fcc001_slab(8)}[which]()
+    m = TBModel(spec)
+    dims = (2, 1, 1) if which != "slab8" else (2, 2, 1)
+    y, wy = Ct.imag_epoints(1, 3)
+    kp, wk = Ct.uniform_kmesh(m.b, *dims)
+    K = np.tile(kp, (len(y), 1)); Y = np.repeat(y, len(wk)); W = np.outer(wy, wk).ravel()
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    om = 2e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 2)
+    ref = h.chi0([0.02, 0, 0], om, m.Ef, 1e-3, 1e-3, x2, p2)
+    for v in variants:
+        print(which, "variant", v, flush=True)
+        h.tune(green_variant=v)
+        g = h.chi0([0.02, 0, 0], om, m.Ef, 1e-3, 1e-3, x2, p2)
+        assert np.array_equal(g, ref), (v, rel(g, ref))
+    h.close()
+
+
+@pytest.mark.parametrize("prec", [B.PREC_FP64, B.PREC_MIXED])
+def test_graph_sweep_equals_plain_sweep(cfg0, prec):
+    """th_tune(graph=1): each frequency step is a replay of a captured
+    hipGraph (one with and one without real axis).  Every replay, with
+    plain launches and host synchronisations of the program in between,
+    must reproduce the plain sweep bit for bit."""
+    m, _ = cfg0
+    (K, Y, W), (kp, wk) = _work(m, 4, n1gl=8)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    oms = [0.0, 2e-3, 4e-3, 0.0, 6e-3]
+    args = ([0.0, 0.0, 0.0], oms, m.Ef, 5e-4, 5e-4, 1, 8, m.Umat)
+    ref = h.sweep(*args, precision=prec, full=True)
+    for rep in range(3):
+        print("graph replay", rep, flush=True)
+        h.tune(graph=1)
+        got = h.sweep(*args, precision=prec, full=True)
+        h.tune(graph=0)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+        h.chi0([0.0, 0.0, 0.0], 1e-3, m.Ef, 5e-4, 5e-4, *Ct.real_epoints(m.Ef, 1e-3, 1, 8), precision=prec)
+    h.close()

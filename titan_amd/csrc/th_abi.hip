@@ -170,41 +170,51 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
     return TH_OK;
 }
 
-// chi0 of one omega into h->d_chi0 (device)
-int chi0_device(th_handle *h, const double *q, double omega, double Ef, double eta, double etap, int n_x,
-                const double *x2, const double *p2, int prec, stage_timer &tm)
+// Host side of one omega of chi0: argument checks, real-axis nodes to the
+// device, the omega-independent B cache, panel buffers.  Issues no kernel of
+// the contraction itself.
+int chi0_prepare(th_handle *h, double omega, double Ef, double etap, int n_x, const double *x2,
+                 const double *p2, int prec, stage_timer &tm, bool *real_axis, bool *have_b)
 {
     if (!h->d_kim || h->n_im <= 0) {
         th_set_error("th_set_imag_points must be called before th_chi0");
         return TH_E_STATE;
     }
-    const bool real_axis = std::fabs(omega) >= 1e-15 && n_x > 0;
-    if (real_axis && (!h->d_kre || h->nk_re <= 0)) {
+    *real_axis = std::fabs(omega) >= 1e-15 && n_x > 0;
+    if (*real_axis && (!h->d_kre || h->nk_re <= 0)) {
         th_set_error("th_set_real_kmesh must be called before th_chi0 at omega != 0");
         return TH_E_STATE;
     }
-    if (real_axis) {
-        int rc0 = th_upload_nodes(h, n_x, x2, p2);
-        if (rc0) return rc0;
-    }
+    int rc;
+    if (*real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
+    if ((rc = ensure_bcache(h, Ef, etap, prec, tm, have_b))) return rc;
+    long need = batch_pairs(h, h->n_im);
+    if (*real_axis) need = std::max(need, batch_pairs(h, (long)n_x * h->nk_re));
+    return ensure_panels(h, need);
+}
+
+// Kernel launches of one omega of chi0 into h->d_chi0 (everything on the
+// handle's stream, no host synchronisation, no allocation: capturable).
+// omega_p, when set, points to omega in device memory.
+int chi0_launch(th_handle *h, const double *q, double omega, const double *omega_p, double Ef, double eta,
+                double etap, int n_x, bool real_axis, bool have_b, int prec, stage_timer *tm)
+{
     const size_t nacc = (size_t)2 * h->nA2 * 2 * h->Mpad * h->Mpad;
     int rc = th_launch_zero(h, h->d_acc, nacc);
     if (rc) return rc;
-    bool have_b = false;
-    rc = ensure_bcache(h, Ef, etap, prec, tm, &have_b);
-    if (rc) return rc;
     const size_t pstride = (size_t)h->nA2 * 2 * h->Mpad * prec_size(prec);
+    auto begin = [&](int st) { return tm ? tm->begin(st) : -1; };
+    auto end = [&](int id) { if (tm) tm->end(id); };
     // imaginary axis
     const long bp = batch_pairs(h, h->n_im);
-    if ((rc = ensure_panels(h, bp))) return rc;
     for (long p0 = 0; p0 < h->n_im; p0 += bp) {
         const long np = std::min(bp, h->n_im - p0);
         th_green_args a{};
         a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
-        a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
+        a.omega = omega; a.omega_p = omega_p; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
         a.mode = TH_G_IMAG_A;
         a.panel = h->d_pA;
-        int id = tm.begin(0);
+        int id = begin(0);
         if ((rc = th_launch_green(h, a))) return rc;
         const void *pB = nullptr;
         if (have_b) {
@@ -215,38 +225,47 @@ int chi0_device(th_handle *h, const double *q, double omega, double Ef, double e
             if ((rc = th_launch_green(h, a))) return rc;
             pB = h->d_pB;
         }
-        tm.end(id);
-        id = tm.begin(1);
+        end(id);
+        id = begin(1);
         if ((rc = th_launch_contract(h, h->d_pA, pB, np, prec, 0))) return rc;
-        tm.end(id);
+        end(id);
     }
     // real axis
     if (real_axis) {
         const long n_re = (long)n_x * h->nk_re;
         const long br = batch_pairs(h, n_re);
-        if ((rc = ensure_panels(h, br))) return rc;
         for (long p0 = 0; p0 < n_re; p0 += br) {
             const long np = std::min(br, n_re - p0);
             th_green_args a{};
             a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
-            a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
+            a.omega = omega; a.omega_p = omega_p; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
             a.mode = TH_G_REAL_A;
             a.panel = h->d_pA;
-            int id = tm.begin(0);
+            int id = begin(0);
             if ((rc = th_launch_green(h, a))) return rc;
             a.mode = TH_G_REAL_B;
             a.panel = h->d_pB;
             if ((rc = th_launch_green(h, a))) return rc;
-            tm.end(id);
-            id = tm.begin(1);
+            end(id);
+            id = begin(1);
             if ((rc = th_launch_contract(h, h->d_pA, h->d_pB, np, prec, 1))) return rc;
-            tm.end(id);
+            end(id);
         }
     }
-    int id = tm.begin(2);
+    int id = begin(2);
     rc = th_launch_finalize(h, h->d_chi0);
-    tm.end(id);
+    end(id);
     return rc;
+}
+
+// chi0 of one omega into h->d_chi0 (device)
+int chi0_device(th_handle *h, const double *q, double omega, double Ef, double eta, double etap, int n_x,
+                const double *x2, const double *p2, int prec, stage_timer &tm)
+{
+    bool real_axis = false, have_b = false;
+    int rc = chi0_prepare(h, omega, Ef, etap, n_x, x2, p2, prec, tm, &real_axis, &have_b);
+    if (rc) return rc;
+    return chi0_launch(h, q, omega, nullptr, Ef, eta, etap, n_x, real_axis, have_b, prec, &tm);
 }
 
 bool bad(const void *p) { return p == nullptr; }
@@ -420,6 +439,42 @@ int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out,
     return rc;
 }
 
+namespace {
+
+// One omega-step of the sweep captured as a hipGraph: chi0 launches with
+// omega read from device memory, the Dyson kernels and both traces.  All
+// host work (nodes, caches, buffers, U support) is done before capture, so
+// the graph only references buffers that stay in place for the whole sweep.
+int capture_step(th_handle *h, const double *q, double Ef, double eta, double etap, int n_x, bool real_axis,
+                 bool have_b, int prec, int r, double2 *d_s, hipGraphExec_t *out)
+{
+    const int n4 = 4 * h->nA;
+    hipGraph_t graph = nullptr;
+    TH_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = chi0_launch(h, q, 0.0, h->d_omega, Ef, eta, etap, n_x, real_axis, have_b, prec, nullptr);
+    if (!rc) rc = th_dyson_launch(h, r, h->d_chi0, h->d_chi, h->d_norms);
+    if (!rc) rc = th_launch_trace(h, h->d_chi0, d_s);
+    if (!rc) rc = th_launch_trace(h, h->d_chi, d_s + n4 * n4);
+    hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        th_set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return TH_E_HIP;
+    }
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        th_set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return TH_E_HIP;
+    }
+    return TH_OK;
+}
+
+}  // namespace
+
 int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega, double Ef, double eta,
                  double etap, int parts3, int n3gl, const double *U, int precision, double *schi0_out,
                  double *schi_out, double *cond_out, double *chi_full_out)
@@ -430,45 +485,95 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
         return TH_E_ARG;
     }
     const int n4 = 4 * h->nA, n_x = parts3 * n3gl;
-    std::vector<double> x2(n_x), p2(n_x);
+    std::vector<double> x2(n_x), p2(n_x), om(omega, omega + n_omega);
     double2 *d_s = nullptr;
     TH_HIP(hipMalloc(&d_s, sizeof(double2) * 2 * n4 * n4));
+    if (h->use_graph && (!h->d_omega || !h->d_norms)) {
+        if (!h->d_omega) TH_HIP(hipMalloc(&h->d_omega, sizeof(double)));
+        if (!h->d_norms) TH_HIP(hipMalloc(&h->d_norms, 2 * sizeof(double)));
+    }
+    hipGraphExec_t ge[2] = {nullptr, nullptr};
     stage_timer tm(h);
     int idall = tm.begin(4);
     for (auto &c : h->n_launch) c = 0;
+    int r = 0;
     int rc = th_reset_status(h);
+    if (!rc && h->use_graph) rc = th_dyson_prepare(h, reinterpret_cast<const double2 *>(U), &r);
     for (int w = 0; w < n_omega && !rc; ++w) {
-        const double e = omega[w];
+        const double e = om[w];
         if (std::fabs(e) >= 1e-15)
             for (int k = 1; k <= parts3; ++k) {
                 double e1 = Ef - ((parts3 - (k - 1)) * e / parts3);
                 double e2 = Ef - ((parts3 - k) * e / parts3);
                 gauleg(e1, e2, n3gl, x2.data() + (k - 1) * n3gl, p2.data() + (k - 1) * n3gl);
             }
-        rc = chi0_device(h, q, e, Ef, eta, etap, n_x, x2.data(), p2.data(), precision, tm);
-        if (rc) break;
-        int id = tm.begin(3);
         double cond = 0.0;
-        rc = th_dyson_device(h, h->d_chi0, reinterpret_cast<const double2 *>(U), h->d_chi, cond_out ? &cond : nullptr);
-        if (!rc) rc = th_launch_trace(h, h->d_chi0, d_s);
-        if (!rc) rc = th_launch_trace(h, h->d_chi, d_s + n4 * n4);
-        tm.end(id);
-        if (rc) break;
+        if (h->use_graph) {
+            bool real_axis = false, have_b = false;
+            rc = chi0_prepare(h, e, Ef, etap, n_x, x2.data(), p2.data(), precision, tm, &real_axis, &have_b);
+            if (rc) break;
+            if (hipMemcpyAsync(h->d_omega, &om[w], sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+                th_set_error("th_chi_sweep: omega upload failed");
+                rc = TH_E_HIP;
+                break;
+            }
+            const int kind = real_axis ? 1 : 0;
+            if (!ge[kind] && (rc = capture_step(h, q, Ef, eta, etap, n_x, real_axis, have_b, precision, r, d_s,
+                                                &ge[kind])))
+                break;
+            if (hipGraphLaunch(ge[kind], h->stream) != hipSuccess) {
+                th_set_error("th_chi_sweep: hipGraphLaunch failed");
+                rc = TH_E_HIP;
+                break;
+            }
+            h->n_launch[0] += 1;
+            if (cond_out) {
+                double nv[2];
+                if (hipMemcpyAsync(nv, h->d_norms, sizeof(nv), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                    hipStreamSynchronize(h->stream) != hipSuccess) {
+                    th_set_error("th_chi_sweep: norm download failed");
+                    rc = TH_E_HIP;
+                    break;
+                }
+                cond = nv[0] * nv[1];
+            }
+        } else {
+            rc = chi0_device(h, q, e, Ef, eta, etap, n_x, x2.data(), p2.data(), precision, tm);
+            if (rc) break;
+            int id = tm.begin(3);
+            rc = th_dyson_device(h, h->d_chi0, reinterpret_cast<const double2 *>(U), h->d_chi,
+                                 cond_out ? &cond : nullptr);
+            if (!rc) rc = th_launch_trace(h, h->d_chi0, d_s);
+            if (!rc) rc = th_launch_trace(h, h->d_chi, d_s + n4 * n4);
+            tm.end(id);
+            if (rc) break;
+        }
         if (cond_out) cond_out[w] = cond;
+        hipError_t ce = hipSuccess;
         if (schi0_out)
-            TH_HIP(hipMemcpyAsync(schi0_out + (size_t)w * 2 * n4 * n4, d_s, sizeof(double2) * n4 * n4,
-                                  hipMemcpyDeviceToHost, h->stream));
-        if (schi_out)
-            TH_HIP(hipMemcpyAsync(schi_out + (size_t)w * 2 * n4 * n4, d_s + n4 * n4, sizeof(double2) * n4 * n4,
-                                  hipMemcpyDeviceToHost, h->stream));
-        TH_HIP(hipStreamSynchronize(h->stream));
+            ce = hipMemcpyAsync(schi0_out + (size_t)w * 2 * n4 * n4, d_s, sizeof(double2) * n4 * n4,
+                                hipMemcpyDeviceToHost, h->stream);
+        if (schi_out && ce == hipSuccess)
+            ce = hipMemcpyAsync(schi_out + (size_t)w * 2 * n4 * n4, d_s + n4 * n4, sizeof(double2) * n4 * n4,
+                                hipMemcpyDeviceToHost, h->stream);
+        if (ce == hipSuccess) ce = hipStreamSynchronize(h->stream);
+        if (ce != hipSuccess) {
+            th_set_error(std::string("th_chi_sweep: ") + hipGetErrorString(ce));
+            rc = TH_E_HIP;
+        }
     }
     tm.end(idall);
-    if (!rc && chi_full_out)
-        TH_HIP(hipMemcpyAsync(chi_full_out, h->d_chi, sizeof(double2) * (size_t)h->D * h->D, hipMemcpyDeviceToHost,
-                              h->stream));
+    if (!rc && chi_full_out &&
+        hipMemcpyAsync(chi_full_out, h->d_chi, sizeof(double2) * (size_t)h->D * h->D, hipMemcpyDeviceToHost,
+                       h->stream) != hipSuccess) {
+        th_set_error("th_chi_sweep: chi download failed");
+        rc = TH_E_HIP;
+    }
     if (!rc) rc = th_check_status(h);
+    hipStreamSynchronize(h->stream);
     tm.collect();
+    for (auto &g : ge)
+        if (g) hipGraphExecDestroy(g);
     hipFree(d_s);
     return rc;
 }
@@ -489,7 +594,7 @@ int th_tune(th_handle *h, const char *key, long value)
     if (!h || !key) { th_set_error("th_tune: bad arguments"); return TH_E_ARG; }
     const std::string k(key);
     if (k == "green_variant") {
-        if (value < 0 || value > 1) { th_set_error("th_tune: green_variant is 0 or 1"); return TH_E_ARG; }
+        if (value < 0 || value > 3) { th_set_error("th_tune: green_variant is 0..3"); return TH_E_ARG; }
         h->kernel_variant = (int)value;
     } else if (k == "splits") {
         if (value < 1 || value > 4096) { th_set_error("th_tune: splits in 1..4096"); return TH_E_ARG; }
@@ -500,6 +605,9 @@ int th_tune(th_handle *h, const char *key, long value)
             TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)value * h->nA2 * 2 * MM));
         }
         h->splits = (int)value;
+    } else if (k == "graph") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: graph is 0 or 1"); return TH_E_ARG; }
+        h->use_graph = (int)value;
     } else if (k == "batch_mb") {
         if (value < 16 || value > 65536) { th_set_error("th_tune: batch_mb in 16..65536"); return TH_E_ARG; }
         h->batch_mb = (int)value;
@@ -525,7 +633,7 @@ int th_destroy(th_handle *h)
     hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
-                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj};
+                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (size_t i = 0; i < g_status.size(); ++i)

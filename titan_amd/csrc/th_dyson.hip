@@ -10,6 +10,7 @@
 //   W = I + chi0[S,S] U_S,   chi = chi0 - (chi0[:,S] U_S) (W^-1 chi0[S,:]).
 // All of the near-pole ill-conditioning of (I + chi0 U) sits in W; its
 // 1-norm condition number is returned.
+#include <algorithm>
 #include <vector>
 
 #include "th_internal.h"
@@ -204,12 +205,15 @@ __global__ void trace_kernel(const double2 *chi, int nA, int nOrb, int D, double
 
 inline unsigned nblk(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// writes ||W||_1 = ||W^-1||_1 = 1 (U = 0: chi = chi0, condition number 1)
+__global__ void unit_norms(double *norms)
+{
+    if (threadIdx.x < 2) norms[threadIdx.x] = 1.0;
+}
+
 }  // namespace
 
-int *th_status_word(th_handle *h);
-int th_check_status(th_handle *h);
-
-int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U, double2 *chi, double *cond)
+int th_dyson_prepare(th_handle *h, const double2 *U, int *r_out)
 {
     const int D = h->D;
     // support of U: indices with a nonzero row or column
@@ -223,17 +227,16 @@ int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U, double2
     for (int a = 0; a < D; ++a)
         if (on[a]) S.push_back(a);
     const int r = (int)S.size();
-    if (r == 0) {
-        TH_HIP(hipMemcpyAsync(chi, chi0, sizeof(double2) * (size_t)D * D, hipMemcpyDeviceToDevice, h->stream));
-        if (cond) *cond = 1.0;
-        return TH_OK;
-    }
+    *r_out = r;
+    if (!h->d_X) TH_HIP(hipMalloc(&h->d_X, sizeof(double) * 2 + sizeof(int) * (size_t)std::max(r, 1)));
+    if (r == 0) return TH_OK;
     std::vector<double2> US((size_t)r * r);
     for (int b = 0; b < r; ++b)
         for (int a = 0; a < r; ++a) US[a + (size_t)r * b] = U[S[a] + (size_t)D * S[b]];
     if (r > h->sup_cap) {
         hipFree(h->d_sup); hipFree(h->d_U); hipFree(h->d_W); hipFree(h->d_V); hipFree(h->d_Z); hipFree(h->d_X);
         h->d_sup = nullptr; h->d_U = h->d_W = h->d_V = h->d_Z = h->d_X = nullptr;
+        h->sup_cap = 0;
         TH_HIP(hipMalloc(&h->d_sup, sizeof(int) * (r + 2)));
         TH_HIP(hipMalloc(&h->d_U, sizeof(double2) * (size_t)r * r));
         TH_HIP(hipMalloc(&h->d_W, sizeof(double2) * (size_t)r * r));
@@ -244,8 +247,22 @@ int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U, double2
     }
     TH_HIP(hipMemcpyAsync(h->d_sup, S.data(), sizeof(int) * r, hipMemcpyHostToDevice, h->stream));
     TH_HIP(hipMemcpyAsync(h->d_U, US.data(), sizeof(double2) * (size_t)r * r, hipMemcpyHostToDevice, h->stream));
-    double *norms = reinterpret_cast<double *>(h->d_X);
-    int *perm = reinterpret_cast<int *>(norms + 2);
+    // the host vectors go out of scope: the copies must have landed
+    TH_HIP(hipStreamSynchronize(h->stream));
+    return TH_OK;
+}
+
+int th_dyson_launch(th_handle *h, int r, const double2 *chi0, double2 *chi, double *norms_dev)
+{
+    const int D = h->D;
+    double *norms = norms_dev ? norms_dev : reinterpret_cast<double *>(h->d_X);
+    if (r == 0) {
+        TH_HIP(hipMemcpyAsync(chi, chi0, sizeof(double2) * (size_t)D * D, hipMemcpyDeviceToDevice, h->stream));
+        hipLaunchKernelGGL(unit_norms, dim3(1), dim3(64), 0, h->stream, norms);
+        TH_HIP(hipGetLastError());
+        return TH_OK;
+    }
+    int *perm = reinterpret_cast<int *>(reinterpret_cast<double *>(h->d_X) + 2);
     hipLaunchKernelGGL(form_w, dim3(nblk((size_t)r * r, 256)), dim3(256), 0, h->stream, chi0, D, h->d_sup, h->d_U,
                        r, h->d_W);
     TH_HIP(hipGetLastError());
@@ -262,9 +279,18 @@ int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U, double2
                        D, r, chi);
     TH_HIP(hipGetLastError());
     h->n_launch[3] += 5;
+    return TH_OK;
+}
+
+int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U, double2 *chi, double *cond)
+{
+    int r = 0;
+    int rc = th_dyson_prepare(h, U, &r);
+    if (!rc) rc = th_dyson_launch(h, r, chi0, chi, nullptr);
+    if (rc) return rc;
     if (cond) {
         double nv[2];
-        TH_HIP(hipMemcpyAsync(nv, norms, sizeof(nv), hipMemcpyDeviceToHost, h->stream));
+        TH_HIP(hipMemcpyAsync(nv, h->d_X, sizeof(nv), hipMemcpyDeviceToHost, h->stream));
         TH_HIP(hipStreamSynchronize(h->stream));
         *cond = nv[0] * nv[1];
     }

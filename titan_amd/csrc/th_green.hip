@@ -36,6 +36,7 @@ struct green_params {
     int nA, nOrb, bs, d, nhop, Mpad, nA2;
     long p0, nk_re;
     double q0, q1, q2, omega, Ef, eta, etap;
+    const double *omega_p;   // when set, omega is read from device memory (graph replays)
     const double *R;
     const int *jb, *ib;
     const double2 *t, *h0;
@@ -122,11 +123,12 @@ __device__ inline void pair_point(const green_params &g, long p, double &k0, dou
                                   double &ei, double &w)
 {
     w = 1.0;
+    const double omega = g.omega_p ? *g.omega_p : g.omega;
     if (g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B) {
         k0 = g.kim[3 * p]; k1 = g.kim[3 * p + 1]; k2 = g.kim[3 * p + 2];
         if (g.mode == TH_G_IMAG_A) {
             k0 += g.q0; k1 += g.q1; k2 += g.q2;
-            er = g.Ef + g.omega; ei = g.yim[p] + g.eta; w = g.wim[p];
+            er = g.Ef + omega; ei = g.yim[p] + g.eta; w = g.wim[p];
         } else {
             er = g.Ef; ei = g.yim[p] + g.etap;
         }
@@ -135,7 +137,7 @@ __device__ inline void pair_point(const green_params &g, long p, double &k0, dou
         k0 = g.kre[3 * ik]; k1 = g.kre[3 * ik + 1]; k2 = g.kre[3 * ik + 2];
         if (g.mode == TH_G_REAL_A) {
             k0 += g.q0; k1 += g.q1; k2 += g.q2;
-            er = g.x2[ie] + g.omega; ei = g.eta; w = g.p2[ie] * g.wre[ik];
+            er = g.x2[ie] + omega; ei = g.eta; w = g.p2[ie] * g.wre[ik];
         } else {
             er = g.x2[ie]; ei = g.etap;
         }
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
         // Entry (r, c') of the registers is G[r][pos[c']] with pos = idx^-1.
         int *pos = perm;   // perm is no longer needed
         __syncthreads();
-        if (tid < DD) pos[idx[tid]] = tid;
+        for (int r = tid; r < DD; r += NT) pos[idx[r]] = r;
         __syncthreads();
         const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
         cplx<double> *Sij = S, *Sji = S + Mb;          // blocks (i,j) and (j,i), row-major bs x bs
@@ -365,14 +367,21 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
 // pivot row pass through LDS (double-buffered by step parity), two barriers.
 extern "C" __device__ unsigned __ockl_wfred_max_u32(unsigned);
 
+// The whole inverse is staged through LDS for the panel scatter when it
+// takes at most 12 KB per wave of the workgroup (so the staging buffer does
+// not limit occupancy below what the registers allow); otherwise one pair of
+// atom blocks at a time.
+constexpr bool full_stage(int dd, int nt) { return dd * dd * 16 <= 12288 * (nt / 64); }
+
 template <typename P, int DD, int TY, int TX>
 __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
 {
     constexpr int NT = TY * TX;
     constexpr int RI = (DD + TY - 1) / TY, CJ = (DD + TX - 1) / TX;
     constexpr int NE = DD * DD;
-    constexpr bool FULL_STAGE = (NE * 16) <= 49152;
+    constexpr bool FULL_STAGE = full_stage(DD, NT);
     static_assert(DD <= 255, "row index packed in 8 bits");
+    static_assert(CJ <= 8, "the column loop is unrolled over the CJ register slots");
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);
     cplx<double> *ph = S + (FULL_STAGE ? NE : 2 * g.bs * g.bs);
@@ -419,72 +428,69 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
                 v[i][j] = x;
             }
         }
-    for (int k = 0; k < DD; ++k) {
-        const int bb = k & 1;
-        if (tx == k % TX) {
-            const int jk = k / TX;
+    // Column k = kx + jb*TX lives in register slot jb of the lanes with
+    // tx == kx: with jb unrolled, the pivot column is published from a static
+    // register slot.  The pivot row is data dependent (ip = pr / TY is uniform
+    // across the workgroup, so the tests on it are scalar branches).
 #pragma unroll
-            for (int j = 0; j < CJ; ++j)
-                if (j == jk)
+    for (int jb = 0; jb < CJ; ++jb) {
+        for (int kx = 0; kx < TX; ++kx) {
+            const int k = kx + jb * TX;
+            if (k >= DD) break;
+            const int bb = k & 1;
+            if (tx == kx) {
 #pragma unroll
-                    for (int i = 0; i < RI; ++i) {
-                        const int r = ty + i * TY;
-                        if (r < DD) colk[bb][r] = v[i][j];
-                    }
-        }
-        __syncthreads();
-        // pivot: key = magnitude (float, low 8 mantissa bits dropped) | 255 - row,
-        // so the maximum key is the largest |re|+|im|, lowest row on ties
-        unsigned key = 0u;
-        for (int r = lane; r < DD; r += 64)
-            if (!used[r]) {
-                const float a = (float)cabs1(colk[bb][r]);
-                const unsigned kk = (__float_as_uint(a) & 0xFFFFFF00u) | (unsigned)(255 - r);
-                key = kk > key ? kk : key;
+                for (int i = 0; i < RI; ++i) {
+                    const int r = ty + i * TY;
+                    if (r < DD) colk[bb][r] = v[i][jb];
+                }
             }
-        key = __ockl_wfred_max_u32(key);
-        const int pr = 255 - (int)(key & 0xFFu);
-        cplx<double> piv = colk[bb][pr];
-        if (piv.x == 0.0 && piv.y == 0.0) {
-            if (tid == 0) atomicMin(g.status, k + 1);
-            piv.x = 1.0;
-        }
-        const cplx<double> inv = cinv(piv);
-        if (ty == pr % TY) {
-            const int ip = pr / TY;
+            __syncthreads();
+            // pivot: key = magnitude (float, low 8 mantissa bits dropped) | 255 - row,
+            // so the maximum key is the largest |re|+|im|, lowest row on ties
+            unsigned key = 0u;
+            for (int r = lane; r < DD; r += 64)
+                if (!used[r]) {
+                    const float a = (float)cabs1(colk[bb][r]);
+                    const unsigned kk = (__float_as_uint(a) & 0xFFFFFF00u) | (unsigned)(255 - r);
+                    key = kk > key ? kk : key;
+                }
+            key = __ockl_wfred_max_u32(key);
+            const int pr = 255 - (int)(key & 0xFFu);
+            const int ip = pr / TY, typ = pr - ip * TY;
+            cplx<double> piv = colk[bb][pr];
+            if (piv.x == 0.0 && piv.y == 0.0) {
+                if (tid == 0) atomicMin(g.status, k + 1);
+                piv.x = 1.0;
+            }
+            const cplx<double> inv = cinv(piv);
+            // the owners of row pr publish it scaled by 1/piv, with 1 + 1/piv in
+            // column k, so every other row is updated by v -= f * prs
 #pragma unroll
             for (int i = 0; i < RI; ++i)
-                if (i == ip)
+                if (i == ip && ty == typ) {
 #pragma unroll
                     for (int j = 0; j < CJ; ++j) {
                         const int c = tx + j * TX;
                         if (c < DD) prs[bb][c] = (c == k) ? cplx<double>{1.0 + inv.x, inv.y} : cmul(v[i][j], inv);
                     }
-        }
-        __syncthreads();
-        // after the barrier: every wave has finished this step's pivot search
-        if (tid == 0) {
-            perm[k] = pr;
-            used[pr] = 1;
-        }
-        cplx<double> pj[CJ];
-#pragma unroll
-        for (int j = 0; j < CJ; ++j) {
-            const int c = tx + j * TX;
-            pj[j] = (c < DD) ? prs[bb][c] : cplx<double>{0.0, 0.0};
-        }
-#pragma unroll
-        for (int i = 0; i < RI; ++i) {
-            const int r = ty + i * TY;
-            if (r >= DD) continue;
-            if (r == pr) {
-#pragma unroll
-                for (int j = 0; j < CJ; ++j) {
-                    const int c = tx + j * TX;
-                    v[i][j] = (c == k) ? inv : cmul(v[i][j], inv);
                 }
-            } else {
-                const cplx<double> f = colk[bb][r];
+            __syncthreads();
+            // after the barrier: every wave has finished this step's pivot search
+            if (tid == 0) {
+                perm[k] = pr;
+                used[pr] = 1;
+            }
+            cplx<double> pj[CJ];
+#pragma unroll
+            for (int j = 0; j < CJ; ++j) {
+                const int c = tx + j * TX;
+                pj[j] = (c < DD) ? prs[bb][c] : cplx<double>{0.0, 0.0};
+            }
+#pragma unroll
+            for (int i = 0; i < RI; ++i) {
+                const int r = ty + i * TY;
+                const cplx<double> f = (r < DD) ? colk[bb][r] : cplx<double>{0.0, 0.0};
 #pragma unroll
                 for (int j = 0; j < CJ; ++j) {
                     // v -= f * pj
@@ -492,11 +498,21 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
                                fma(-f.x, pj[j].y, fma(-f.y, pj[j].x, v[i][j].y))};
                 }
             }
+            // row pr takes the published values: v/piv, and 1/piv in column k
+#pragma unroll
+            for (int i = 0; i < RI; ++i)
+                if (i == ip && ty == typ) {
+#pragma unroll
+                    for (int j = 0; j < CJ; ++j) {
+                        const int c = tx + j * TX;
+                        v[i][j] = (c == k) ? inv : pj[j];
+                    }
+                }
         }
     }
     // unscramble: G[k][p_c] = V[p_k][c], i.e. entry (rho, c) goes to (q[rho], perm[c])
     __syncthreads();
-    if (tid < DD) qinv[perm[tid]] = tid;
+    for (int r = tid; r < DD; r += NT) qinv[perm[r]] = r;   // NT may be below DD
     __syncthreads();
     int dcol[CJ];
 #pragma unroll
@@ -664,7 +680,7 @@ int launch_green_reg(th_handle *h, const th_green_args &a, const green_params &g
 template <typename P, int DD, int TY, int TX>
 int launch_green_2d(th_handle *h, const th_green_args &a, const green_params &g)
 {
-    const bool full = (DD * DD * 16) <= 49152;
+    const bool full = full_stage(DD, TY * TX);
     size_t lds = sizeof(double2) * ((full ? (size_t)DD * DD : (size_t)2 * h->bs * h->bs) + h->nhop);
     hipLaunchKernelGGL((green_2d_kernel<P, DD, TY, TX>), dim3((unsigned)a.np), dim3(TY * TX), lds, h->stream, g);
     TH_HIP(hipGetLastError());
@@ -681,13 +697,27 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
         default: break;
         }
     }
+    if (h->kernel_variant == 2) {   // one wave per Green function
+        switch (h->d) {
+        case 36: return launch_green_2d<P, 36, 8, 8>(h, a, g);
+        case 54: return launch_green_2d<P, 54, 8, 8>(h, a, g);
+        default: break;
+        }
+    }
+    if (h->kernel_variant == 3) {   // two waves per Green function
+        switch (h->d) {
+        case 54: return launch_green_2d<P, 54, 8, 16>(h, a, g);
+        case 72: return launch_green_2d<P, 72, 8, 16>(h, a, g);
+        default: break;
+        }
+    }
     switch (h->d) {
     case 18: return launch_green_2d<P, 18, 8, 8>(h, a, g);
     case 36: return launch_green_2d<P, 36, 8, 16>(h, a, g);
     case 40: return launch_green_2d<P, 40, 8, 16>(h, a, g);
     case 54: return launch_green_2d<P, 54, 16, 16>(h, a, g);
     case 72: return launch_green_2d<P, 72, 16, 16>(h, a, g);
-    case 144: return launch_green_2d<P, 144, 32, 16>(h, a, g);
+    case 144: return launch_green_2d<P, 144, 16, 32>(h, a, g);
     default: break;
     }
     if (h->d <= 24) return launch_green_t<P, 64>(h, a, g);
@@ -720,6 +750,7 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.Mpad = h->Mpad; g.nA2 = h->nA2; g.p0 = a.p0; g.nk_re = h->nk_re;
     g.q0 = a.q[0]; g.q1 = a.q[1]; g.q2 = a.q[2];
     g.omega = a.omega; g.Ef = a.Ef; g.eta = a.eta; g.etap = a.etap;
+    g.omega_p = a.omega_p;
     g.R = h->d_R; g.jb = h->d_jb; g.ib = h->d_ib; g.t = h->d_t; g.h0 = h->d_h0; g.hoff = h->d_hoff;
     g.kim = h->d_kim; g.yim = h->d_yim; g.wim = h->d_wim;
     g.kre = h->d_kre; g.wre = h->d_wre; g.x2 = h->d_x2; g.p2 = h->d_p2;

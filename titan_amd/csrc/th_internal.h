@@ -59,7 +59,12 @@ struct th_handle {
     hipEvent_t ev[8] = {};
     double t_ms[5] = {0, 0, 0, 0, 0};
     long n_launch[4] = {0, 0, 0, 0};
-    int kernel_variant = 0;      // 0 default kernels, 1 alternates (tuning / comparison)
+    int kernel_variant = 0;      // 0 default kernels, 1..3 alternates (tuning / comparison)
+    int use_graph = 0;           // th_chi_sweep replays one captured hipGraph per omega
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // [omega == 0, omega != 0]
+    int graph_prec[2] = {-1, -1};
+    double *d_omega = nullptr;   // omega of the current replay
+    double *d_norms = nullptr;   // Dyson norms of the current replay
     int batch_mb = 512;          // panel budget per batch (MB)
     // adjoint work space and the caller's hop order (hops are stored sorted)
     std::vector<int> hop_order;  // stored hop n came from caller's entry hop_order[n]
@@ -137,6 +142,7 @@ struct th_green_args {
     int prec;             // TH_PREC_*
     void *panel;          // output panel, [np][nA2][2][Mpad] of double or float
     double2 *raw = nullptr;   // optional: G itself, [np][d][d] row-major FP64
+    const double *omega_p = nullptr;   // optional: omega in device memory (graph replays)
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
 
@@ -155,4 +161,9 @@ int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const dou
 // ---- Dyson (th_dyson.hip) ----
 int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,
                     double2 *chi, double *cond);
+// Graph-friendly split of the Dyson step: set-up from U (host work, uploads,
+// allocations; returns the support size r, 0 if U = 0), then kernels only,
+// norms ||W||_1, ||W^-1||_1 left in norms_dev[0..1].
+int th_dyson_prepare(th_handle *h, const double2 *U_host_colmajor, int *r_out);
+int th_dyson_launch(th_handle *h, int r, const double2 *chi0, double2 *chi, double *norms_dev);
 int th_launch_trace(th_handle *h, const double2 *chi, double2 *schi_dev);
