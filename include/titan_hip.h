@@ -21,9 +21,12 @@
  * 796-829); < 0 = error (see TH_E_* below).  The Fortran shim maps any
  * nonzero status to abortProgram (source/mod_mpi_pars.F90:59-72).
  *
- * Threading: one handle per process and device; calls on one handle must not
- * overlap.  The handle owns every device buffer; the caller owns all host
- * arrays, and no host pointer is retained after a call returns.
+ * Threading: a handle lives on the device that was current at th_create and
+ * owns a HIP stream of its own; calls on one handle must not overlap, calls
+ * on different handles may run concurrently from different host threads
+ * (their kernels then overlap on the GPU).  The handle owns every device
+ * buffer; the caller owns all host arrays, and no host pointer is retained
+ * after a call returns.
  */
 #ifndef TITAN_HIP_H
 #define TITAN_HIP_H

@@ -367,3 +367,53 @@ def test_graph_sweep_equals_plain_sweep(cfg0, prec):
             assert np.array_equal(a, b)
         h.chi0([0.0, 0.0, 0.0], 1e-3, m.Ef, 5e-4, 5e-4, *Ct.real_epoints(m.Ef, 1e-3, 1, 8), precision=prec)
     h.close()
+
+
+def test_block_elimination_d288_matches_oracle():
+    """d = 288 (16-site bcc supercell, the largest cell of the uneven batch):
+    G by 2 x 2 block elimination with register-resident 144 x 144
+    inversions; chi0 within 1e-10 of the oracle's LU-based path."""
+    spec = CF.bcc_supercell(16)
+    m, s = TBModel(spec), O.OracleSystem(spec)
+    y, wy = Ct.imag_epoints(1, 2)
+    kp, wk = Ct.uniform_kmesh(m.b, 2, 1, 1)
+    K = np.tile(kp, (len(y), 1)); Y = np.repeat(y, len(wk)); W = np.outer(wy, wk).ravel()
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    om = 2e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 2)
+    print("d288 gpu", flush=True)
+    g = h.chi0([0.01, 0, 0], om, m.Ef, 1e-3, 1e-3, x2, p2)
+    print("d288 oracle", flush=True)
+    o = s.chi0([0.01, 0, 0], om, 1e-3, 1e-3, (K, Y, W), (kp, wk), (x2, p2))
+    assert rel(g, o) < 1e-10, rel(g, o)
+    h.close()
+
+
+def test_uneven_batch_matches_oracle():
+    """One cell of each size of the uneven batch (n_atoms 1, 2, 4, 8, 16;
+    d = 18 ... 288) swept concurrently from five host threads on five
+    streams, with autotuned shapes per size: every cell's chi0 trace (and
+    chi trace up to n_atoms = 4) equals the oracle's to 1e-10."""
+    from titan_amd.batch import UnevenBatch
+    cells = CF.uneven_batch(sizes=[1, 2, 4, 8, 16])
+    for spec, run in cells:
+        run["n1gl"], run["n3gl"] = 4, 2
+    ub = UnevenBatch(cells, kmesh=(2, 1, 1), batch_mb=64)
+    best = ub.autotune(precision=B.PREC_FP64, sample=1)
+    print("tuned", {d: b["best"] for d, b in best.items()}, flush=True)
+    oms = [0.0, 3e-3]
+    res, dt, pairs = ub.sweep(B.PREC_FP64, omegas=oms, threads=5)
+    print("batch sweep %.2f s, %d pairs" % (dt, pairs), flush=True)
+    for c, (s0, s, cond) in zip(ub.cells, res):
+        m, o = c.model, O.OracleSystem(c.spec)
+        imag = (np.tile(c.kp, (len(c.y), 1)), np.repeat(c.y, len(c.wk)), np.outer(c.wy, c.wk).ravel())
+        for w, om in enumerate(oms):
+            re_k = (c.kp, c.wk) if om else None
+            re_e = Ct.real_epoints(m.Ef, om, 1, 2) if om else None
+            c0 = o.chi0([0, 0, 0], om, 5e-4, 5e-4, imag, re_k, re_e)
+            assert rel(s0[w], O.trace(m.nA, m.nOrb, c0)) < 1e-10, (m.nA, om)
+            if m.nA <= 4:
+                assert rel(s[w], O.trace(m.nA, m.nOrb, O.dyson(c0, o.umat()))) < 1e-10, (m.nA, om)
+    ub.close()

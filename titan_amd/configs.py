@@ -138,6 +138,36 @@ def run_params(cfg):
     raise ValueError("unknown configuration %r" % cfg)
 
 
+def uneven_cell_sizes(ncells=40, seed=4):
+    """Atoms per cell of the uneven batch, drawn from {1, 2, 4, 8, 16}."""
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in rng.choice([1, 2, 4, 8, 16], size=ncells)]
+
+
+def uneven_kmesh(nA):
+    """Per-cell k-mesh edge of the uneven batch: 24 at nA = 1 down to 8 at
+    nA = 16, scaled as nA^(-log 3 / log 16) (inversely with the cell size)."""
+    return int(round(24.0 / nA ** (np.log(3.0) / np.log(16.0))))
+
+
+def uneven_batch(ncells=40, seed=4, sizes=None):
+    """The uneven batch: cells of n_atoms in {1, 2, 4, 8, 16} (H from 18 x 18
+    to 288 x 288), each a bcc supercell with its own seeded Slater-Koster,
+    SOC and U parameters as in the 1-site system, per-cell k-mesh from 24^3
+    (nA = 1) to 8^3 (nA = 16), 48 imaginary + 16 real energies (64 per k at
+    omega != 0), 16 frequencies in (0, 0.01] Ry.  Returns [(spec, run)]."""
+    sizes = uneven_cell_sizes(ncells, seed) if sizes is None else list(sizes)
+    cells = []
+    for c, nA in enumerate(sizes):
+        spec = bcc_supercell(nA, seed=1000 + seed * 100 + c)
+        spec["name"] = "uneven cell %d: bcc %d-site" % (c, nA)
+        n = uneven_kmesh(nA)
+        run = {"nk": (n, n, n), "parts": 1, "n1gl": 48, "parts3": 1, "n3gl": 16, "eta": 5e-4, "etap": 5e-4,
+               "omega": np.linspace(0.01 / 16, 0.01, 16).tolist(), "q": [0.0, 0.0, 0.0]}
+        cells.append((spec, run))
+    return cells
+
+
 def system(cfg):
     if cfg in (0, 3):
         return bcc_1site()

@@ -4,6 +4,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,9 +51,11 @@ void th_set_error(const std::string &msg) { g_err = msg; }
 // per-handle device status word (first zero pivot, INT_MAX if none); kept
 // in a side table so the handle layout stays plain
 static std::vector<std::pair<th_handle *, int *>> g_status;
+static std::mutex g_status_mu;   // handles may be created and used from several threads
 
 int *th_status_word(th_handle *h)
 {
+    std::lock_guard<std::mutex> lk(g_status_mu);
     for (auto &e : g_status)
         if (e.first == h) return e.second;
     return nullptr;
@@ -300,7 +303,7 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
             return TH_E_ARG;
         }
     const int d = 2 * nOrb * nA;
-    if (d > TH_MAX_D_LDS && d != 72 && d != 144) {
+    if (d > TH_MAX_D_LDS && d != 72 && d != 144 && d != 288) {
         th_set_error("th_create: d = 2*nOrb*nA = " + std::to_string(d) + " exceeds " + std::to_string(TH_MAX_D_LDS));
         return TH_E_UNSUPPORTED;
     }
@@ -325,7 +328,10 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
     h->splits = std::max(1, (2 * prop.multiProcessorCount) / (ntile * ntile * h->nA2));
     int *st = nullptr;
     TH_HIP(hipMalloc(&st, sizeof(int)));
-    g_status.push_back({h, st});
+    {
+        std::lock_guard<std::mutex> lk(g_status_mu);
+        g_status.push_back({h, st});
+    }
     // hops grouped by atom pair (jb, ib) so an entry of H(k) visits only its own hops
     std::vector<int> order(nhop), hoff(nA * nA + 1, 0);
     for (int n = 0; n < nhop; ++n) {
@@ -633,15 +639,18 @@ int th_destroy(th_handle *h)
     hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
-                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms};
+                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk};
     for (void *b : bufs)
         if (b) hipFree(b);
-    for (size_t i = 0; i < g_status.size(); ++i)
-        if (g_status[i].first == h) {
-            hipFree(g_status[i].second);
-            g_status.erase(g_status.begin() + i);
-            break;
-        }
+    {
+        std::lock_guard<std::mutex> lk(g_status_mu);
+        for (size_t i = 0; i < g_status.size(); ++i)
+            if (g_status[i].first == h) {
+                hipFree(g_status[i].second);
+                g_status.erase(g_status.begin() + i);
+                break;
+            }
+    }
     hipStreamDestroy(h->stream);
     delete h;
     return TH_OK;

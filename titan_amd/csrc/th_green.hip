@@ -367,67 +367,23 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
 // pivot row pass through LDS (double-buffered by step parity), two barriers.
 extern "C" __device__ unsigned __ockl_wfred_max_u32(unsigned);
 
-// The whole inverse is staged through LDS for the panel scatter when it
-// takes at most 12 KB per wave of the workgroup (so the staging buffer does
-// not limit occupancy below what the registers allow); otherwise one pair of
-// atom blocks at a time.
-constexpr bool full_stage(int dd, int nt) { return dd * dd * 16 <= 12288 * (nt / 64); }
-
-template <typename P, int DD, int TY, int TX>
-__global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
+// Gauss-Jordan elimination with implicit pivoting on a register tile: thread
+// (ty,tx) of a TY x TX grid owns rows ty + i*TY and columns tx + j*TX of the
+// DD x DD matrix in v.  Rows are never moved; step k picks the unused row
+// p_k with the largest |re|+|im| in column k (one packed-key wavefront max,
+// repeated by every wave).  On return v holds V with G[k][p_c] = V[p_k][c]:
+// perm[k] = p_k and qinv[p_k] = k.  'used' must be zero on entry.
+template <int DD, int TY, int TX>
+__device__ __forceinline__ void gj2d_eliminate(cplx<double> (&v)[(DD + TY - 1) / TY][(DD + TX - 1) / TX],
+                                               cplx<double> (*colk)[DD], cplx<double> (*prs)[DD], int *perm,
+                                               int *qinv, unsigned char *used, int *status)
 {
     constexpr int NT = TY * TX;
     constexpr int RI = (DD + TY - 1) / TY, CJ = (DD + TX - 1) / TX;
-    constexpr int NE = DD * DD;
-    constexpr bool FULL_STAGE = full_stage(DD, NT);
     static_assert(DD <= 255, "row index packed in 8 bits");
     static_assert(CJ <= 8, "the column loop is unrolled over the CJ register slots");
-    extern __shared__ __align__(16) unsigned char smem[];
-    cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);
-    cplx<double> *ph = S + (FULL_STAGE ? NE : 2 * g.bs * g.bs);
-    __shared__ cplx<double> colk[2][DD], prs[2][DD];
-    __shared__ int perm[DD], qinv[DD], ident[DD];
-    __shared__ unsigned char used[DD];
     const int tid = threadIdx.x, lane = tid & 63;
     const int ty = tid / TX, tx = tid - (tid / TX) * TX;
-    const long pl = blockIdx.x;
-    const long p = g.p0 + pl;
-    double k0, k1, k2, er, ei, w;
-    pair_point(g, p, k0, k1, k2, er, ei, w);
-    for (int n = tid; n < g.nhop; n += NT) {
-        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
-        double sn, cs;
-        sincos(a, &sn, &cs);
-        ph[n] = {cs, sn};
-    }
-    for (int r = tid; r < DD; r += NT) {
-        used[r] = 0;
-        ident[r] = r;
-    }
-    __syncthreads();
-    const int nO = g.nOrb, nO2 = nO * nO;
-    cplx<double> v[RI][CJ];
-#pragma unroll
-    for (int i = 0; i < RI; ++i)
-#pragma unroll
-        for (int j = 0; j < CJ; ++j) {
-            const int r = ty + i * TY, c = tx + j * TX;
-            v[i][j] = {0.0, 0.0};
-            if (r < DD && c < DD) {
-                double2 hv = g.h0[r + (size_t)c * DD];
-                cplx<double> x = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
-                const int ja = r / (2 * nO), rs = (r / nO) & 1, nu = r - (r / nO) * nO;
-                const int ia = c / (2 * nO), csp = (c / nO) & 1, mu = c - (c / nO) * nO;
-                if (rs == csp) {
-                    const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
-                    for (int n = h1; n < h2; ++n) {
-                        double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
-                        x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
-                    }
-                }
-                v[i][j] = x;
-            }
-        }
     // Column k = kx + jb*TX lives in register slot jb of the lanes with
     // tx == kx: with jb unrolled, the pivot column is published from a static
     // register slot.  The pivot row is data dependent (ip = pr / TY is uniform
@@ -460,7 +416,7 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
             const int ip = pr / TY, typ = pr - ip * TY;
             cplx<double> piv = colk[bb][pr];
             if (piv.x == 0.0 && piv.y == 0.0) {
-                if (tid == 0) atomicMin(g.status, k + 1);
+                if (tid == 0) atomicMin(status, k + 1);
                 piv.x = 1.0;
             }
             const cplx<double> inv = cinv(piv);
@@ -514,6 +470,68 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
     __syncthreads();
     for (int r = tid; r < DD; r += NT) qinv[perm[r]] = r;   // NT may be below DD
     __syncthreads();
+}
+
+// The whole inverse is staged through LDS for the panel scatter when it
+// takes at most 12 KB per wave of the workgroup (so the staging buffer does
+// not limit occupancy below what the registers allow); otherwise one pair of
+// atom blocks at a time.
+constexpr bool full_stage(int dd, int nt) { return dd * dd * 16 <= 12288 * (nt / 64); }
+
+template <typename P, int DD, int TY, int TX>
+__global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
+{
+    constexpr int NT = TY * TX;
+    constexpr int RI = (DD + TY - 1) / TY, CJ = (DD + TX - 1) / TX;
+    constexpr int NE = DD * DD;
+    constexpr bool FULL_STAGE = full_stage(DD, NT);
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double> *S = reinterpret_cast<cplx<double> *>(smem);
+    cplx<double> *ph = S + (FULL_STAGE ? NE : 2 * g.bs * g.bs);
+    __shared__ cplx<double> colk[2][DD], prs[2][DD];
+    __shared__ int perm[DD], qinv[DD], ident[DD];
+    __shared__ unsigned char used[DD];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int ty = tid / TX, tx = tid - (tid / TX) * TX;
+    const long pl = blockIdx.x;
+    const long p = g.p0 + pl;
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, p, k0, k1, k2, er, ei, w);
+    for (int n = tid; n < g.nhop; n += NT) {
+        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[n] = {cs, sn};
+    }
+    for (int r = tid; r < DD; r += NT) {
+        used[r] = 0;
+        ident[r] = r;
+    }
+    __syncthreads();
+    const int nO = g.nOrb, nO2 = nO * nO;
+    cplx<double> v[RI][CJ];
+#pragma unroll
+    for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < CJ; ++j) {
+            const int r = ty + i * TY, c = tx + j * TX;
+            v[i][j] = {0.0, 0.0};
+            if (r < DD && c < DD) {
+                double2 hv = g.h0[r + (size_t)c * DD];
+                cplx<double> x = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
+                const int ja = r / (2 * nO), rs = (r / nO) & 1, nu = r - (r / nO) * nO;
+                const int ia = c / (2 * nO), csp = (c / nO) & 1, mu = c - (c / nO) * nO;
+                if (rs == csp) {
+                    const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
+                    for (int n = h1; n < h2; ++n) {
+                        double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                        x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
+                    }
+                }
+                v[i][j] = x;
+            }
+        }
+    gj2d_eliminate<DD, TY, TX>(v, colk, prs, perm, qinv, used, g.status);
     int dcol[CJ];
 #pragma unroll
     for (int j = 0; j < CJ; ++j) dcol[j] = (tx + j * TX < DD) ? perm[tx + j * TX] : -1;
@@ -591,6 +609,169 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
             __syncthreads();
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// d = 2b beyond what one CU holds (d = 288: 1.3 MB per matrix): 2 x 2 block
+// elimination.  With (z - H) = [[A, B], [C, E]] in b x b blocks and H
+// Hermitian, Im z > 0, both A and the Schur complement S = E - C A^-1 B are
+// invertible and their inverses have norm at most 1/Im z (they are
+// compressions of the resolvent), so no pivoting across blocks is needed:
+//   G = [[A^-1 - X G_BA, -X S^-1], [-S^-1 Y, S^-1]],  X = A^-1 B, Y = C A^-1.
+// The two b x b inversions use the register-resident elimination above;
+// the six b x b products run on a batched complex GEMM.  Matrices are
+// row-major in global memory, one per pair.
+
+// M = z - H(k), row-major d x d per pair
+__global__ __launch_bounds__(256) void assemble_m_kernel(green_params g, double2 *M)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double> *ph = reinterpret_cast<cplx<double> *>(smem);
+    const int tid = threadIdx.x, d = g.d, nO = g.nOrb, nO2 = nO * nO;
+    const long pl = blockIdx.x;
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, g.p0 + pl, k0, k1, k2, er, ei, w);
+    for (int n = tid; n < g.nhop; n += blockDim.x) {
+        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[n] = {cs, sn};
+    }
+    __syncthreads();
+    double2 *o = M + (size_t)pl * d * d;
+    for (int e = tid; e < d * d; e += blockDim.x) {
+        const int r = e / d, c = e - (e / d) * d;
+        double2 hv = g.h0[r + (size_t)c * d];
+        cplx<double> x = {(r == c ? er : 0.0) - hv.x, (r == c ? ei : 0.0) - hv.y};
+        const int ja = r / (2 * nO), rs = (r / nO) & 1, nu = r - (r / nO) * nO;
+        const int ia = c / (2 * nO), csp = (c / nO) & 1, mu = c - (c / nO) * nO;
+        if (rs == csp) {
+            const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
+            for (int n = h1; n < h2; ++n) {
+                double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                x = csub(x, cmul(cplx<double>{tv.x, tv.y}, ph[n]));
+            }
+        }
+        o[e] = make_double2(x.x, x.y);
+    }
+}
+
+// out = in^-1 for one b x b block per workgroup (row-major, leading
+// dimensions and pair strides given)
+template <int DD, int TY, int TX>
+__global__ __launch_bounds__(TY *TX) void inv_block_kernel(const double2 *in, long in_stride, int in_ld,
+                                                           double2 *out, long out_stride, int out_ld, int *status)
+{
+    constexpr int NT = TY * TX;
+    constexpr int RI = (DD + TY - 1) / TY, CJ = (DD + TX - 1) / TX;
+    __shared__ cplx<double> colk[2][DD], prs[2][DD];
+    __shared__ int perm[DD], qinv[DD];
+    __shared__ unsigned char used[DD];
+    const int tid = threadIdx.x;
+    const int ty = tid / TX, tx = tid - (tid / TX) * TX;
+    const long pl = blockIdx.x;
+    for (int r = tid; r < DD; r += NT) used[r] = 0;
+    __syncthreads();
+    const double2 *src = in + pl * in_stride;
+    cplx<double> v[RI][CJ];
+#pragma unroll
+    for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < CJ; ++j) {
+            const int r = ty + i * TY, c = tx + j * TX;
+            v[i][j] = {0.0, 0.0};
+            if (r < DD && c < DD) {
+                const double2 x = src[(size_t)r * in_ld + c];
+                v[i][j] = {x.x, x.y};
+            }
+        }
+    gj2d_eliminate<DD, TY, TX>(v, colk, prs, perm, qinv, used, status);
+    double2 *dst = out + pl * out_stride;
+#pragma unroll
+    for (int i = 0; i < RI; ++i) {
+        const int r = ty + i * TY;
+        if (r >= DD) continue;
+        const int dr = qinv[r];
+#pragma unroll
+        for (int j = 0; j < CJ; ++j) {
+            const int c = tx + j * TX;
+            if (c < DD) dst[(size_t)dr * out_ld + perm[c]] = make_double2(v[i][j].x, v[i][j].y);
+        }
+    }
+}
+
+// Out = alpha A B + beta Cin per pair (row-major, complex FP64, real alpha
+// and beta); 32 x 32 output tile per 256-thread workgroup, K staged by 16.
+__global__ __launch_bounds__(256) void zgemm_batched_kernel(int m, int n, int k, double alpha, const double2 *A,
+                                                            long sa, int lda, const double2 *B, long sb, int ldb,
+                                                            double beta, const double2 *Cin, long sc, int ldc,
+                                                            double2 *Out, long so, int ldo, int ntn)
+{
+    __shared__ cplx<double> As[32][17], Bs[16][33];
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const long pl = blockIdx.y;
+    const int i0 = (blockIdx.x / ntn) * 32, j0 = (blockIdx.x % ntn) * 32;
+    const double2 *Ap = A + pl * sa, *Bp = B + pl * sb;
+    cplx<double> acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = {0.0, 0.0};
+    for (int k0 = 0; k0 < k; k0 += 16) {
+        for (int u = 0; u < 2; ++u) {
+            const int v = t + 256 * u;
+            const int row = v >> 4, kk = v & 15;
+            const int gi = i0 + row, gk = k0 + kk;
+            double2 x = make_double2(0.0, 0.0);
+            if (gi < m && gk < k) x = Ap[(size_t)gi * lda + gk];
+            As[row][kk] = {x.x, x.y};
+            const int kr = v >> 5, col = v & 31;
+            const int gk2 = k0 + kr, gj = j0 + col;
+            double2 y = make_double2(0.0, 0.0);
+            if (gk2 < k && gj < n) y = Bp[(size_t)gk2 * ldb + gj];
+            Bs[kr][col] = {y.x, y.y};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const cplx<double> a0 = As[ty][kk], a1 = As[ty + 16][kk];
+            const cplx<double> b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+            acc[0][0] = {fma(a0.x, b0.x, fma(-a0.y, b0.y, acc[0][0].x)), fma(a0.x, b0.y, fma(a0.y, b0.x, acc[0][0].y))};
+            acc[0][1] = {fma(a0.x, b1.x, fma(-a0.y, b1.y, acc[0][1].x)), fma(a0.x, b1.y, fma(a0.y, b1.x, acc[0][1].y))};
+            acc[1][0] = {fma(a1.x, b0.x, fma(-a1.y, b0.y, acc[1][0].x)), fma(a1.x, b0.y, fma(a1.y, b0.x, acc[1][0].y))};
+            acc[1][1] = {fma(a1.x, b1.x, fma(-a1.y, b1.y, acc[1][1].x)), fma(a1.x, b1.y, fma(a1.y, b1.x, acc[1][1].y))};
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int gi = i0 + ty + 16 * a, gj = j0 + tx + 16 * b;
+            if (gi >= m || gj >= n) continue;
+            double re = alpha * acc[a][b].x, im = alpha * acc[a][b].y;
+            if (beta != 0.0) {
+                const double2 c = Cin[pl * sc + (size_t)gi * ldc + gj];
+                re += beta * c.x;
+                im += beta * c.y;
+            }
+            Out[pl * so + (size_t)gi * ldo + gj] = make_double2(re, im);
+        }
+}
+
+// panels (and the optional raw copy) from G held row-major in global memory
+template <typename P, int NT>
+__global__ __launch_bounds__(NT) void panel_from_global_kernel(green_params g, const double2 *G)
+{
+    __shared__ int ident[TH_MAX_D];
+    const int tid = threadIdx.x, d = g.d;
+    const long pl = blockIdx.x;
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, g.p0 + pl, k0, k1, k2, er, ei, w);
+    for (int r = tid; r < d; r += NT) ident[r] = r;
+    __syncthreads();
+    const double2 *Gp = G + (size_t)pl * d * d;
+    if (g.raw) {
+        double2 *o = g.raw + (size_t)pl * d * d;
+        for (int e = tid; e < d * d; e += NT) o[e] = Gp[e];
+    }
+    write_panel<P, NT>(g, pl, reinterpret_cast<const cplx<double> *>(Gp), ident, d, w);
 }
 
 template <typename P, int NT>
@@ -687,6 +868,62 @@ int launch_green_2d(th_handle *h, const th_green_args &a, const green_params &g)
     return TH_OK;
 }
 
+// G of a batch by 2 x 2 block elimination (d = 288), in chunks of pairs so
+// that the work space stays bounded
+template <typename P>
+int launch_green_block(th_handle *h, const th_green_args &a, const green_params &g0)
+{
+    const int d = h->d, b = d / 2;
+    const size_t dd = (size_t)d * d, bb = (size_t)b * b;
+    constexpr long CHUNK = 64;
+    const size_t need = (size_t)CHUNK * (2 * dd + 4 * bb) * sizeof(double2);
+    if (need > h->blk_bytes) {
+        hipFree(h->d_blk);
+        h->d_blk = nullptr;
+        h->blk_bytes = 0;
+        TH_HIP(hipMalloc(&h->d_blk, need));
+        h->blk_bytes = need;
+    }
+    double2 *M = reinterpret_cast<double2 *>(h->d_blk);
+    double2 *G = M + CHUNK * dd;
+    double2 *Ai = G + CHUNK * dd, *X = Ai + CHUNK * bb, *Y = X + CHUNK * bb, *S = Y + CHUNK * bb;
+    const size_t pstride = (size_t)h->nA2 * 2 * h->Mpad * (a.prec == TH_PREC_FP64 ? sizeof(double) : sizeof(float));
+    const int nt = (b + 31) / 32;
+    auto gemm = [&](long np, double alpha, const double2 *A_, long sa, int lda, const double2 *B_, long sb, int ldb,
+                    double beta, const double2 *C_, long sc, int ldc, double2 *O_, long so, int ldo) {
+        hipLaunchKernelGGL(zgemm_batched_kernel, dim3(nt * nt, (unsigned)np), dim3(256), 0, h->stream, b, b, b, alpha,
+                           A_, sa, lda, B_, sb, ldb, beta, C_, sc, ldc, O_, so, ldo, nt);
+    };
+    for (long c0 = 0; c0 < a.np; c0 += CHUNK) {
+        const long np = std::min(CHUNK, a.np - c0);
+        green_params g = g0;
+        g.p0 = a.p0 + c0;
+        g.panel = (char *)a.panel + (size_t)c0 * pstride;
+        g.raw = a.raw ? a.raw + (size_t)c0 * dd : nullptr;
+        hipLaunchKernelGGL(assemble_m_kernel, dim3((unsigned)np), dim3(256), sizeof(double2) * std::max(1, h->nhop),
+                           h->stream, g, M);
+        // Ai = A^-1
+        hipLaunchKernelGGL((inv_block_kernel<144, 16, 32>), dim3((unsigned)np), dim3(512), 0, h->stream,
+                           (const double2 *)M, (long)dd, d, Ai, (long)bb, b, g.status);
+        // X = Ai B, Y = C Ai
+        gemm(np, 1.0, Ai, bb, b, M + b, dd, d, 0.0, nullptr, 0, 0, X, bb, b);
+        gemm(np, 1.0, M + (size_t)b * d, dd, d, Ai, bb, b, 0.0, nullptr, 0, 0, Y, bb, b);
+        // S = E - C X
+        gemm(np, -1.0, M + (size_t)b * d, dd, d, X, bb, b, 1.0, M + (size_t)b * d + b, dd, d, S, bb, b);
+        // G_BB = S^-1
+        hipLaunchKernelGGL((inv_block_kernel<144, 16, 32>), dim3((unsigned)np), dim3(512), 0, h->stream,
+                           (const double2 *)S, (long)bb, b, G + (size_t)b * d + b, (long)dd, d, g.status);
+        // G_AB = -X G_BB, G_BA = -G_BB Y, G_AA = Ai - X G_BA
+        gemm(np, -1.0, X, bb, b, G + (size_t)b * d + b, dd, d, 0.0, nullptr, 0, 0, G + b, dd, d);
+        gemm(np, -1.0, G + (size_t)b * d + b, dd, d, Y, bb, b, 0.0, nullptr, 0, 0, G + (size_t)b * d, dd, d);
+        gemm(np, -1.0, X, bb, b, G + (size_t)b * d, dd, d, 1.0, Ai, bb, b, G, dd, d);
+        hipLaunchKernelGGL((panel_from_global_kernel<P, 256>), dim3((unsigned)np), dim3(256), 0, h->stream, g,
+                           (const double2 *)G);
+        TH_HIP(hipGetLastError());
+    }
+    return TH_OK;
+}
+
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
@@ -711,6 +948,7 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
         default: break;
         }
     }
+    if (h->d == 288) return launch_green_block<P>(h, a, g);
     switch (h->d) {
     case 18: return launch_green_2d<P, 18, 8, 8>(h, a, g);
     case 36: return launch_green_2d<P, 36, 8, 16>(h, a, g);
@@ -740,7 +978,7 @@ int *th_status_word(th_handle *h);
 
 int th_launch_green(th_handle *h, const th_green_args &a)
 {
-    if (h->d > TH_MAX_D_LDS && h->d != 72 && h->d != 144) {
+    if (h->d > TH_MAX_D_LDS && h->d != 72 && h->d != 144 && h->d != 288) {
         th_set_error("d = 2*nOrb*nA = " + std::to_string(h->d) + " has no inversion kernel");
         return TH_E_UNSUPPORTED;
     }

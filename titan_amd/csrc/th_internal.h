@@ -17,7 +17,7 @@ constexpr int TH_KC = 8;            // pairs per LDS stage of the GEMM
 constexpr int TH_GEMM_WAVES = 7;    // one wave per 16-row block of the tile
 constexpr int TH_MAX_NORB = 9;
 constexpr int TH_MAX_D_LDS = 64;    // largest d of the generic LDS-resident inversion
-constexpr int TH_MAX_D = 144;       // largest d of any inversion kernel
+constexpr int TH_MAX_D = 288;       // largest d of any inversion kernel (288: block elimination)
 constexpr int TH_MAX_NA = 32;
 
 inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
@@ -70,6 +70,8 @@ struct th_handle {
     std::vector<int> hop_order;  // stored hop n came from caller's entry hop_order[n]
     void *d_adj = nullptr;
     size_t adj_bytes = 0;
+    void *d_blk = nullptr;       // work space of the block elimination (d = 288)
+    size_t blk_bytes = 0;
 };
 
 // error plumbing
