@@ -75,17 +75,23 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
     from titan_amd import configs as CF
     from titan_amd import contour as Ct
     from titan_amd.chi import ChiWorkload
+    from titan_amd import autotune as AT
     spec, run = CF.system(cfg), CF.run_params(cfg)
     wl = ChiWorkload(spec, run)
     m, h = wl.model, wl.h
     x2, p2 = Ct.real_epoints(m.Ef, omega, run["parts3"], run["n3gl"])
+    # kernel shapes tuned on every 8th k-point, as for the main workload
+    ks, ws = wl.kp[::8], wl.wk[::8] * 8
+    h.set_imag_points(np.tile(ks, (len(wl.y), 1)), np.repeat(wl.y, len(ws)), np.outer(wl.wy, ws).ravel())
+    best = AT.tune(h, run["q"], omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=B.PREC_MIXED, repeat=1)["best"]
+    h.set_imag_points(np.tile(wl.kp, (len(wl.y), 1)), np.repeat(wl.y, wl.nk), np.outer(wl.wy, wl.wk).ravel())
     rng = np.random.default_rng(1)
     W = rng.standard_normal((m.D, m.D)) + 1j * rng.standard_normal((m.D, m.D))
     out = {"workload": "cfg%d: %s, k-mesh %s, %d imag + %d real energies, one omega" %
                        (cfg, spec["name"], "x".join(map(str, run["nk"])), len(wl.y), wl.n_x),
-           "pairs": wl.pairs([omega])}
+           "pairs": wl.pairs([omega]), "tuning": best}
     for name, prec in (("fp64", B.PREC_FP64), ("mixed", B.PREC_MIXED)):
-        tf, tb = [], []
+        tf, tb, stages = [], [], None
         for r in range(reps + 1):
             t0 = time.perf_counter()
             c0 = h.chi0(run["q"], omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=prec)
@@ -97,7 +103,11 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
             if r:
                 tf.append(t1 - t0)
                 tb.append(t2 - t1)
-        out[name] = {"forward_ms": 1e3 * min(tf), "adjoint_ms": 1e3 * min(tb), "ratio": min(tb) / min(tf)}
+                tm = h.last_timing()
+                stages = {"green_ms": tm["green_ms"], "panel_adjoint_ms": tm["contract_ms"],
+                          "hamiltonian_adjoint_ms": tm["finalize_ms"], "dyson_adjoint_ms": tm["dyson_ms"]}
+        out[name] = {"forward_ms": 1e3 * min(tf), "adjoint_ms": 1e3 * min(tb), "ratio": min(tb) / min(tf),
+                     "adjoint_stages": stages}
     wl.close()
     return out
 
@@ -111,6 +121,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default kernel shapes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the chi(omega) path is built for a single MI355X "
@@ -134,6 +145,22 @@ def main():
     log("set-up %.1f s, n_imag %d, n_k %d" % (time.perf_counter() - t0, wl.nk * len(wl.y), wl.nk))
     m = wl.model
     P = wl.pairs(om_time)
+    tuning = None
+    if not args.no_tune:
+        # kernel shapes tuned on every 64th k-point (the autotuner of the library,
+        # titan_amd/autotune.py), then the full work list is restored
+        from titan_amd import autotune as AT
+        from titan_amd import contour as Ct
+        step = max(1, wl.nk // 1728)
+        ks, ws = wl.kp[::step], wl.wk[::step] * step
+        wl.h.set_imag_points(np.tile(ks, (len(wl.y), 1)), np.repeat(wl.y, len(ws)), np.outer(wl.wy, ws).ravel())
+        om_t = float(om_time[-1])
+        x2, p2 = Ct.real_epoints(m.Ef, om_t, run["parts3"], run["n3gl"])
+        r = AT.tune(wl.h, run["q"], om_t, m.Ef, run["eta"], run["etap"], x2, p2, precision=B.PREC_MIXED, repeat=1)
+        tuning = {"best": r["best"], "sample_k": len(ws), "seconds": round(r["seconds"], 1),
+                  "default_ms": r["table"][0][1], "best_ms": r["best_ms"]}
+        log("autotune", tuning)
+        wl.h.set_imag_points(np.tile(wl.kp, (len(wl.y), 1)), np.repeat(wl.y, wl.nk), np.outer(wl.wy, wl.wk).ravel())
     res = {}
     for name, prec in (("mixed", B.PREC_MIXED), ("fp64", B.PREC_FP64)):
         if name == "fp64" and args.no_fp64:
@@ -201,6 +228,7 @@ def main():
         im_f = np.abs(ref[:, 0, 0].imag)
         out["peak_bin_match"] = bool(np.argmax(im_m) == np.argmax(im_f))
     out["dyson_cond_max"] = float(np.max(mx["cond"]))
+    out["tuning"] = tuning
     if not args.no_adjoint:
         try:
             out["adjoint_cost"] = adjoint_cost()

@@ -116,9 +116,11 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
  * in the same convention.  chi0 and chi are the forward results at this
  * omega (from th_chi0 and th_dyson with the same arguments).  With U == NULL
  * the Dyson step is skipped: chi_hat is then the cotangent of chi0 and chi is
- * not read.  The Green functions are recomputed, never stored; 'precision'
- * selects the panel precision of the two adjoint contractions as in
- * th_chi0 (the Green functions, the adjoint of H and every sum are FP64). */
+ * not read.  The Green functions are recomputed, not stored for the
+ * backward (the omega-independent imaginary-axis ones are read from the
+ * forward's panel cache when it holds them); 'precision' selects the panel
+ * precision of the two adjoint contractions as in th_chi0 (the adjoint of H
+ * and every sum are FP64). */
 int th_chi_adjoint(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
                    int n_x, const double *x2, const double *p2, int precision,
                    const double *chi0, const double *U, const double *chi, const double *chi_hat,
@@ -147,6 +149,9 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *   "green_variant"  inversion kernel shape: 0 = 2-D register-blocked (default
  *                    tiles), 1 = 1-D ownership, 2 = one wave per matrix,
  *                    3 = two waves per matrix (2 and 3 where compiled for d)
+ *   "contract_3m"    1 = the contraction forms each complex product from three
+ *                    real MFMA products (Ar Br, Ai Bi, (Ar+Ai)(Br+Bi)) instead
+ *                    of four; 0 = four (default)
  *   "graph"          1 = th_chi_sweep captures one frequency step (chi0, Dyson,
  *                    traces) as a hipGraph at its first frequency with and
  *                    without real axis and replays it for the others, omega

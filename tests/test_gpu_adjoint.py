@@ -9,7 +9,10 @@ of titan_amd.autodiff.
     on-site energy through torch.autograd against central finite differences
     (h = 1e-5, five-point stencil) of the C oracle's chi built from the
     perturbed parameter set, to 1e-6 relative.
-  * bitwise reproducibility of the adjoint.
+  * bitwise reproducibility of the adjoint; the three-product contraction.
+  The FP64 calls follow an FP64 forward, so they read the imaginary-axis B
+  side from the forward's panel cache; the mixed calls find an FP64 cache
+  and recompute it.
 """
 import copy
 
@@ -50,6 +53,8 @@ CASES = {
     "bcc2_real_chi0": (lambda: CF.bcc_supercell(2), 2, 4, 4, 2e-3, (0.0, 0.0, 0.07), False),
     "bcc2_imag_U": (lambda: CF.bcc_supercell(2), 2, 4, 4, 0.0, (0.0, 0.0, 0.0), True),
     "pchain_real_U0": (lambda: CF.chain_p(), 5, 6, 6, 2e-3, (0.1, 0.0, 0.0), True),
+    "bcc3_real_U": (lambda: CF.system(2), 1, 3, 3, 3e-3, (0.0, 0.0, 0.0), True),
+    "bcc4_imag_chi0": (lambda: CF.bcc_supercell(4), 1, 3, 3, 0.0, (0.0, 0.0, 0.0), False),
 }
 
 
@@ -83,6 +88,10 @@ def test_adjoint_matches_fp64_reference(name):
     # bitwise reproducible
     th3, hh3, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"], p2=c["p2"])
     assert np.array_equal(th3, th) and np.array_equal(hh3, hh)
+    # three real MFMA products per complex one in the adjoint contractions
+    h.tune(contract_3m=1)
+    th4, hh4, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"], p2=c["p2"])
+    assert _rel(th4, rt) <= 1e-10 and _rel(hh4, rh) <= 1e-10
     h.close()
 
 

@@ -417,3 +417,25 @@ def test_uneven_batch_matches_oracle():
             if m.nA <= 4:
                 assert rel(s[w], O.trace(m.nA, m.nOrb, O.dyson(c0, o.umat()))) < 1e-10, (m.nA, om)
     ub.close()
+
+
+@pytest.mark.parametrize("which", ["cfg0", "cfg2"])
+def test_three_product_contraction(which, cfg0, cfg2):
+    """th_tune(contract_3m=1): three real MFMA products per complex one.
+    FP64 path still within 1e-10 of the oracle; mixed path within 1e-6."""
+    m, s = {"cfg0": cfg0, "cfg2": cfg2}[which]
+    nk = 6 if which == "cfg0" else 3
+    (K, Y, W), (kp, wk) = _work(m, nk, n1gl=16)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    h.tune(contract_3m=1)
+    om = 4e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 8)
+    o = s.chi0([0, 0, 0], om, 5e-4, 5e-4, (K, Y, W), (kp, wk), (x2, p2))
+    g = h.chi0([0, 0, 0], om, m.Ef, 5e-4, 5e-4, x2, p2, precision=B.PREC_FP64)
+    assert rel(g, o) < 1e-10, rel(g, o)
+    gm = h.chi0([0, 0, 0], om, m.Ef, 5e-4, 5e-4, x2, p2, precision=B.PREC_MIXED)
+    print(which, "3M rel fp64 %.2e mixed %.2e" % (rel(g, o), rel(gm, o)), flush=True)
+    assert rel(gm, o) < 1e-6, rel(gm, o)
+    h.close()

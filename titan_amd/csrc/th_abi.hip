@@ -611,6 +611,9 @@ int th_tune(th_handle *h, const char *key, long value)
             TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)value * h->nA2 * 2 * MM));
         }
         h->splits = (int)value;
+    } else if (k == "contract_3m") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: contract_3m is 0 or 1"); return TH_E_ARG; }
+        h->contract_3m = (int)value;
     } else if (k == "graph") {
         if (value < 0 || value > 1) { th_set_error("th_tune: graph is 0 or 1"); return TH_E_ARG; }
         h->use_graph = (int)value;

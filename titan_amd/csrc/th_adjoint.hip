@@ -16,8 +16,10 @@
 //   inversion  Hhat = G^H Ghat G^H for both Green functions of the pair;
 //   assembly   that_n = sum_p exp(-i k_p.R_n) sum_s Hhat_p[(jb,s,nu),(ib,s,mu)],
 //              h0hat = sum_p Hhat_p.
-// Green functions are recomputed batch by batch (never stored across
-// batches).  All sums are FP64 and run in a fixed order, so the result is
+// Green functions are recomputed batch by batch (never stored for the
+// backward), except the omega-independent imaginary-axis ones G(k, E_F +
+// i(y + eta')) that the forward keeps in its panel cache: those are read from
+// there.  All sums are FP64 and run in a fixed order, so the result is
 // bitwise reproducible.
 #include <algorithm>
 #include <cmath>
@@ -369,10 +371,142 @@ __global__ __launch_bounds__(256) void bmm_kernel(const double2 *__restrict__ A,
 }
 
 // ---------------------------------------------------------------------------
+// Hhat = G^H Ghat G^H with G and Ghat of a pair in LDS (d a multiple of 3,
+// d <= 54).  A thread owns a 3 x 3 block of the d x d result, so each LDS
+// value it loads feeds three complex FMAs; ppw pairs share a workgroup of
+// ppw (d/3)^2 threads (rounded up to whole waves).  X = Ghat G^H is held in
+// registers, then overwrites Ghat in LDS for the second product.  Ghat is
+// assembled from the panel cotangent as assemble_kernel does.
+__device__ inline double2 ghat_entry(const assemble_params &a, const double *H, int r, int c, double w)
+{
+    const int bs = a.bs, nA = a.nA, Mpad = a.Mpad;
+    const int ra = r / bs, ca = c / bs, rr = r - ra * bs, cc = c - ca * bs;
+    auto hat = [&](int blk, int idx) { return make_double2(H[(size_t)blk * 2 * Mpad + idx], H[(size_t)blk * 2 * Mpad + Mpad + idx]); };
+    if (a.mode == TH_G_IMAG_A) {
+        const double2 x = hat(ra * nA + ca, rr * bs + cc);
+        return make_double2(w * x.x, w * x.y);
+    }
+    if (a.mode == TH_G_IMAG_B) return hat(ca * nA + ra, rr * bs + cc);
+    if (a.mode == TH_G_REAL_A) {
+        const double2 x = hat(ra * nA + ca, rr * bs + cc), y = hat(ca * nA + ra, cc * bs + rr);
+        return make_double2(-w * (x.y - y.y), w * (x.x + y.x));
+    }
+    return zconj(hat(ra * nA + ca, cc * bs + rr));
+}
+
+__global__ __launch_bounds__(512) void hhat_fused_kernel(assemble_params a, const double2 *__restrict__ Graw, int ppw)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int d = a.d, dd = d * d, nb3 = d / 3, nblk3 = nb3 * nb3;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const long pbase = (long)blockIdx.x * ppw;
+    // stage G and Ghat of the workgroup's pairs
+    for (int e = tid; e < ppw * dd; e += NT) {
+        const int s = e / dd, f = e - s * dd;
+        const long pl = pbase + s;
+        double2 *G = reinterpret_cast<double2 *>(smem) + (size_t)s * 2 * dd;
+        double2 *Gh = G + dd;
+        if (pl >= a.np) {
+            G[f] = make_double2(0.0, 0.0);
+            Gh[f] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const long p = a.p0 + pl;
+        double w = 0.0;
+        if (a.mode == TH_G_IMAG_A) w = a.wim[p];
+        if (a.mode == TH_G_REAL_A) {
+            const long ie = p / a.nk_re, ik = p - ie * a.nk_re;
+            w = a.p2[ie] * a.wre[ik];
+        }
+        const double *H = a.hat + (size_t)pl * a.nA2 * 2 * a.Mpad;
+        G[f] = Graw[(size_t)pl * dd + f];
+        Gh[f] = ghat_entry(a, H, f / d, f - (f / d) * d, w);
+    }
+    __syncthreads();
+    const int s = tid / nblk3, bidx = tid - s * nblk3;
+    const bool active = s < ppw && pbase + s < a.np;
+    const int r0 = 3 * (bidx / nb3), c0 = 3 * (bidx - (bidx / nb3) * nb3);
+    double2 *G = reinterpret_cast<double2 *>(smem) + (size_t)(s < ppw ? s : 0) * 2 * dd;
+    double2 *Gh = G + dd;
+    double2 acc[3][3];
+    // X = Ghat G^H: X[r][c] = sum_m Ghat[r][m] conj(G[c][m])
+    if (active) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
+        for (int m = 0; m < d; ++m) {
+            double2 x[3], y[3];
+            for (int i = 0; i < 3; ++i) x[i] = Gh[(r0 + i) * d + m];
+            for (int j = 0; j < 3; ++j) y[j] = G[(c0 + j) * d + m];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                             fma(x[i].y, y[j].x, fma(-x[i].x, y[j].y, acc[i][j].y)));
+        }
+    }
+    __syncthreads();
+    if (active)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Gh[(r0 + i) * d + c0 + j] = acc[i][j];
+    __syncthreads();
+    if (!active) return;
+    // Hhat = G^H X: Hhat[r][c] = sum_m conj(G[m][r]) X[m][c]
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
+    for (int m = 0; m < d; ++m) {
+        double2 x[3], y[3];
+        for (int i = 0; i < 3; ++i) x[i] = G[m * d + r0 + i];
+        for (int j = 0; j < 3; ++j) y[j] = Gh[m * d + c0 + j];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                         fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y)));
+    }
+    double2 *out = a.out + (size_t)(pbase + s) * dd;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[(r0 + i) * d + c0 + j] = acc[i][j];
+}
+
+// raw G of the imaginary-axis B side from the omega-independent panel cache:
+// B(i,j) = vec G(j,i), so G[(j bs + rr) d + i bs + cc] = B[p][i nA + j][rr bs + cc]
+template <typename P>
+__global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, int bs, int Mpad, double2 *raw)
+{
+    const int d = nA * bs;
+    const size_t dd = (size_t)d * d;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)np * dd) return;
+    const long pl = (long)(e / dd);
+    const int f = (int)(e - pl * dd);
+    const int r = f / d, c = f - (f / d) * d;
+    const int j = r / bs, rr = r - j * bs, i = c / bs, cc = c - i * bs;
+    const P *b = panel + ((size_t)pl * nA * nA + (size_t)i * nA + j) * 2 * Mpad;
+    raw[e] = make_double2((double)b[rr * bs + cc], (double)b[Mpad + rr * bs + cc]);
+}
+
+// exp(-i k_p.R_n) for the pairs of a batch: ph[pl * nhop + n]
+__global__ void phase_kernel(long p0, long np, int nhop, int imag, int side_q, long nk_re, double q0, double q1,
+                             double q2, const double *kim, const double *kre, const double *R, double2 *ph)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)np * nhop) return;
+    const long pl = (long)(e / nhop);
+    const int n = (int)(e - (size_t)pl * nhop);
+    const long p = p0 + pl;
+    const double *k = imag ? kim + 3 * p : kre + 3 * (p % nk_re);
+    double k0 = k[0], k1 = k[1], k2 = k[2];
+    if (side_q) { k0 += q0; k1 += q1; k2 += q2; }
+    double sn, cs;
+    sincos(k0 * R[3 * n] + k1 * R[3 * n + 1] + k2 * R[3 * n + 2], &sn, &cs);
+    ph[e] = make_double2(cs, -sn);
+}
+
+// ---------------------------------------------------------------------------
 // Partial sums over a chunk of pairs of the hop and h0 cotangents:
 //   that[n][nu + nOrb mu] = sum_p exp(-i k_p.R_n) sum_s Hhat_p[(2jb+s)nO+nu][(2ib+s)nO+mu]
-//   h0hat[r + d c]        = sum_p Hhat_p[r][c]
-// Output e < nhop nO^2 is a hop entry, the rest the h0 entries.
+//   h0hat[r d + c]        = sum_p Hhat_p[r][c]   (row-major; the host transposes)
+// Output e < nhop nO^2 is a hop entry, the rest the h0 entries.  The hop
+// entries read the spin-summed blocks S_p, so consecutive outputs read
+// consecutive memory.
 struct hreduce_params {
     int side_q;              // 1: k + q (the Gq side)
     int imag;                // 1: imaginary-axis pair list
@@ -381,42 +515,94 @@ struct hreduce_params {
     double q0, q1, q2;
     const double *kim, *kre, *R;
     const int *jb, *ib;
+    int nA;
     const double2 *Hh;       // [np][d][d]
+    const double2 *S;        // [np][nA nA][nOrb nOrb] spin-summed blocks of Hh
+    const double2 *ph;       // [np][nhop] exp(-i k_p.R_n)
     double2 *partial;        // [nchunk][nout]
 };
 
-__global__ void hreduce_kernel(hreduce_params a)
+// Hop entries: a thread owns one entry loc of the spin-summed block of a
+// group of at most HG hops of one atom pair (hops are stored grouped by atom
+// pair), reads S once per pair and updates HG accumulators.
+constexpr int HG = 8;
+
+// A workgroup = 64 outputs x 4 waves; wave y takes the pairs pl = c0 + y,
+// c0 + y + 4, ... of the chunk and the four wave sums are added in a fixed
+// order through LDS.
+__global__ __launch_bounds__(256) void hop_reduce_kernel(hreduce_params a, const int *__restrict__ groups,
+                                                         int ngroups)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nout) return;
+    __shared__ double2 red[4][64][HG];
+    const int nO2 = a.nOrb * a.nOrb;
+    const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + x;
+    const bool on = t < ngroups * nO2;
+    const int g = on ? t / nO2 : 0, loc = on ? t - g * nO2 : 0;
+    const int b = groups[3 * g], h0 = groups[3 * g + 1], cnt = on ? groups[3 * g + 2] : 0;
     const long c0 = (long)blockIdx.y * a.chunk, c1 = min(a.np, c0 + a.chunk);
-    const int nO = a.nOrb, nO2 = nO * nO, d = a.d;
-    const size_t dd = (size_t)d * d;
-    double2 s = make_double2(0.0, 0.0);
-    if (e < a.nhop * nO2) {
-        const int n = e / nO2, loc = e - n * nO2;
-        const int nu = loc % nO, mu = loc / nO;
-        const int j = a.jb[n], i = a.ib[n];
-        const double R0 = a.R[3 * n], R1 = a.R[3 * n + 1], R2 = a.R[3 * n + 2];
-        const size_t o0 = (size_t)(2 * j * nO + nu) * d + 2 * i * nO + mu;
-        const size_t o1 = (size_t)((2 * j + 1) * nO + nu) * d + (2 * i + 1) * nO + mu;
-        for (long pl = c0; pl < c1; ++pl) {
-            const long p = a.p0 + pl;
-            const double *k = a.imag ? a.kim + 3 * p : a.kre + 3 * (p % a.nk_re);
-            double k0 = k[0], k1 = k[1], k2 = k[2];
-            if (a.side_q) { k0 += a.q0; k1 += a.q1; k2 += a.q2; }
-            double sn, cs;
-            sincos(k0 * R0 + k1 * R1 + k2 * R2, &sn, &cs);
-            const double2 *Hp = a.Hh + pl * dd;
-            const double2 h = zadd(Hp[o0], Hp[o1]);
-            s = zadd(s, zmul(h, make_double2(cs, -sn)));
+    const size_t sstride = (size_t)a.nA * a.nA * nO2;
+    double2 acc[HG];
+#pragma unroll
+    for (int c = 0; c < HG; ++c) acc[c] = make_double2(0.0, 0.0);
+    if (on)
+        for (long pl = c0 + y; pl < c1; pl += 4) {
+            const double2 sv = a.S[pl * sstride + (size_t)b * nO2 + loc];
+            const double2 *ph = a.ph + pl * a.nhop + h0;
+#pragma unroll
+            for (int c = 0; c < HG; ++c)
+                if (c < cnt) acc[c] = zadd(acc[c], zmul(sv, ph[c]));
         }
-    } else {
-        const int f = e - a.nhop * nO2;
-        const int r = f % d, c = f / d;
-        for (long pl = c0; pl < c1; ++pl) s = zadd(s, a.Hh[pl * dd + (size_t)r * d + c]);
-    }
-    a.partial[(size_t)blockIdx.y * a.nout + e] = s;
+#pragma unroll
+    for (int c = 0; c < HG; ++c) red[y][x][c] = acc[c];
+    __syncthreads();
+    if (y != 0 || !on) return;
+    double2 *out = a.partial + (size_t)blockIdx.y * a.nout;
+#pragma unroll
+    for (int c = 0; c < HG; ++c)
+        if (c < cnt) {
+            double2 v = red[0][x][c];
+            v = zadd(v, red[1][x][c]);
+            v = zadd(v, red[2][x][c]);
+            v = zadd(v, red[3][x][c]);
+            out[(size_t)(h0 + c) * nO2 + loc] = v;
+        }
+}
+
+// h0 entries (row-major here; the host transposes), same work split
+__global__ __launch_bounds__(256) void h0_reduce_kernel(hreduce_params a)
+{
+    __shared__ double2 red[4][64];
+    const int dd = a.d * a.d;
+    const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
+    const int f = blockIdx.x * 64 + x;
+    const long c0 = (long)blockIdx.y * a.chunk, c1 = min(a.np, c0 + a.chunk);
+    double2 s = make_double2(0.0, 0.0);
+    if (f < dd)
+        for (long pl = c0 + y; pl < c1; pl += 4) s = zadd(s, a.Hh[pl * (size_t)dd + f]);
+    red[y][x] = s;
+    __syncthreads();
+    if (y != 0 || f >= dd) return;
+    double2 v = red[0][x];
+    v = zadd(v, red[1][x]);
+    v = zadd(v, red[2][x]);
+    v = zadd(v, red[3][x]);
+    a.partial[(size_t)blockIdx.y * a.nout + (size_t)a.nhop * a.nOrb * a.nOrb + f] = v;
+}
+
+// S[pl][j nA + i][nu + nOrb mu] = sum over spin s of Hhat[(2j+s)nOrb+nu][(2i+s)nOrb+mu]
+__global__ void spin_sum_kernel(long np, int nA, int nOrb, const double2 *__restrict__ Hh, double2 *__restrict__ S)
+{
+    const int nO2 = nOrb * nOrb, per = nA * nA * nO2, d = 2 * nOrb * nA;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)np * per) return;
+    const long pl = (long)(e / per);
+    const int f = (int)(e - (size_t)pl * per);
+    const int b = f / nO2, loc = f - b * nO2;
+    const int j = b / nA, i = b - j * nA, nu = loc % nOrb, mu = loc / nOrb;
+    const double2 *Hp = Hh + (size_t)pl * d * d;
+    S[e] = zadd(Hp[(size_t)(2 * j * nOrb + nu) * d + 2 * i * nOrb + mu],
+                Hp[(size_t)((2 * j + 1) * nOrb + nu) * d + (2 * i + 1) * nOrb + mu]);
 }
 
 // acc[e] += sum_ch partial[ch][e], fixed order
@@ -569,10 +755,11 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     const int D = h->D, d = h->d, nO = h->nOrb;
     const size_t DD = (size_t)D * D, dd = (size_t)d * d;
     const int nout = h->nhop * nO * nO + (int)dd;
-    constexpr int CHUNK = 64;
+    constexpr int CHUNK = 256;
     // pairs per batch: adjoint buffers within twice the panel budget
     const size_t per = (size_t)2 * h->nA2 * 2 * h->Mpad * sizeof(double) + 5 * dd * sizeof(double2) +
-                       (size_t)nout * sizeof(double2) / CHUNK + 64;
+                       (size_t)nout * sizeof(double2) / CHUNK + (size_t)h->nhop * sizeof(double2) +
+                       (size_t)h->nA2 * nO * nO * sizeof(double2) + 64;
     const long n_re = real_axis ? (long)n_x * h->nk_re : 0;
     const long total = std::max(h->n_im, n_re);
     long bp = (long)std::max<size_t>(TH_KC, ((size_t)h->batch_mb << 21) / per);
@@ -591,6 +778,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     need += (((size_t)nch_max * nout * sizeof(double2)) + 255) & ~(size_t)255;  // partial sums
     need += 2 * (((size_t)bp * h->nA2 * 2 * h->Mpad * sizeof(double) + 255) & ~(size_t)255);
     need += 5 * (((size_t)bp * dd * sizeof(double2) + 255) & ~(size_t)255);
+    need += ((size_t)bp * std::max(1, h->nhop) * sizeof(double2) + 255) & ~(size_t)255;
+    need += ((size_t)bp * h->nA2 * nO * nO * sizeof(double2) + 255) & ~(size_t)255;
+    need += ((size_t)3 * (h->nhop + h->nA2) * sizeof(int) + 255) & ~(size_t)255;   // hop groups
     need += 4096;
     if (need > h->adj_bytes) {
         hipFree(h->d_adj);
@@ -613,13 +803,28 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     double2 *gA = ar.take<double2>((size_t)bp * dd);
     double2 *gB = ar.take<double2>((size_t)bp * dd);
     double2 *tmp = ar.take<double2>((size_t)bp * dd);
+    double2 *d_ph = ar.take<double2>((size_t)bp * std::max(1, h->nhop));
+    double2 *d_S = ar.take<double2>((size_t)bp * h->nA2 * nO * nO);
     double2 *d_Uhat = U_hat ? ar.take<double2>(DD) : nullptr;
 
+    // hop groups: at most HG hops of one atom pair (stored hops are grouped by pair)
+    std::vector<int> hoff(h->nA2 + 1), grp;
+    TH_HIP(hipMemcpy(hoff.data(), h->d_hoff, sizeof(int) * (h->nA2 + 1), hipMemcpyDeviceToHost));
+    for (int b = 0; b < h->nA2; ++b)
+        for (int n = hoff[b]; n < hoff[b + 1]; n += HG) {
+            grp.push_back(b);
+            grp.push_back(n);
+            grp.push_back(std::min(HG, hoff[b + 1] - n));
+        }
+    const int ngroups = (int)grp.size() / 3;
+    int *d_groups = ar.take<int>(std::max<size_t>(grp.size(), 1));
     stage_timer tm(h);
     const int idall = tm.begin(4);
     for (auto &c : h->n_launch) c = 0;
     int rc = th_reset_status(h);
     if (rc) return rc;
+    if (ngroups > 0)
+        TH_HIP(hipMemcpyAsync(d_groups, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice, h->stream));
     if ((rc = th_ensure_panels(h, bp))) return rc;
     if (real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
     TH_HIP(hipMemcpyAsync(d_chat, chi_hat, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));
@@ -645,6 +850,8 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     if ((rc = zero2(h, d_grad, nout))) return rc;
     tm.end(id);
     const size_t accsz = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
+    const bool use_bcache = h->d_bcache && h->bcache_prec == precision && h->bcache_key[0] == Ef &&
+                            h->bcache_key[1] == etap;
     const int ntile = (d + 31) / 32;
     // 3. pair batches of each axis
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
@@ -660,13 +867,29 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             a.panel = h->d_pA;
             a.raw = rawA;
             if ((rc = th_launch_green(h, a))) return rc;
-            a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
-            a.panel = h->d_pB;
-            a.raw = rawB;
-            if ((rc = th_launch_green(h, a))) return rc;
+            // the imaginary-axis B side does not depend on omega: when the forward's
+            // panel cache holds it (same E_F, eta', precision), read G from there
+            const void *pB = h->d_pB;
+            if (axis == 0 && use_bcache) {
+                const size_t psz = precision == TH_PREC_FP64 ? sizeof(double) : sizeof(float);
+                pB = (const char *)h->d_bcache + (size_t)p0 * h->nA2 * 2 * h->Mpad * psz;
+                const unsigned g = nblk((size_t)np * dd, 256);
+                if (precision == TH_PREC_FP64)
+                    hipLaunchKernelGGL(raw_from_bpanel<double>, dim3(g), dim3(256), 0, h->stream, (const double *)pB,
+                                       np, h->nA, h->bs, h->Mpad, rawB);
+                else
+                    hipLaunchKernelGGL(raw_from_bpanel<float>, dim3(g), dim3(256), 0, h->stream, (const float *)pB, np,
+                                       h->nA, h->bs, h->Mpad, rawB);
+                TH_HIP(hipGetLastError());
+            } else {
+                a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
+                a.panel = h->d_pB;
+                a.raw = rawB;
+                if ((rc = th_launch_green(h, a))) return rc;
+            }
             tm.end(id);
             id = tm.begin(1);
-            if ((rc = th_launch_panel_adjoint(h, h->d_pB, precision, That, np, 1, hatA))) return rc;
+            if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA))) return rc;
             if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB))) return rc;
             h->n_launch[1] += 2;
             tm.end(id);
@@ -681,14 +904,32 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 double2 *gh = side == 0 ? gA : gB;
                 const double2 *G = side == 0 ? rawA : rawB;
                 ap.out = gh;
-                hipLaunchKernelGGL(assemble_kernel, dim3(nblk((size_t)np * dd, 256)), dim3(256), 0, h->stream, ap);
-                TH_HIP(hipGetLastError());
-                // Hhat = G^H (Ghat G^H)
-                const unsigned nwg = (unsigned)(np * ntile * ntile);
-                hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, (const double2 *)gh, 0, G, 1, tmp, d,
-                                   ntile);
-                hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, G, 1, (const double2 *)tmp, 0, gh, d,
-                                   ntile);
+                if (d <= 54 && d % 3 == 0) {
+                    // Hhat = G^H Ghat G^H with the pairs' G, Ghat in LDS, 3 x 3 register blocks
+                    const int nblk3 = (d / 3) * (d / 3);
+                    const int ppw = std::max(1, 256 / nblk3);
+                    const int nt = ((ppw * nblk3 + 63) / 64) * 64;
+                    const size_t lds = (size_t)ppw * 2 * dd * sizeof(double2);
+                    TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(hhat_fused_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    hipLaunchKernelGGL(hhat_fused_kernel, dim3((unsigned)((np + ppw - 1) / ppw)), dim3(nt), lds,
+                                       h->stream, ap, G, ppw);
+                    TH_HIP(hipGetLastError());
+                } else {
+                    hipLaunchKernelGGL(assemble_kernel, dim3(nblk((size_t)np * dd, 256)), dim3(256), 0, h->stream, ap);
+                    TH_HIP(hipGetLastError());
+                    // Hhat = G^H (Ghat G^H)
+                    const unsigned nwg = (unsigned)(np * ntile * ntile);
+                    hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, (const double2 *)gh, 0, G, 1, tmp,
+                                       d, ntile);
+                    hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, G, 1, (const double2 *)tmp, 0, gh,
+                                       d, ntile);
+                    TH_HIP(hipGetLastError());
+                }
+                hipLaunchKernelGGL(phase_kernel, dim3(nblk((size_t)np * std::max(1, h->nhop), 256)), dim3(256), 0,
+                                   h->stream, p0, np, h->nhop, axis == 0 ? 1 : 0, side == 0 ? 1 : 0, h->nk_re, q[0], q[1],
+                                   q[2], (const double *)h->d_kim, (const double *)h->d_kre, (const double *)h->d_R,
+                                   d_ph);
                 TH_HIP(hipGetLastError());
                 hreduce_params hp{};
                 hp.side_q = side == 0 ? 1 : 0;
@@ -698,9 +939,17 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 hp.q0 = q[0]; hp.q1 = q[1]; hp.q2 = q[2];
                 hp.kim = h->d_kim; hp.kre = h->d_kre; hp.R = h->d_R; hp.jb = h->d_jb; hp.ib = h->d_ib;
                 hp.Hh = gh;
+                hp.nA = h->nA;
+                hp.S = d_S;
+                hp.ph = d_ph;
                 hp.partial = d_part;
+                hipLaunchKernelGGL(spin_sum_kernel, dim3(nblk((size_t)np * h->nA2 * nO * nO, 256)), dim3(256), 0,
+                                   h->stream, np, h->nA, nO, (const double2 *)gh, d_S);
                 const int nch = (int)((np + CHUNK - 1) / CHUNK);
-                hipLaunchKernelGGL(hreduce_kernel, dim3(nblk(nout, 128), nch), dim3(128), 0, h->stream, hp);
+                if (ngroups > 0)
+                    hipLaunchKernelGGL(hop_reduce_kernel, dim3(nblk((size_t)ngroups * nO * nO, 64), nch), dim3(256), 0,
+                                       h->stream, hp, (const int *)d_groups, ngroups);
+                hipLaunchKernelGGL(h0_reduce_kernel, dim3(nblk(dd, 64), nch), dim3(256), 0, h->stream, hp);
                 hipLaunchKernelGGL(hsum_kernel, dim3(nblk(nout, 256)), dim3(256), 0, h->stream, (const double2 *)d_part,
                                    nch, nout, d_grad);
                 TH_HIP(hipGetLastError());
@@ -718,7 +967,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     double2 *th = reinterpret_cast<double2 *>(t_hat);
     for (int n = 0; n < h->nhop; ++n)
         for (int e = 0; e < nO2; ++e) th[(size_t)h->hop_order[n] * nO2 + e] = g[(size_t)n * nO2 + e];
-    std::copy(g.begin() + (size_t)h->nhop * nO2, g.end(), reinterpret_cast<double2 *>(h0_hat));
+    double2 *h0o = reinterpret_cast<double2 *>(h0_hat);
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c) h0o[r + (size_t)d * c] = g[(size_t)h->nhop * nO2 + (size_t)r * d + c];
     rc = th_check_status(h);
     tm.collect();
     return rc;

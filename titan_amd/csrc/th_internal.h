@@ -61,6 +61,7 @@ struct th_handle {
     long n_launch[4] = {0, 0, 0, 0};
     int kernel_variant = 0;      // 0 default kernels, 1..3 alternates (tuning / comparison)
     int use_graph = 0;           // th_chi_sweep replays one captured hipGraph per omega
+    int contract_3m = 0;         // contraction with three real products per complex one
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // [omega == 0, omega != 0]
     int graph_prec[2] = {-1, -1};
     double *d_omega = nullptr;   // omega of the current replay

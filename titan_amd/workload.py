@@ -5,6 +5,7 @@ timings (a fixed workload for profilers):
 """
 import argparse
 import json
+import time
 
 import numpy as np
 
@@ -22,6 +23,10 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
     ap.add_argument("--repeat", type=int, default=1)
     ap.add_argument("--green-variant", type=int, default=0)
+    ap.add_argument("--sweep", type=int, default=0, help="run th_chi_sweep over this many frequencies instead")
+    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs in the sweep")
+    ap.add_argument("--adjoint", action="store_true", help="also run th_chi_adjoint after each forward")
+    ap.add_argument("--contract-3m", type=int, default=0)
     a = ap.parse_args()
     spec = CF.system(a.config)
     rp = CF.run_params(a.config)
@@ -30,15 +35,32 @@ def main():
     dims = tuple(a.nk if n > 1 else 1 for n in rp["nk"])
     kp, wk = Ct.uniform_kmesh(m.b, *dims)
     h = m.handle()
-    h.tune(green_variant=a.green_variant)
+    h.tune(green_variant=a.green_variant, contract_3m=a.contract_3m)
     h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
     h.set_real_kmesh(kp, wk)
     x2, p2 = Ct.real_epoints(m.Ef, a.omega, rp["parts3"], rp["n3gl"])
     prec = B.PREC_MIXED if a.precision == "mixed" else B.PREC_FP64
+    if a.sweep:
+        h.tune(graph=1 if a.graph else 0)
+        oms = np.linspace(a.omega / a.sweep, a.omega, a.sweep)
+        for _ in range(a.repeat):
+            t0 = time.perf_counter()
+            h.sweep(rp["q"], oms, m.Ef, rp["eta"], rp["etap"], rp["parts3"], rp["n3gl"], m.Umat, precision=prec)
+            print(json.dumps({"config": a.config, "d": m.d, "sweep": a.sweep, "graph": a.graph,
+                              "pairs": len(wk) * (len(y) + len(x2)) * a.sweep, "wall_ms": 1e3 * (time.perf_counter() - t0),
+                              "timing": h.last_timing()}), flush=True)
+        h.close()
+        return
+    W = np.random.default_rng(1).standard_normal((m.D, m.D)) if a.adjoint else None
     for _ in range(a.repeat):
-        h.chi0(rp["q"], a.omega, m.Ef, rp["eta"], rp["etap"], x2, p2, precision=prec)
-        print(json.dumps({"config": a.config, "d": m.d, "pairs": len(wk) * (len(y) + len(x2)),
-                          "timing": h.last_timing()}), flush=True)
+        c0 = h.chi0(rp["q"], a.omega, m.Ef, rp["eta"], rp["etap"], x2, p2, precision=prec)
+        out = {"config": a.config, "d": m.d, "pairs": len(wk) * (len(y) + len(x2)), "timing": h.last_timing()}
+        if a.adjoint:
+            chi, _ = h.dyson(c0, m.Umat)
+            h.adjoint(rp["q"], a.omega, m.Ef, rp["eta"], rp["etap"], c0, W, U=m.Umat, chi=chi, x2=x2, p2=p2,
+                      precision=prec)
+            out["adjoint_timing"] = h.last_timing()
+        print(json.dumps(out), flush=True)
     h.close()
 
 
