@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default kernel shapes")
+    ap.add_argument("--tuning", default=None,
+                    help="JSON of th_tune settings to use instead of tuning (e.g. for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the chi(omega) path is built for a single MI355X "
@@ -146,7 +148,11 @@ def main():
     m = wl.model
     P = wl.pairs(om_time)
     tuning = None
-    if not args.no_tune:
+    if args.tuning:
+        fixed = json.loads(args.tuning)
+        wl.h.tune(**fixed)
+        tuning = {"best": fixed, "fixed": True}
+    elif not args.no_tune:
         # kernel shapes tuned on every 64th k-point (the autotuner of the library,
         # titan_amd/autotune.py), then the full work list is restored
         from titan_amd import autotune as AT

@@ -136,8 +136,10 @@ const char *th_last_error(void);
  * LDS-resident inversion, max supported nA. */
 int th_limits(int *max_nOrb, int *max_d, int *max_nA);
 
-/* Device time of the most recent th_chi0 / th_chi_sweep call split by stage
- * (milliseconds, measured with HIP events on the handle's stream):
+/* Device time of the most recent th_chi0 / th_chi_sweep / th_chi_adjoint call
+ * split by stage (milliseconds, HIP events on the stream each stage runs on;
+ * with "overlap" the inversion and contraction stages run concurrently, so
+ * their sum can exceed the whole):
  * t[0] green-function kernels, t[1] contraction GEMM, t[2] reduction +
  * finalize, t[3] Dyson, t[4] whole call.  Also the number of kernel
  * launches of each kind in n[0..3]. */
@@ -149,6 +151,9 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *   "green_variant"  inversion kernel shape: 0 = 2-D register-blocked (default
  *                    tiles), 1 = 1-D ownership, 2 = one wave per matrix,
  *                    3 = two waves per matrix (2 and 3 where compiled for d)
+ *   "overlap"        1 = th_chi0 pipelines the batches on two streams: the
+ *                    inversions of a batch run while the MFMA contraction of
+ *                    the previous one runs (default); 0 = one stream
  *   "contract_3m"    1 = the contraction forms each complex product from three
  *                    real MFMA products (Ar Br, Ai Bi, (Ar+Ai)(Br+Bi)) instead
  *                    of four; 0 = four (default)

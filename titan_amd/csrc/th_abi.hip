@@ -88,12 +88,17 @@ int ensure_panels(th_handle *h, long pairs)
 {
     const size_t need = (size_t)pairs * h->nA2 * 2 * h->Mpad * sizeof(double);
     if (need <= h->panel_bytes) return TH_OK;
+    TH_HIP(hipStreamSynchronize(h->stream2));
     hipFree(h->d_pA);
     hipFree(h->d_pB);
-    h->d_pA = h->d_pB = nullptr;
+    hipFree(h->d_pA2);
+    hipFree(h->d_pB2);
+    h->d_pA = h->d_pB = h->d_pA2 = h->d_pB2 = nullptr;
     h->panel_bytes = 0;
     TH_HIP(hipMalloc(&h->d_pA, need));
     TH_HIP(hipMalloc(&h->d_pB, need));
+    TH_HIP(hipMalloc(&h->d_pA2, need));
+    TH_HIP(hipMalloc(&h->d_pB2, need));
     h->panel_bytes = need;
     return TH_OK;
 }
@@ -206,56 +211,70 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
     int rc = th_launch_zero(h, h->d_acc, nacc);
     if (rc) return rc;
     const size_t pstride = (size_t)h->nA2 * 2 * h->Mpad * prec_size(prec);
-    auto begin = [&](int st) { return tm ? tm->begin(st) : -1; };
+    // The inversions of a batch (VALU) and the contraction of the previous
+    // batch (MFMA) are independent: with overlap on, the inversions go to
+    // stream2 into one of two panel slots, and the contraction of slot s waits
+    // for its inversions; the inversions into slot s wait for the contraction
+    // that last read it.  The fork and join events make the two streams one
+    // dependency graph, so the step can be captured as a hipGraph.
+    const bool ov = h->overlap != 0;
+    hipStream_t gs = ov ? h->stream2 : h->stream;
+    void *pA[2] = {h->d_pA, ov ? h->d_pA2 : h->d_pA}, *pB[2] = {h->d_pB, ov ? h->d_pB2 : h->d_pB};
+    if (ov) {
+        TH_HIP(hipEventRecord(h->ev_fork, h->stream));
+        TH_HIP(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+    }
+    long nb = 0;   // batches issued in this call
+    auto begin = [&](int st, hipStream_t s) { return tm ? tm->begin(st, s) : -1; };
     auto end = [&](int id) { if (tm) tm->end(id); };
-    // imaginary axis
-    const long bp = batch_pairs(h, h->n_im);
-    for (long p0 = 0; p0 < h->n_im; p0 += bp) {
-        const long np = std::min(bp, h->n_im - p0);
+    auto batch = [&](int mode_a, int mode_b, long p0, long np, int axis) -> int {
+        const int slot = ov ? (int)(nb & 1) : 0;
+        if (ov && nb >= 2) TH_HIP(hipStreamWaitEvent(h->stream2, h->ev_contract[slot], 0));
         th_green_args a{};
         a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
         a.omega = omega; a.omega_p = omega_p; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
-        a.mode = TH_G_IMAG_A;
-        a.panel = h->d_pA;
-        int id = begin(0);
-        if ((rc = th_launch_green(h, a))) return rc;
-        const void *pB = nullptr;
-        if (have_b) {
-            pB = (const char *)h->d_bcache + (size_t)p0 * pstride;
+        a.stream = gs;
+        a.mode = mode_a;
+        a.panel = pA[slot];
+        int id = begin(0, gs);
+        int r = th_launch_green(h, a);
+        if (r) return r;
+        const void *b = nullptr;
+        if (mode_b < 0) {
+            b = (const char *)h->d_bcache + (size_t)p0 * pstride;
         } else {
-            a.mode = TH_G_IMAG_B;
-            a.panel = h->d_pB;
-            if ((rc = th_launch_green(h, a))) return rc;
-            pB = h->d_pB;
+            a.mode = mode_b;
+            a.panel = pB[slot];
+            if ((r = th_launch_green(h, a))) return r;
+            b = pB[slot];
         }
         end(id);
-        id = begin(1);
-        if ((rc = th_launch_contract(h, h->d_pA, pB, np, prec, 0))) return rc;
+        if (ov) {
+            TH_HIP(hipEventRecord(h->ev_green[slot], h->stream2));
+            TH_HIP(hipStreamWaitEvent(h->stream, h->ev_green[slot], 0));
+        }
+        id = begin(1, h->stream);
+        if ((r = th_launch_contract(h, pA[slot], b, np, prec, axis))) return r;
         end(id);
-    }
+        if (ov) TH_HIP(hipEventRecord(h->ev_contract[slot], h->stream));
+        ++nb;
+        return TH_OK;
+    };
+    // imaginary axis
+    const long bp = batch_pairs(h, h->n_im);
+    for (long p0 = 0; p0 < h->n_im; p0 += bp)
+        if ((rc = batch(TH_G_IMAG_A, have_b ? -1 : TH_G_IMAG_B, p0, std::min(bp, h->n_im - p0), 0))) return rc;
     // real axis
     if (real_axis) {
         const long n_re = (long)n_x * h->nk_re;
         const long br = batch_pairs(h, n_re);
-        for (long p0 = 0; p0 < n_re; p0 += br) {
-            const long np = std::min(br, n_re - p0);
-            th_green_args a{};
-            a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
-            a.omega = omega; a.omega_p = omega_p; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
-            a.mode = TH_G_REAL_A;
-            a.panel = h->d_pA;
-            int id = begin(0);
-            if ((rc = th_launch_green(h, a))) return rc;
-            a.mode = TH_G_REAL_B;
-            a.panel = h->d_pB;
-            if ((rc = th_launch_green(h, a))) return rc;
-            end(id);
-            id = begin(1);
-            if ((rc = th_launch_contract(h, h->d_pA, h->d_pB, np, prec, 1))) return rc;
-            end(id);
-        }
+        for (long p0 = 0; p0 < n_re; p0 += br)
+            if ((rc = batch(TH_G_REAL_A, TH_G_REAL_B, p0, std::min(br, n_re - p0), 1))) return rc;
     }
-    int id = begin(2);
+    // join: every inversion of this step is done before the finalize (the
+    // contractions on 'stream' already waited for them)
+    if (ov && nb > 0) TH_HIP(hipStreamWaitEvent(h->stream, h->ev_green[(nb - 1) & 1], 0));
+    int id = begin(2, h->stream);
     rc = th_launch_finalize(h, h->d_chi0);
     end(id);
     return rc;
@@ -322,6 +341,12 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
         return TH_E_NODEVICE;
     }
     TH_HIP(hipStreamCreate(&h->stream));
+    TH_HIP(hipStreamCreate(&h->stream2));
+    TH_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    for (int s = 0; s < 2; ++s) {
+        TH_HIP(hipEventCreateWithFlags(&h->ev_green[s], hipEventDisableTiming));
+        TH_HIP(hipEventCreateWithFlags(&h->ev_contract[s], hipEventDisableTiming));
+    }
     h->nA = nA; h->nOrb = nOrb; h->bs = 2 * nOrb; h->d = d; h->D = 4 * nOrb * nOrb * nA; h->nhop = nhop;
     h->M = h->bs * h->bs; h->Mpad = th_mpad(h->M); h->nA2 = nA * nA;
     const int ntile = h->Mpad / TH_TILE;
@@ -611,6 +636,9 @@ int th_tune(th_handle *h, const char *key, long value)
             TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)value * h->nA2 * 2 * MM));
         }
         h->splits = (int)value;
+    } else if (k == "overlap") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: overlap is 0 or 1"); return TH_E_ARG; }
+        h->overlap = (int)value;
     } else if (k == "contract_3m") {
         if (value < 0 || value > 1) { th_set_error("th_tune: contract_3m is 0 or 1"); return TH_E_ARG; }
         h->contract_3m = (int)value;
@@ -641,7 +669,7 @@ int th_destroy(th_handle *h)
     if (!h) return TH_OK;
     hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
-                    h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
+                    h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
                     h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -653,6 +681,13 @@ int th_destroy(th_handle *h)
                 g_status.erase(g_status.begin() + i);
                 break;
             }
+    }
+    hipStreamSynchronize(h->stream2);
+    hipStreamDestroy(h->stream2);
+    hipEventDestroy(h->ev_fork);
+    for (int s = 0; s < 2; ++s) {
+        hipEventDestroy(h->ev_green[s]);
+        hipEventDestroy(h->ev_contract[s]);
     }
     hipStreamDestroy(h->stream);
     delete h;

@@ -845,6 +845,8 @@ __global__ __launch_bounds__(NT) void green_kernel(green_params g)
     write_panel<P, NT>(g, pl, M, idx, d, w);
 }
 
+inline hipStream_t gstream(th_handle *h, const th_green_args &a) { return a.stream ? a.stream : h->stream; }
+
 template <typename P, int NT>
 int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g);
 
@@ -853,7 +855,7 @@ int launch_green_reg(th_handle *h, const th_green_args &a, const green_params &g
 {
     const bool full = (DD * DD * 16) <= 49152;
     size_t lds = sizeof(double2) * ((full ? (size_t)DD * DD : (size_t)2 * h->bs * h->bs) + h->nhop);
-    hipLaunchKernelGGL((green_reg_kernel<P, DD, NT>), dim3((unsigned)a.np), dim3(NT), lds, h->stream, g);
+    hipLaunchKernelGGL((green_reg_kernel<P, DD, NT>), dim3((unsigned)a.np), dim3(NT), lds, gstream(h, a), g);
     TH_HIP(hipGetLastError());
     return TH_OK;
 }
@@ -863,7 +865,7 @@ int launch_green_2d(th_handle *h, const th_green_args &a, const green_params &g)
 {
     const bool full = full_stage(DD, TY * TX);
     size_t lds = sizeof(double2) * ((full ? (size_t)DD * DD : (size_t)2 * h->bs * h->bs) + h->nhop);
-    hipLaunchKernelGGL((green_2d_kernel<P, DD, TY, TX>), dim3((unsigned)a.np), dim3(TY * TX), lds, h->stream, g);
+    hipLaunchKernelGGL((green_2d_kernel<P, DD, TY, TX>), dim3((unsigned)a.np), dim3(TY * TX), lds, gstream(h, a), g);
     TH_HIP(hipGetLastError());
     return TH_OK;
 }
@@ -891,7 +893,7 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
     const int nt = (b + 31) / 32;
     auto gemm = [&](long np, double alpha, const double2 *A_, long sa, int lda, const double2 *B_, long sb, int ldb,
                     double beta, const double2 *C_, long sc, int ldc, double2 *O_, long so, int ldo) {
-        hipLaunchKernelGGL(zgemm_batched_kernel, dim3(nt * nt, (unsigned)np), dim3(256), 0, h->stream, b, b, b, alpha,
+        hipLaunchKernelGGL(zgemm_batched_kernel, dim3(nt * nt, (unsigned)np), dim3(256), 0, gstream(h, a), b, b, b, alpha,
                            A_, sa, lda, B_, sb, ldb, beta, C_, sc, ldc, O_, so, ldo, nt);
     };
     for (long c0 = 0; c0 < a.np; c0 += CHUNK) {
@@ -901,9 +903,9 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
         g.panel = (char *)a.panel + (size_t)c0 * pstride;
         g.raw = a.raw ? a.raw + (size_t)c0 * dd : nullptr;
         hipLaunchKernelGGL(assemble_m_kernel, dim3((unsigned)np), dim3(256), sizeof(double2) * std::max(1, h->nhop),
-                           h->stream, g, M);
+                           gstream(h, a), g, M);
         // Ai = A^-1
-        hipLaunchKernelGGL((inv_block_kernel<144, 16, 32>), dim3((unsigned)np), dim3(512), 0, h->stream,
+        hipLaunchKernelGGL((inv_block_kernel<144, 16, 32>), dim3((unsigned)np), dim3(512), 0, gstream(h, a),
                            (const double2 *)M, (long)dd, d, Ai, (long)bb, b, g.status);
         // X = Ai B, Y = C Ai
         gemm(np, 1.0, Ai, bb, b, M + b, dd, d, 0.0, nullptr, 0, 0, X, bb, b);
@@ -911,13 +913,13 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
         // S = E - C X
         gemm(np, -1.0, M + (size_t)b * d, dd, d, X, bb, b, 1.0, M + (size_t)b * d + b, dd, d, S, bb, b);
         // G_BB = S^-1
-        hipLaunchKernelGGL((inv_block_kernel<144, 16, 32>), dim3((unsigned)np), dim3(512), 0, h->stream,
+        hipLaunchKernelGGL((inv_block_kernel<144, 16, 32>), dim3((unsigned)np), dim3(512), 0, gstream(h, a),
                            (const double2 *)S, (long)bb, b, G + (size_t)b * d + b, (long)dd, d, g.status);
         // G_AB = -X G_BB, G_BA = -G_BB Y, G_AA = Ai - X G_BA
         gemm(np, -1.0, X, bb, b, G + (size_t)b * d + b, dd, d, 0.0, nullptr, 0, 0, G + b, dd, d);
         gemm(np, -1.0, G + (size_t)b * d + b, dd, d, Y, bb, b, 0.0, nullptr, 0, 0, G + (size_t)b * d, dd, d);
         gemm(np, -1.0, X, bb, b, G + (size_t)b * d, dd, d, 1.0, Ai, bb, b, G, dd, d);
-        hipLaunchKernelGGL((panel_from_global_kernel<P, 256>), dim3((unsigned)np), dim3(256), 0, h->stream, g,
+        hipLaunchKernelGGL((panel_from_global_kernel<P, 256>), dim3((unsigned)np), dim3(256), 0, gstream(h, a), g,
                            (const double2 *)G);
         TH_HIP(hipGetLastError());
     }
@@ -967,7 +969,7 @@ int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g)
 {
     size_t lds = sizeof(double2) * (size_t)h->d * h->d + sizeof(double2) * h->nhop +
                  2 * sizeof(double2) * h->d + 2 * sizeof(int) * h->d + 32;
-    hipLaunchKernelGGL((green_kernel<P, NT>), dim3((unsigned)a.np), dim3(NT), lds, h->stream, g);
+    hipLaunchKernelGGL((green_kernel<P, NT>), dim3((unsigned)a.np), dim3(NT), lds, gstream(h, a), g);
     TH_HIP(hipGetLastError());
     return TH_OK;
 }

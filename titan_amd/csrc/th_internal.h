@@ -25,6 +25,12 @@ inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
 struct th_handle {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the chi0 pipeline: the inversions of batch b + 1 run
+    // there while the MFMA contraction of batch b runs on 'stream'
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_green[2] = {nullptr, nullptr}, ev_contract[2] = {nullptr, nullptr};
+    int overlap = 1;             // pipeline the inversions and the contraction on two streams
+    void *d_pA2 = nullptr, *d_pB2 = nullptr;   // second panel slot of the pipeline
     int nA = 0, nOrb = 0, bs = 0, d = 0, D = 0, nhop = 0, M = 0, Mpad = 0, nA2 = 0;
     // model
     double *d_R = nullptr;       // 3*nhop
@@ -89,19 +95,22 @@ void th_set_error(const std::string &msg);
 // Stage timing with HIP events on the handle's stream
 struct stage_timer {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    std::vector<hipStream_t> st;
     th_handle *h;
     explicit stage_timer(th_handle *hh) : h(hh) {}
-    int begin(int stage)
+    int begin(int stage, hipStream_t s = nullptr)
     {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
-        hipEventRecord(a, h->stream);
+        if (!s) s = h->stream;
+        hipEventRecord(a, s);
         ev.push_back({stage, {a, b}});
+        st.push_back(s);
         return (int)ev.size() - 1;
     }
     void end(int id)
     {
-        if (id >= 0) hipEventRecord(ev[id].second.second, h->stream);
+        if (id >= 0) hipEventRecord(ev[id].second.second, st[id]);
     }
     void collect()
     {
@@ -146,6 +155,7 @@ struct th_green_args {
     void *panel;          // output panel, [np][nA2][2][Mpad] of double or float
     double2 *raw = nullptr;   // optional: G itself, [np][d][d] row-major FP64
     const double *omega_p = nullptr;   // optional: omega in device memory (graph replays)
+    hipStream_t stream = nullptr;      // optional: launch stream (default: the handle's)
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
 
