@@ -151,9 +151,18 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *   "green_variant"  inversion kernel shape: 0 = 2-D register-blocked (default
  *                    tiles), 1 = 1-D ownership, 2 = one wave per matrix,
  *                    3 = two waves per matrix (2 and 3 where compiled for d)
+ *   "spectral"       1 = where H(k) is Hermitian and d is 18, 36 or 54, each
+ *                    distinct k is diagonalised once (Jacobi, H = V
+ *                    diag(lambda) V^H; at the first frequency of every
+ *                    th_chi_sweep, th_chi0 keeps the eigenpairs of earlier
+ *                    calls at the same q) and G(k,z) = V diag(1/(z-lambda))
+ *                    V^H is formed on the matrix cores for every z (FP64
+ *                    MFMA; FP32 MFMA in TH_PREC_MIXED) (default); 0 = every
+ *                    G by Gauss-Jordan elimination of z-H
  *   "overlap"        1 = th_chi0 pipelines the batches on two streams: the
- *                    inversions of a batch run while the MFMA contraction of
- *                    the previous one runs (default); 0 = one stream
+ *                    eliminations of a batch run while the MFMA contraction
+ *                    of the previous one runs (default; it applies when the
+ *                    spectral path is not taken); 0 = one stream
  *   "contract_3m"    1 = the contraction forms each complex product from three
  *                    real MFMA products (Ar Br, Ai Bi, (Ar+Ai)(Br+Bi)) instead
  *                    of four; 0 = four (default)

@@ -331,6 +331,7 @@ def test_inversion_variants_are_bitwise_identical(which, variants):
     kp, wk = Ct.uniform_kmesh(m.b, *dims)
     K = np.tile(kp, (len(y), 1)); Y = np.repeat(y, len(wk)); W = np.outer(wy, wk).ravel()
     h = m.handle()
+    h.tune(spectral=0)
     h.set_imag_points(K, Y, W)
     h.set_real_kmesh(kp, wk)
     om = 2e-3
@@ -438,4 +439,38 @@ def test_three_product_contraction(which, cfg0, cfg2):
     gm = h.chi0([0, 0, 0], om, m.Ef, 5e-4, 5e-4, x2, p2, precision=B.PREC_MIXED)
     print(which, "3M rel fp64 %.2e mixed %.2e" % (rel(g, o), rel(gm, o)), flush=True)
     assert rel(gm, o) < 1e-6, rel(gm, o)
+    h.close()
+
+
+@pytest.mark.parametrize("which,q", [("cfg0", (0.0, 0.0, 0.0)), ("cfg0", (0.05, -0.02, 0.1)),
+                                     ("bcc2", (0.0, 0.0, 0.07)), ("cfg2", (0.0, 0.0, 0.0)),
+                                     ("cfg2", (0.03, 0.0, 0.0))])
+def test_spectral_green_matches_elimination_and_oracle(which, q):
+    """Spectral Green functions (one Jacobi eigen-decomposition of H(k) per
+    k-point, then G = V diag(1/(z - lambda)) V^H on the FP64 MFMA units)
+    against the Gauss-Jordan kernels and the oracle, with q = 0 (one
+    eigen-set) and q != 0 (a second set for k + q), both axes, eta down to
+    5e-4."""
+    spec = {"cfg0": lambda: CF.system(0), "bcc2": lambda: CF.bcc_supercell(2), "cfg2": lambda: CF.system(2)}[which]()
+    m, s = TBModel(spec), O.OracleSystem(spec)
+    nk = 4 if which == "cfg0" else 2
+    (K, Y, W), (kp, wk) = _work(m, nk, n1gl=8)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    om = 3e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 8)
+    for eta in (5e-4, 5e-3):
+        print(which, q, "eta", eta, flush=True)
+        h.tune(spectral=1)
+        gs = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2)
+        h.tune(spectral=0)
+        ge = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2)
+        o = s.chi0(list(q), om, eta, eta, (K, Y, W), (kp, wk), (x2, p2))
+        print("  spectral vs elimination %.2e, vs oracle %.2e" % (rel(gs, ge), rel(gs, o)), flush=True)
+        assert rel(gs, o) < 1e-10, rel(gs, o)
+        assert rel(gs, ge) < 1e-10
+        h.tune(spectral=1)
+        gm = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED)
+        assert rel(gm, o) < 1e-6, rel(gm, o)
     h.close()

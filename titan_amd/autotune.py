@@ -3,8 +3,12 @@
 Times th_chi0 on a sample of the caller's (k,E) work list for each candidate
 of the library's tuning knobs (th_tune):
 
-  green_variant  inversion kernel shape (0 default tiles, 2 one wave per
+  spectral       G(k,z) from one eigendecomposition per k (1) or by
+                 elimination of every z - H (0), where built for d
+  green_variant  elimination kernel shape (0 default tiles, 2 one wave per
                  matrix, 3 two waves per matrix, where built for d)
+  overlap        eliminations of one batch concurrent with the contraction
+                 of the previous one on a second stream
   contract_3m    three instead of four real MFMA products per complex one
   splits         K-splits of the contraction (workgroups per output tile)
   batch_mb       panel memory per batch (pairs per launch)
@@ -13,8 +17,9 @@ and leaves the fastest setting on the handle.  The device time of each call
 comes from th_last_timing (HIP events on the library's stream).  Tuning is
 per handle: the best shapes depend on d = 2 nOrb nA, the number of atom
 pairs and the batch sizes.  Results are bitwise identical for every
-green_variant (same operations per matrix entry); splits and batch_mb change
-the order of the FP64 sums of the contraction only, contract_3m its rounding.
+green_variant and overlap setting (same operations per matrix entry); splits
+and batch_mb change the order of the FP64 sums of the contraction only,
+contract_3m and spectral their rounding.
 """
 import time
 
@@ -22,17 +27,24 @@ import numpy as np
 
 from titan_amd import binding as B
 
-# inversion kernel shapes built per d (th_green.hip)
+# elimination kernel shapes built per d, and the d with a spectral kernel
+# (th_green.hip)
 VARIANTS = {36: (0, 2), 54: (0, 2, 3), 72: (0, 3)}
+SPECTRAL_D = (18, 36, 54)
 
 
-def _knobs(h, base):
-    """Candidate values of each knob, tuned one knob at a time."""
+def _knobs(h, base, setting):
+    """Candidate values of the next knob given the setting so far (knobs are
+    tuned one at a time; the elimination knobs only matter without the
+    spectral path)."""
     s0 = base["splits"]
-    return [("green_variant", list(VARIANTS.get(h.d, (0,)))),
-            ("contract_3m", [0, 1]),
-            ("splits", sorted({max(1, s0 // 2), s0, 2 * s0})),
-            ("batch_mb", [256, 512, 1024])]
+    yield "spectral", [1, 0] if h.d in SPECTRAL_D else [0]
+    if not setting["spectral"]:
+        yield "green_variant", list(VARIANTS.get(h.d, (0,)))
+        yield "overlap", [1, 0]
+    yield "contract_3m", [0, 1]
+    yield "splits", sorted({max(1, s0 // 2), s0, 2 * s0})
+    yield "batch_mb", [256, 512, 1024]
 
 
 def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, repeat=2, log=None):
@@ -43,8 +55,8 @@ def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, r
     "seconds"}."""
     t0 = time.perf_counter()
     base = h.tuning()
-    setting = {"green_variant": base["green_variant"], "contract_3m": 0, "splits": base["splits"],
-               "batch_mb": base["batch_mb"]}
+    setting = {"spectral": 1 if h.d in SPECTRAL_D else 0, "green_variant": base["green_variant"],
+               "overlap": 1, "contract_3m": 0, "splits": base["splits"], "batch_mb": base["batch_mb"]}
     table = []
 
     def timed(st):
@@ -59,14 +71,15 @@ def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, r
         return best
 
     best_ms = timed(setting)
-    for knob, values in _knobs(h, base):
+    for knob, values in _knobs(h, base, setting):
         for v in values:
             if v == setting[knob]:
                 continue
             trial = dict(setting, **{knob: v})
             ms = timed(trial)
-            if ms < best_ms:
-                best_ms, setting = ms, trial
+            if ms < best_ms:          # in place: _knobs reads the live setting
+                best_ms = ms
+                setting.update(trial)
     h.tune(**setting)
     return {"best": setting, "best_ms": best_ms, "default": base, "table": table,
             "seconds": time.perf_counter() - t0}

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,29 @@ void gauleg(double x1, double x2, int n, double *x, double *w)
 struct status_block {
     int *d = nullptr;
 };
+
+// exact k-point identity (bit patterns of the three coordinates)
+struct kkey {
+    uint64_t a, b, c;
+    bool operator==(const kkey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct kkey_hash {
+    size_t operator()(const kkey &k) const
+    {
+        uint64_t h = k.a * 0x9E3779B97F4A7C15ull;
+        h ^= k.b + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        h ^= k.c + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+kkey make_key(const double *k)
+{
+    kkey r;
+    std::memcpy(&r.a, k, 8);
+    std::memcpy(&r.b, k + 1, 8);
+    std::memcpy(&r.c, k + 2, 8);
+    return r;
+}
 
 }  // namespace
 
@@ -124,6 +148,77 @@ int th_upload_nodes(th_handle *h, int n_x, const double *x2, const double *p2)
     return TH_OK;
 }
 
+// Unique k-points of the imaginary list and the real mesh (the eigenpairs
+// of H are computed once per unique k) and the maps to them.
+static int rebuild_kunion(th_handle *h)
+{
+    h->eig_valid[0] = h->eig_valid[1] = 0;
+    std::unordered_map<kkey, int, kkey_hash> ids;
+    std::vector<double> ku;
+    auto add = [&](const double *k) {
+        auto it = ids.find(make_key(k));
+        if (it != ids.end()) return it->second;
+        const int id = (int)(ku.size() / 3);
+        ids.emplace(make_key(k), id);
+        ku.insert(ku.end(), k, k + 3);
+        return id;
+    };
+    std::vector<int> map_im(h->kim_uniq.size() / 3), kid_re(h->kre_host.size() / 3);
+    for (size_t u = 0; u < map_im.size(); ++u) map_im[u] = add(&h->kim_uniq[3 * u]);
+    for (size_t u = 0; u < kid_re.size(); ++u) kid_re[u] = add(&h->kre_host[3 * u]);
+    std::vector<int> kid_im(h->kid_im_local.size());
+    for (size_t p = 0; p < kid_im.size(); ++p) kid_im[p] = map_im[h->kid_im_local[p]];
+    hipFree(h->d_kuniq); hipFree(h->d_kid_im); hipFree(h->d_kid_re);
+    h->d_kuniq = nullptr; h->d_kid_im = h->d_kid_re = nullptr;
+    h->n_kuniq = (long)(ku.size() / 3);
+    if (h->n_kuniq > 0) {
+        TH_HIP(hipMalloc(&h->d_kuniq, sizeof(double) * ku.size()));
+        TH_HIP(hipMemcpy(h->d_kuniq, ku.data(), sizeof(double) * ku.size(), hipMemcpyHostToDevice));
+    }
+    if (!kid_im.empty()) {
+        TH_HIP(hipMalloc(&h->d_kid_im, sizeof(int) * kid_im.size()));
+        TH_HIP(hipMemcpy(h->d_kid_im, kid_im.data(), sizeof(int) * kid_im.size(), hipMemcpyHostToDevice));
+    }
+    if (!kid_re.empty()) {
+        TH_HIP(hipMalloc(&h->d_kid_re, sizeof(int) * kid_re.size()));
+        TH_HIP(hipMemcpy(h->d_kid_re, kid_re.data(), sizeof(int) * kid_re.size(), hipMemcpyHostToDevice));
+    }
+    return TH_OK;
+}
+
+// Eigenpairs for the sides of th_chi0 at this q: set 0 (q = 0) always, set
+// 1 for k + q when q != 0.  Skipped (the elimination kernels are used) when
+// H is not Hermitian, d has no spectral kernel, or the sets would take more
+// than 30 % of free HBM.
+int th_ensure_eig(th_handle *h, const double *q)
+{
+    if (!h->spectral || !h->hermitian || (h->d != 18 && h->d != 36 && h->d != 54) || h->n_kuniq <= 0) return TH_OK;
+    const bool qz = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
+    const size_t bytes = (size_t)h->n_kuniq * h->d * (h->d * sizeof(double2) + sizeof(double));
+    for (int which = 0; which < (qz ? 1 : 2); ++which) {
+        th_eig_set &es = h->eig[which];
+        const double qq[3] = {which ? q[0] : 0.0, which ? q[1] : 0.0, which ? q[2] : 0.0};
+        if (h->eig_valid[which] && es.q[0] == qq[0] && es.q[1] == qq[1] && es.q[2] == qq[2]) continue;
+        h->eig_valid[which] = 0;
+        if (!es.V) {
+            size_t freeb = 0, totb = 0;
+            if (hipMemGetInfo(&freeb, &totb) != hipSuccess || bytes > (size_t)(0.3 * (double)freeb)) return TH_OK;
+            if (hipMalloc(&es.V, (size_t)h->n_kuniq * h->d * h->d * sizeof(double2)) != hipSuccess ||
+                hipMalloc(&es.L, (size_t)h->n_kuniq * h->d * sizeof(double)) != hipSuccess) {
+                hipFree(es.V); hipFree(es.L);
+                es.V = nullptr; es.L = nullptr;
+                (void)hipGetLastError();
+                return TH_OK;
+            }
+        }
+        int rc = th_launch_eig(h, which, qq);
+        if (rc) return rc;
+        es.q[0] = qq[0]; es.q[1] = qq[1]; es.q[2] = qq[2];
+        h->eig_valid[which] = 1;
+    }
+    return TH_OK;
+}
+
 namespace {
 
 long batch_pairs(th_handle *h, long total)
@@ -181,7 +276,7 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
 // Host side of one omega of chi0: argument checks, real-axis nodes to the
 // device, the omega-independent B cache, panel buffers.  Issues no kernel of
 // the contraction itself.
-int chi0_prepare(th_handle *h, double omega, double Ef, double etap, int n_x, const double *x2,
+int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double etap, int n_x, const double *x2,
                  const double *p2, int prec, stage_timer &tm, bool *real_axis, bool *have_b)
 {
     if (!h->d_kim || h->n_im <= 0) {
@@ -195,6 +290,12 @@ int chi0_prepare(th_handle *h, double omega, double Ef, double etap, int n_x, co
     }
     int rc;
     if (*real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
+    {
+        const int id = tm.begin(0);
+        rc = th_ensure_eig(h, q);
+        tm.end(id);
+        if (rc) return rc;
+    }
     if ((rc = ensure_bcache(h, Ef, etap, prec, tm, have_b))) return rc;
     long need = batch_pairs(h, h->n_im);
     if (*real_axis) need = std::max(need, batch_pairs(h, (long)n_x * h->nk_re));
@@ -217,7 +318,10 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
     // for its inversions; the inversions into slot s wait for the contraction
     // that last read it.  The fork and join events make the two streams one
     // dependency graph, so the step can be captured as a hipGraph.
-    const bool ov = h->overlap != 0;
+    // (the spectral Green functions run on the matrix cores too, so the
+    // overlap only pays with the elimination kernels)
+    const bool spectral_on = h->spectral && h->eig_valid[0];
+    const bool ov = h->overlap != 0 && !spectral_on;
     hipStream_t gs = ov ? h->stream2 : h->stream;
     void *pA[2] = {h->d_pA, ov ? h->d_pA2 : h->d_pA}, *pB[2] = {h->d_pB, ov ? h->d_pB2 : h->d_pB};
     if (ov) {
@@ -285,7 +389,7 @@ int chi0_device(th_handle *h, const double *q, double omega, double Ef, double e
                 const double *x2, const double *p2, int prec, stage_timer &tm)
 {
     bool real_axis = false, have_b = false;
-    int rc = chi0_prepare(h, omega, Ef, etap, n_x, x2, p2, prec, tm, &real_axis, &have_b);
+    int rc = chi0_prepare(h, q, omega, Ef, etap, n_x, x2, p2, prec, tm, &real_axis, &have_b);
     if (rc) return rc;
     return chi0_launch(h, q, omega, nullptr, Ef, eta, etap, n_x, real_axis, have_b, prec, &tm);
 }
@@ -389,6 +493,40 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
         TH_HIP(hipMemcpy(h->d_ib, ibs.data(), sizeof(int) * nhop, hipMemcpyHostToDevice));
         TH_HIP(hipMemcpy(h->d_t, ts.data(), sizeof(double2) * (size_t)nhop * nOrb * nOrb, hipMemcpyHostToDevice));
     }
+    // H(k) is Hermitian for every k when h0 is and the hops come in pairs
+    // (R, jb, ib, t) and (-R, ib, jb, t^H)
+    {
+        const double2 *H0 = reinterpret_cast<const double2 *>(h0);
+        double scale = 0.0;
+        for (size_t e = 0; e < (size_t)d * d; ++e) scale = std::max(scale, std::fabs(H0[e].x) + std::fabs(H0[e].y));
+        const double tol = 1e-12 * std::max(scale, 1.0);
+        bool herm = true;
+        for (int c = 0; c < d && herm; ++c)
+            for (int r = 0; r < d; ++r) {
+                const double2 a = H0[r + (size_t)d * c], b = H0[c + (size_t)d * r];
+                if (std::fabs(a.x - b.x) > tol || std::fabs(a.y + b.y) > tol) { herm = false; break; }
+            }
+        const size_t nO2 = (size_t)nOrb * nOrb;
+        const double2 *T = reinterpret_cast<const double2 *>(t);
+        for (int n = 0; n < nhop && herm; ++n) {
+            bool found = false;
+            for (int m = 0; m < nhop && !found; ++m) {
+                if (jb[m] != ib[n] || ib[m] != jb[n]) continue;
+                if (std::fabs(R[3 * m] + R[3 * n]) > 1e-9 || std::fabs(R[3 * m + 1] + R[3 * n + 1]) > 1e-9 ||
+                    std::fabs(R[3 * m + 2] + R[3 * n + 2]) > 1e-9)
+                    continue;
+                bool same = true;
+                for (int nu = 0; nu < nOrb && same; ++nu)
+                    for (int mu = 0; mu < nOrb; ++mu) {
+                        const double2 x = T[n * nO2 + nu + (size_t)nOrb * mu], y = T[m * nO2 + mu + (size_t)nOrb * nu];
+                        if (std::fabs(x.x - y.x) > tol || std::fabs(x.y + y.y) > tol) { same = false; break; }
+                    }
+                found = same;
+            }
+            herm = found;
+        }
+        h->hermitian = herm ? 1 : 0;
+    }
     TH_HIP(hipMalloc(&h->d_h0, sizeof(double2) * (size_t)d * d));
     TH_HIP(hipMemcpy(h->d_h0, h0, sizeof(double2) * (size_t)d * d, hipMemcpyHostToDevice));
     const size_t MM = (size_t)h->Mpad * h->Mpad;
@@ -414,7 +552,23 @@ int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, c
     TH_HIP(hipMemcpy(h->d_yim, y, sizeof(double) * n, hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(h->d_wim, w, sizeof(double) * n, hipMemcpyHostToDevice));
     h->n_im = n;
-    return TH_OK;
+    // unique k-points of the list
+    std::unordered_map<kkey, int, kkey_hash> ids;
+    h->kim_uniq.clear();
+    h->kid_im_local.assign(n, 0);
+    for (long p = 0; p < n; ++p) {
+        auto key = make_key(k + 3 * p);
+        auto it = ids.find(key);
+        if (it == ids.end()) {
+            const int id = (int)(h->kim_uniq.size() / 3);
+            ids.emplace(key, id);
+            h->kim_uniq.insert(h->kim_uniq.end(), k + 3 * p, k + 3 * p + 3);
+            h->kid_im_local[p] = id;
+        } else {
+            h->kid_im_local[p] = it->second;
+        }
+    }
+    return rebuild_kunion(h);
 }
 
 int th_set_real_kmesh(th_handle *h, long nk, const double *k, const double *w)
@@ -427,7 +581,8 @@ int th_set_real_kmesh(th_handle *h, long nk, const double *k, const double *w)
     TH_HIP(hipMemcpy(h->d_kre, k, sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(h->d_wre, w, sizeof(double) * nk, hipMemcpyHostToDevice));
     h->nk_re = nk;
-    return TH_OK;
+    h->kre_host.assign(k, k + 3 * nk);
+    return rebuild_kunion(h);
 }
 
 int th_chi0(th_handle *h, const double *q, double omega, double Ef, double eta, double etap, int n_x,
@@ -528,6 +683,8 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
     int idall = tm.begin(4);
     for (auto &c : h->n_launch) c = 0;
     int r = 0;
+    // a sweep diagonalises H(k) afresh (once, at its first frequency)
+    h->eig_valid[0] = h->eig_valid[1] = 0;
     int rc = th_reset_status(h);
     if (!rc && h->use_graph) rc = th_dyson_prepare(h, reinterpret_cast<const double2 *>(U), &r);
     for (int w = 0; w < n_omega && !rc; ++w) {
@@ -541,7 +698,7 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
         double cond = 0.0;
         if (h->use_graph) {
             bool real_axis = false, have_b = false;
-            rc = chi0_prepare(h, e, Ef, etap, n_x, x2.data(), p2.data(), precision, tm, &real_axis, &have_b);
+            rc = chi0_prepare(h, q, e, Ef, etap, n_x, x2.data(), p2.data(), precision, tm, &real_axis, &have_b);
             if (rc) break;
             if (hipMemcpyAsync(h->d_omega, &om[w], sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
                 th_set_error("th_chi_sweep: omega upload failed");
@@ -636,6 +793,9 @@ int th_tune(th_handle *h, const char *key, long value)
             TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)value * h->nA2 * 2 * MM));
         }
         h->splits = (int)value;
+    } else if (k == "spectral") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: spectral is 0 or 1"); return TH_E_ARG; }
+        h->spectral = (int)value;
     } else if (k == "overlap") {
         if (value < 0 || value > 1) { th_set_error("th_tune: overlap is 0 or 1"); return TH_E_ARG; }
         h->overlap = (int)value;
@@ -670,7 +830,8 @@ int th_destroy(th_handle *h)
     hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
-                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk};
+                    h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk, h->d_kuniq, h->d_kid_im, h->d_kid_re,
+                    h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L};
     for (void *b : bufs)
         if (b) hipFree(b);
     {

@@ -827,6 +827,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
         TH_HIP(hipMemcpyAsync(d_groups, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice, h->stream));
     if ((rc = th_ensure_panels(h, bp))) return rc;
     if (real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
+    if ((rc = th_ensure_eig(h, q))) return rc;
     TH_HIP(hipMemcpyAsync(d_chat, chi_hat, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));
     TH_HIP(hipMemcpyAsync(d_x0, chi0, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));
     if (U) TH_HIP(hipMemcpyAsync(d_x, chi, sizeof(double2) * DD, hipMemcpyHostToDevice, h->stream));

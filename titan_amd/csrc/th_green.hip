@@ -45,6 +45,11 @@ struct green_params {
     void *panel;
     double2 *raw;
     int *status;
+    // spectral mode: eigenpairs of H(k) per k-point of the current sets and
+    // the k-point index of every pair (imaginary list, real mesh)
+    const double2 *eigV;
+    const double *eigL;
+    const int *kid_im, *kid_re;
 };
 
 // In-place Gauss-Jordan inverse of the d x d row-major matrix M in LDS.
@@ -774,6 +779,245 @@ __global__ __launch_bounds__(NT) void panel_from_global_kernel(green_params g, c
     write_panel<P, NT>(g, pl, reinterpret_cast<const cplx<double> *>(Gp), ident, d, w);
 }
 
+// ---------------------------------------------------------------------------
+// Spectral Green functions.  H(k) does not depend on z, so one eigen-
+// decomposition H(k) = V diag(lambda) V^H per k-point serves every energy and
+// frequency of a sweep, and each G(k,z) = V diag(1/(z - lambda)) V^H is one
+// complex product on the MFMA units instead of an elimination.  Valid for
+// Hermitian H (checked when the model is uploaded).
+
+// Eigen-decomposition of H(k) for a list of k-points by cyclic two-sided
+// Jacobi in parallel (round-robin) order, one workgroup per k-point, A and V
+// in LDS.  Rotation of the pair (p,q): with apq = |apq| e^{i phi} the
+// 2 x 2 block is made real by diag(1, e^{-i phi}) and diagonalised by the
+// symmetric Schur rotation (c, s) of tau = (aqq - app) / (2 |apq|); J =
+// [[c, s], [-s e^{-i phi}, c e^{-i phi}]], A <- J^H A J, V <- V J.
+// Sweeps until the off-diagonal Frobenius norm is below 1e-15 of the
+// whole (at most 30 sweeps).  Output V row-major (V[r][m] = component r of
+// eigenvector m) and lambda.
+template <int DD>
+__global__ __launch_bounds__(256) void eig_jacobi_kernel(green_params g, const double *klist, double q0, double q1,
+                                                         double q2, double2 *Vout, double *Lout)
+{
+    static_assert(DD % 2 == 0 && DD <= 64, "even DD <= 64");
+    constexpr int NP = DD / 2, NR = DD - 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double> *ph = reinterpret_cast<cplx<double> *>(smem);
+    __shared__ cplx<double> A[DD * DD], V[DD * DD];
+    __shared__ double rc[NP], rs[NP];
+    __shared__ cplx<double> re[NP];
+    __shared__ int rp[NP], rq[NP];
+    __shared__ double red[256];
+    __shared__ int s_more;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const long kb = blockIdx.x;
+    const double k0 = klist[3 * kb] + q0, k1 = klist[3 * kb + 1] + q1, k2 = klist[3 * kb + 2] + q2;
+    for (int n = tid; n < g.nhop; n += NT) {
+        double a = k0 * g.R[3 * n] + k1 * g.R[3 * n + 1] + k2 * g.R[3 * n + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[n] = {cs, sn};
+    }
+    __syncthreads();
+    const int nO = g.nOrb, nO2 = nO * nO;
+    for (int e = tid; e < DD * DD; e += NT) {
+        const int r = e / DD, c = e - (e / DD) * DD;
+        double2 hv = g.h0[r + (size_t)c * DD];
+        cplx<double> x = {hv.x, hv.y};
+        const int ja = r / (2 * nO), rsp = (r / nO) & 1, nu = r - (r / nO) * nO;
+        const int ia = c / (2 * nO), csp = (c / nO) & 1, mu = c - (c / nO) * nO;
+        if (rsp == csp) {
+            const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
+            for (int n = h1; n < h2; ++n) {
+                double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                const cplx<double> tp = cmul(cplx<double>{tv.x, tv.y}, ph[n]);
+                x.x += tp.x;
+                x.y += tp.y;
+            }
+        }
+        A[e] = x;
+        V[e] = {r == c ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        for (int rnd = 0; rnd < NR; ++rnd) {
+            if (tid < NP) {
+                int p, q;
+                if (tid == 0) { p = NR; q = rnd; }
+                else { p = (rnd + tid) % NR; q = (rnd - tid + NR) % NR; }
+                const cplx<double> apq = A[p * DD + q];
+                const double app = A[p * DD + p].x, aqq = A[q * DD + q].x;
+                const double mag = sqrt(apq.x * apq.x + apq.y * apq.y);
+                double c = 1.0, sv = 0.0;
+                cplx<double> e = {1.0, 0.0};
+                if (mag > 0.0) {
+                    const double tau = (aqq - app) / (2.0 * mag);
+                    const double at = fabs(tau);
+                    const double t = (at > 1e150 ? 0.5 / at : 1.0 / (at + sqrt(1.0 + tau * tau))) * (tau >= 0.0 ? 1.0 : -1.0);
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    sv = t * c;
+                    e = {apq.x / mag, apq.y / mag};
+                }
+                rp[tid] = p; rq[tid] = q; rc[tid] = c; rs[tid] = sv; re[tid] = e;
+            }
+            __syncthreads();
+            // rows: A[p,:] = c A[p,:] - s e A[q,:],  A[q,:] = s A[p,:] + c e A[q,:]
+            for (int it = tid; it < NP * DD; it += NT) {
+                const int j = it / DD, col = it - (it / DD) * DD;
+                const int p = rp[j], q = rq[j];
+                const double c = rc[j], sv = rs[j];
+                const cplx<double> e = re[j];
+                const cplx<double> x = A[p * DD + col], y = A[q * DD + col];
+                const cplx<double> ey = cmul(e, y);
+                A[p * DD + col] = {c * x.x - sv * ey.x, c * x.y - sv * ey.y};
+                A[q * DD + col] = {sv * x.x + c * ey.x, sv * x.y + c * ey.y};
+            }
+            __syncthreads();
+            // columns of A and V: M[:,p] = c M[:,p] - s conj(e) M[:,q],  M[:,q] = s M[:,p] + c conj(e) M[:,q]
+            for (int it = tid; it < 2 * NP * DD; it += NT) {
+                const int which = it / (NP * DD), rem = it - which * NP * DD;
+                const int j = rem / DD, row = rem - (rem / DD) * DD;
+                cplx<double> *M = which ? V : A;
+                const int p = rp[j], q = rq[j];
+                const double c = rc[j], sv = rs[j];
+                const cplx<double> ec = {re[j].x, -re[j].y};
+                const cplx<double> x = M[row * DD + p], y = M[row * DD + q];
+                const cplx<double> ey = cmul(ec, y);
+                M[row * DD + p] = {c * x.x - sv * ey.x, c * x.y - sv * ey.y};
+                M[row * DD + q] = {sv * x.x + c * ey.x, sv * x.y + c * ey.y};
+            }
+            __syncthreads();
+        }
+        // convergence: off-diagonal against total Frobenius norm
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < DD * DD; e += NT) {
+            const cplx<double> x = A[e];
+            const double m2 = x.x * x.x + x.y * x.y;
+            tot += m2;
+            if (e / DD != e - (e / DD) * DD) off += m2;
+        }
+        red[tid] = off;
+        __syncthreads();
+        for (int o = NT / 2; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        const double offs = red[0];
+        __syncthreads();
+        red[tid] = tot;
+        __syncthreads();
+        for (int o = NT / 2; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) s_more = offs > 1e-30 * red[0];
+        __syncthreads();
+        if (!s_more) break;
+    }
+    double2 *vo = Vout + (size_t)kb * DD * DD;
+    for (int e = tid; e < DD * DD; e += NT) vo[e] = make_double2(V[e].x, V[e].y);
+    for (int m = tid; m < DD; m += NT) Lout[(size_t)kb * DD + m] = A[m * DD + m].x;
+}
+
+typedef double sdv4 __attribute__((ext_vector_type(4)));
+
+// G(k,z) = V diag(1/(z - lambda)) V^H for one pair per workgroup on the
+// FP64 MFMA units (v_mfma_f64_16x16x4: lane l holds A[l & 15][l >> 4],
+// B[l >> 4][l & 15], result rows (l >> 4) + 4q, column l & 15).  Wave w owns
+// the output rows 16w .. 16w+15; V of the pair's k-point is staged in LDS
+// and then overwritten by G, which is scattered into the panel as by the
+// elimination kernels.
+// MFMA flavour of the spectral product: FP64 (v_mfma_f64_16x16x4, result
+// rows (l >> 4) + 4q) or, in the mixed mode, FP32 (v_mfma_f32_16x16x4,
+// result rows 4 (l >> 4) + q) with V and 1/(z - lambda) rounded to FP32.
+template <typename R> struct smfma;
+template <> struct smfma<double> {
+    typedef sdv4 acc;
+    static __device__ inline acc mac(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int g, int q) { return g + 4 * q; }
+};
+typedef float sfv4 __attribute__((ext_vector_type(4)));
+template <> struct smfma<float> {
+    typedef sfv4 acc;
+    static __device__ inline acc mac(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int g, int q) { return 4 * g + q; }
+};
+
+template <typename P, int DD>
+__global__ __launch_bounds__(256) void green_spectral_kernel(green_params g)
+{
+    typedef P R;   // arithmetic of the product: FP64 panels -> FP64, FP32 panels -> FP32
+    constexpr int NB = (DD + 15) / 16;        // 16-wide blocks per dimension (waves used)
+    constexpr int KS = (DD + 3) / 4;          // k-steps of 4
+    __shared__ cplx<double> S[DD * DD];
+    __shared__ cplx<double> f[DD];
+    __shared__ int ident[DD];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const long pl = blockIdx.x;
+    const long p = g.p0 + pl;
+    double k0, k1, k2, er, ei, w;
+    pair_point(g, p, k0, k1, k2, er, ei, w);
+    const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
+    const long kb = imag ? g.kid_im[p] : g.kid_re[p % g.nk_re];
+    const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
+    for (int e = tid; e < DD * DD; e += 256) {
+        const double2 v = Vk[e];
+        S[e] = {v.x, v.y};
+    }
+    for (int m = tid; m < DD; m += 256) {
+        const double a = er - g.eigL[(size_t)kb * DD + m];
+        const double n = a * a + ei * ei;
+        f[m] = {a / n, -ei / n};
+        ident[m] = m;
+    }
+    __syncthreads();
+    typename smfma<R>::acc accr[NB], acci[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        for (int q = 0; q < 4; ++q) { accr[j][q] = 0; acci[j][q] = 0; }
+    if (wave < NB) {
+        const int row = 16 * wave + (lane & 15);
+#pragma unroll 2
+        for (int ks = 0; ks < KS; ++ks) {
+            const int m = 4 * ks + (lane >> 4);
+            cplx<double> a = {0.0, 0.0};
+            if (row < DD && m < DD) a = cmul(S[row * DD + m], f[m]);
+            const R ar = (R)a.x, ai = (R)a.y, nai = -ai;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int col = 16 * j + (lane & 15);
+                R br = 0, bi = 0;
+                if (col < DD && m < DD) {
+                    const cplx<double> v = S[col * DD + m];
+                    br = (R)v.x;                           // (V^H)[m][col]
+                    bi = (R)(-v.y);
+                }
+                accr[j] = smfma<R>::mac(ar, br, accr[j]);
+                accr[j] = smfma<R>::mac(nai, bi, accr[j]);
+                acci[j] = smfma<R>::mac(ar, bi, acci[j]);
+                acci[j] = smfma<R>::mac(ai, br, acci[j]);
+            }
+        }
+    }
+    __syncthreads();   // every wave has read V before it is overwritten by G
+    if (wave < NB) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = 16 * wave + smfma<R>::row(lane >> 4, q), c = 16 * j + (lane & 15);
+                if (r < DD && c < DD) S[r * DD + c] = {(double)accr[j][q], (double)acci[j][q]};
+            }
+    }
+    __syncthreads();
+    if (g.raw) {
+        double2 *o = g.raw + (size_t)pl * DD * DD;
+        for (int e = tid; e < DD * DD; e += 256) o[e] = make_double2(S[e].x, S[e].y);
+    }
+    write_panel<P, 256>(g, pl, S, ident, DD, w);
+}
+
+
 template <typename P, int NT>
 __global__ __launch_bounds__(NT) void green_kernel(green_params g)
 {
@@ -926,6 +1170,14 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
     return TH_OK;
 }
 
+template <typename P, int DD>
+int launch_green_spectral(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    hipLaunchKernelGGL((green_spectral_kernel<P, DD>), dim3((unsigned)a.np), dim3(256), 0, gstream(h, a), g);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
@@ -947,6 +1199,14 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
         switch (h->d) {
         case 54: return launch_green_2d<P, 54, 8, 16>(h, a, g);
         case 72: return launch_green_2d<P, 72, 8, 16>(h, a, g);
+        default: break;
+        }
+    }
+    if (g.eigV) {
+        switch (h->d) {
+        case 18: return launch_green_spectral<P, 18>(h, a, g);
+        case 36: return launch_green_spectral<P, 36>(h, a, g);
+        case 54: return launch_green_spectral<P, 54>(h, a, g);
         default: break;
         }
     }
@@ -997,7 +1257,49 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.panel = a.panel;
     g.raw = a.raw;
     g.status = th_status_word(h);
+    // spectral Green functions when the eigenpairs of this side's k-points are ready
+    const bool qside = a.mode == TH_G_IMAG_A || a.mode == TH_G_REAL_A;
+    const bool qzero = a.q[0] == 0.0 && a.q[1] == 0.0 && a.q[2] == 0.0;
+    const th_eig_set *es = nullptr;
+    if (h->spectral && h->eig_valid[0] && (!qside || qzero))
+        es = &h->eig[0];
+    else if (h->spectral && qside && h->eig_valid[1] && h->eig[1].q[0] == a.q[0] && h->eig[1].q[1] == a.q[1] &&
+             h->eig[1].q[2] == a.q[2])
+        es = &h->eig[1];
+    g.eigV = es ? es->V : nullptr;
+    g.eigL = es ? es->L : nullptr;
+    g.kid_im = h->d_kid_im;
+    g.kid_re = h->d_kid_re;
     h->n_launch[0] += 1;
     if (a.prec == TH_PREC_FP64) return launch_green_p<double>(h, a, g);
     return launch_green_p<float>(h, a, g);
+}
+
+// Eigenpairs of H(k + q) for the handle's unique k-points into set 'which'
+int th_launch_eig(th_handle *h, int which, const double *q)
+{
+    green_params g{};
+    g.nA = h->nA; g.nOrb = h->nOrb; g.bs = h->bs; g.d = h->d; g.nhop = h->nhop;
+    g.R = h->d_R; g.jb = h->d_jb; g.ib = h->d_ib; g.t = h->d_t; g.h0 = h->d_h0; g.hoff = h->d_hoff;
+    th_eig_set &es = h->eig[which];
+    const size_t lds = sizeof(double2) * std::max(1, h->nhop);
+    switch (h->d) {
+    case 18:
+        hipLaunchKernelGGL((eig_jacobi_kernel<18>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
+                           (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
+        break;
+    case 36:
+        hipLaunchKernelGGL((eig_jacobi_kernel<36>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
+                           (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
+        break;
+    case 54:
+        hipLaunchKernelGGL((eig_jacobi_kernel<54>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
+                           (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
+        break;
+    default:
+        th_set_error("spectral Green functions are built for d = 18, 36, 54");
+        return TH_E_UNSUPPORTED;
+    }
+    TH_HIP(hipGetLastError());
+    return TH_OK;
 }

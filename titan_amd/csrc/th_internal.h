@@ -22,6 +22,13 @@ constexpr int TH_MAX_NA = 32;
 
 inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
 
+// eigenpairs of H(k + q) for the unique k-points of the work lists
+struct th_eig_set {
+    double2 *V = nullptr;        // [n_kuniq][d][d] row-major, V[r][m] = component r of eigenvector m
+    double *L = nullptr;         // [n_kuniq][d]
+    double q[3] = {0, 0, 0};
+};
+
 struct th_handle {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -68,6 +75,16 @@ struct th_handle {
     int kernel_variant = 0;      // 0 default kernels, 1..3 alternates (tuning / comparison)
     int use_graph = 0;           // th_chi_sweep replays one captured hipGraph per omega
     int contract_3m = 0;         // contraction with three real products per complex one
+    // spectral Green functions (Hermitian H, d = 18, 36, 54)
+    int hermitian = 0;           // H(k) is Hermitian for every k (checked at th_create)
+    int spectral = 1;            // use them when possible
+    long n_kuniq = 0;            // unique k-points of the imaginary list and the real mesh
+    double *d_kuniq = nullptr;   // [n_kuniq][3]
+    int *d_kid_im = nullptr, *d_kid_re = nullptr;   // pair / mesh point -> unique k-point
+    th_eig_set eig[2];           // [0]: q = 0, [1]: the last q != 0
+    int eig_valid[2] = {0, 0};
+    std::vector<int> kid_im_local;        // imaginary point -> index in kim_uniq
+    std::vector<double> kim_uniq, kre_host;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // [omega == 0, omega != 0]
     int graph_prec[2] = {-1, -1};
     double *d_omega = nullptr;   // omega of the current replay
@@ -135,6 +152,7 @@ int th_reset_status(th_handle *h);
 int th_check_status(th_handle *h);
 int th_ensure_panels(th_handle *h, long pairs);
 int th_upload_nodes(th_handle *h, int n_x, const double *x2, const double *p2);
+int th_ensure_eig(th_handle *h, const double *q);
 
 // ---- kernel launchers (th_green.hip) ----
 // Modes of the green-function kernel: which matrix, and how it is written
@@ -158,6 +176,7 @@ struct th_green_args {
     hipStream_t stream = nullptr;      // optional: launch stream (default: the handle's)
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
+int th_launch_eig(th_handle *h, int which, const double *q);
 
 // ---- contraction (th_gemm.hip) ----
 // acc[axis] += sum over the batch of A (x) B, one complex GEMM per atom pair.

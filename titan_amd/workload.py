@@ -27,6 +27,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs in the sweep")
     ap.add_argument("--adjoint", action="store_true", help="also run th_chi_adjoint after each forward")
     ap.add_argument("--contract-3m", type=int, default=0)
+    ap.add_argument("--spectral", type=int, default=1, help="spectral Green functions where possible")
+    ap.add_argument("--overlap", type=int, default=1, help="two-stream pipeline of inversions and contraction")
     a = ap.parse_args()
     spec = CF.system(a.config)
     rp = CF.run_params(a.config)
@@ -35,7 +37,7 @@ def main():
     dims = tuple(a.nk if n > 1 else 1 for n in rp["nk"])
     kp, wk = Ct.uniform_kmesh(m.b, *dims)
     h = m.handle()
-    h.tune(green_variant=a.green_variant, contract_3m=a.contract_3m)
+    h.tune(green_variant=a.green_variant, contract_3m=a.contract_3m, spectral=a.spectral, overlap=a.overlap)
     h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
     h.set_real_kmesh(kp, wk)
     x2, p2 = Ct.real_epoints(m.Ef, a.omega, rp["parts3"], rp["n3gl"])
