@@ -159,6 +159,19 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    V^H is formed on the matrix cores for every z (FP64
  *                    MFMA; FP32 MFMA in TH_PREC_MIXED) (default); 0 = every
  *                    G by Gauss-Jordan elimination of z-H
+ *   "spectral_chunk" 1 = the spectral kernel takes runs of up to 16 pairs of
+ *                    one k-point per workgroup, loads V once per run and
+ *                    writes the panels from the MFMA accumulators (default);
+ *                    0 = one pair per workgroup (G staged in LDS)
+ *   "pair_order"     1 = th_set_imag_points stores the list in runs of 16
+ *                    pairs of one k-point: the r-th pair of a k-point goes to
+ *                    block r/16 and the list is sorted by (block, k-point,
+ *                    r); th_chi0 walks the real axis the same way (nodes in
+ *                    blocks of 16, each block k-point by k-point), and the
+ *                    contraction sums each batch with its runs transposed
+ *                    (default); 0 = the given order.  Applies from the next
+ *                    th_set_imag_points; chi0 changes only in the rounding
+ *                    of its sum over pairs
  *   "overlap"        1 = th_chi0 pipelines the batches on two streams: the
  *                    eliminations of a batch run while the MFMA contraction
  *                    of the previous one runs (default; it applies when the
@@ -182,6 +195,8 @@ int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_m
  * thx_green_panel: runs the green-function kernel in 'mode' (0 imag A,
  *   1 imag B, 2 real A, 3 real B) for pairs p0..p0+np-1 and returns the
  *   panel [np][nA*nA][2][Mpad] as doubles, Mpad = ceil((2 nOrb)^2/112)*112.
+ *   Takes the path th_chi0 takes under the handle's tuning (spectral and
+ *   pair_order: pairs in the stored order, real axis grouped when set).
  * thx_contract: uploads panels A, B of that layout, runs the contraction and
  *   returns the accumulator [nA*nA][2][Mpad][Mpad] (re block, im block). */
 int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, double omega, double Ef,

@@ -62,54 +62,95 @@ def cfg2():
 # kernel level
 # --------------------------------------------------------------------------
 
+RUN = 16
+
+
+def _stored_order(K):
+    """Order in which th_set_imag_points stores the list with pair_order=1:
+    the r-th pair of a k-point (k-points numbered by first occurrence) goes
+    to block r // 16, sorted by (block, k-point, r)."""
+    first, seen, keys = {}, {}, []
+    for i, k in enumerate(K):
+        c = first.setdefault(tuple(k), len(first))
+        r = seen.get(c, 0)
+        seen[c] = r + 1
+        keys.append((r // RUN, c, r))
+    return np.array(sorted(range(len(K)), key=lambda i: keys[i]))
+
+
+def _real_split(p, nx, nk, grouped):
+    """(node, k-point) of real-axis pair p: node-major, or nodes in blocks of
+    16 with each block k-point by k-point."""
+    if not grouped:
+        return divmod(p, nk)
+    nb = -(-nx // RUN)
+    b = min(p // (RUN * nk), nb - 1)
+    off = p - b * RUN * nk
+    bl = min(RUN, nx - RUN * b)
+    return RUN * b + off % bl, off // bl
+
+
+@pytest.mark.parametrize("path", ["elim", "spectral-pair", "spectral-run"])
+@pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("which", ["fe", "cfg2"])
-def test_green_panels_match_numpy(which, fe, cfg2):
+def test_green_panels_match_numpy(which, order, path, fe, cfg2):
+    """Panels of every mode against numpy inverses: elimination, spectral one
+    pair per workgroup, spectral runs of pairs per k-point; pairs in the given
+    order or grouped (24 energies and 20 real nodes per k-point: two blocks of
+    runs on each axis, read from the start and from the second block)."""
     m, _ = {"fe": fe, "cfg2": cfg2}[which]
     L = _hooks()
     h = m.handle()
-    (K, Y, W), (kp, wk) = _work(m, 2, n1gl=3)
+    h.tune(pair_order=order, spectral=int(path != "elim"), spectral_chunk=int(path == "spectral-run"))
+    (K, Y, W), (kp, wk) = _work(m, 2, parts=3, n1gl=8)
     h.set_imag_points(K, Y, W)
     h.set_real_kmesh(kp, wk)
+    perm = _stored_order(K) if order else np.arange(len(K))
     bs, nA = 2 * m.nOrb, m.nA
     Mpad = ((bs * bs + 111) // 112) * 112
     q = np.array([0.013, -0.021, 0.007])
     om, eta, etap = 3e-3, 1e-3, 7e-4
-    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 3)
-    npair = 6
+    x2, p2 = Ct.real_epoints(m.Ef, om, 2, 10)
+    nk = len(wk)
+    assert len(K) == 24 * nk and len(x2) == 20
+    npair = 20               # several runs per k-point and two workgroups of the run kernel
+    tol = 1e-12 if path == "elim" else 1e-10   # spectral: eigen-decomposition rounding / eta
     for mode in range(4):
-        out = np.zeros(npair * nA * nA * 2 * Mpad)
-        rc = L.thx_green_panel(h._h, mode, 0, npair, q.ctypes.data_as(_dp), om, m.Ef, eta, etap, 0, len(x2),
-                               x2.ctypes.data_as(_dp), p2.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
-        assert rc == 0, L.th_last_error()
-        out = out.reshape(npair, nA * nA, 2, Mpad)
-        assert np.abs(out[..., bs * bs:]).max() == 0.0
-        for p in range(npair):
-            if mode <= 1:
-                k, yv, w = K[p], Y[p], W[p]
-                if mode == 0:
-                    G = np.linalg.inv((m.Ef + om + 1j * (yv + eta)) * np.eye(m.d) - m.hk(k + q))
+        for p0 in (0, RUN * nk):  # first block; second block (the shorter one on the real axis)
+            out = np.zeros(npair * nA * nA * 2 * Mpad)
+            rc = L.thx_green_panel(h._h, mode, p0, npair, q.ctypes.data_as(_dp), om, m.Ef, eta, etap, 0, len(x2),
+                                   x2.ctypes.data_as(_dp), p2.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+            assert rc == 0, L.th_last_error()
+            out = out.reshape(npair, nA * nA, 2, Mpad)
+            assert np.abs(out[..., bs * bs:]).max() == 0.0
+            for pl in range(npair):
+                p = p0 + pl
+                if mode <= 1:
+                    k, yv, w = K[perm[p]], Y[perm[p]], W[perm[p]]
+                    if mode == 0:
+                        G = np.linalg.inv((m.Ef + om + 1j * (yv + eta)) * np.eye(m.d) - m.hk(k + q))
+                    else:
+                        G = np.linalg.inv((m.Ef + 1j * (yv + etap)) * np.eye(m.d) - m.hk(k))
                 else:
-                    G = np.linalg.inv((m.Ef + 1j * (yv + etap)) * np.eye(m.d) - m.hk(k))
-            else:
-                ie, ik = divmod(p, len(wk))
-                k, w = kp[ik], p2[ie] * wk[ik]
-                if mode == 2:
-                    G = np.linalg.inv((x2[ie] + om + 1j * eta) * np.eye(m.d) - m.hk(k + q))
-                else:
-                    G = np.linalg.inv((x2[ie] + 1j * etap) * np.eye(m.d) - m.hk(k))
-            for ij in range(nA * nA):
-                i, j = divmod(ij, nA)
-                if mode == 0:
-                    ref = w * G[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
-                elif mode == 1:
-                    ref = G[j * bs:(j + 1) * bs, i * bs:(i + 1) * bs]
-                elif mode == 2:
-                    Ah = G - G.conj().T
-                    ref = -1j * w * Ah[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
-                else:
-                    ref = G.conj().T[j * bs:(j + 1) * bs, i * bs:(i + 1) * bs]
-                got = (out[p, ij, 0, :bs * bs] + 1j * out[p, ij, 1, :bs * bs]).reshape(bs, bs)
-                assert rel(got, ref) < 1e-12, (mode, p, ij, rel(got, ref))
+                    ie, ik = _real_split(p, len(x2), nk, order)
+                    k, w = kp[ik], p2[ie] * wk[ik]
+                    if mode == 2:
+                        G = np.linalg.inv((x2[ie] + om + 1j * eta) * np.eye(m.d) - m.hk(k + q))
+                    else:
+                        G = np.linalg.inv((x2[ie] + 1j * etap) * np.eye(m.d) - m.hk(k))
+                for ij in range(nA * nA):
+                    i, j = divmod(ij, nA)
+                    if mode == 0:
+                        ref = w * G[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
+                    elif mode == 1:
+                        ref = G[j * bs:(j + 1) * bs, i * bs:(i + 1) * bs]
+                    elif mode == 2:
+                        Ah = G - G.conj().T
+                        ref = -1j * w * Ah[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
+                    else:
+                        ref = G.conj().T[j * bs:(j + 1) * bs, i * bs:(i + 1) * bs]
+                    got = (out[pl, ij, 0, :bs * bs] + 1j * out[pl, ij, 1, :bs * bs]).reshape(bs, bs)
+                    assert rel(got, ref) < tol, (mode, p, ij, rel(got, ref))
     h.close()
 
 
@@ -473,4 +514,60 @@ def test_spectral_green_matches_elimination_and_oracle(which, q):
         h.tune(spectral=1)
         gm = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED)
         assert rel(gm, o) < 1e-6, rel(gm, o)
+    h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["cfg0", "bcc2", "cfg2"])
+def test_kmajor_chunked_spectral_matches_given_order(which):
+    """Pairs grouped by k-point with the run-per-k spectral kernel (V loaded
+    once per run, results written from the accumulators) against the lists in
+    the given order with the per-pair spectral kernel, and both against the
+    oracle; q != 0, both axes, both precisions."""
+    spec = {"cfg0": lambda: CF.system(0), "bcc2": lambda: CF.bcc_supercell(2), "cfg2": lambda: CF.system(2)}[which]()
+    m, s = TBModel(spec), O.OracleSystem(spec)
+    nk = 4 if which == "cfg0" else 2
+    (K, Y, W), (kp, wk) = _work(m, nk, n1gl=8)
+    q = [0.013, -0.021, 0.007]
+    om, eta = 3e-3, 1e-3
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 8)
+    o = s.chi0(q, om, eta, eta, (K, Y, W), (kp, wk), (x2, p2))
+    res = {}
+    for order, chunk in ((1, 1), (0, 0), (1, 0)):
+        h = m.handle()
+        h.tune(pair_order=order, spectral_chunk=chunk)
+        h.set_imag_points(K, Y, W)
+        h.set_real_kmesh(kp, wk)
+        res[order, chunk] = (h.chi0(q, om, m.Ef, eta, eta, x2, p2),
+                             h.chi0(q, om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED))
+        h.close()
+    for key, (g64, gmx) in res.items():
+        print(which, key, "fp64 vs oracle %.2e, mixed vs oracle %.2e" % (rel(g64, o), rel(gmx, o)), flush=True)
+        assert rel(g64, o) < 1e-10
+        assert rel(gmx, o) < 1e-6
+    assert rel(res[1, 1][0], res[0, 0][0]) < 1e-12
+    assert rel(res[1, 1][0], res[1, 0][0]) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [0, 1])
+def test_mixed_error_with_full_batches(order):
+    """Mixed precision against FP64 on the 54 x 54 bcc supercell with a 24^3
+    k-mesh (1.3 M pairs, batches of 43690 pairs): the FP32 partial sums of
+    the contraction stay short enough (split length capped by the library,
+    shorter when pairs are grouped by k-point) for <= 1e-6 relative error."""
+    spec, run = CF.system(2), CF.run_params(2)
+    m = TBModel(spec)
+    kp, wk = Ct.uniform_kmesh(m.b, 24, 24, 24)
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    om = 0.00078125
+    x2, p2 = Ct.real_epoints(m.Ef, om, run["parts3"], run["n3gl"])
+    h = m.handle()
+    h.tune(pair_order=order, splits=6, contract_3m=1, batch_mb=1024)
+    h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+    h.set_real_kmesh(kp, wk)
+    g64 = h.chi0(run["q"], om, m.Ef, run["eta"], run["etap"], x2, p2)
+    gmx = h.chi0(run["q"], om, m.Ef, run["eta"], run["etap"], x2, p2, precision=B.PREC_MIXED)
+    print("order", order, "mixed vs fp64 %.3e" % rel(gmx, g64), flush=True)
+    assert rel(gmx, g64) < 5e-7
     h.close()

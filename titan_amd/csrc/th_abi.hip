@@ -131,6 +131,21 @@ int ensure_panels(th_handle *h, long pairs)
 
 int th_ensure_panels(th_handle *h, long pairs) { return ensure_panels(h, pairs); }
 
+// partial-sum buffer of the contraction for at least 'splits' splits
+int th_ensure_part(th_handle *h, int splits)
+{
+    if (splits <= h->part_cap) return TH_OK;
+    TH_HIP(hipStreamSynchronize(h->stream));
+    TH_HIP(hipStreamSynchronize(h->stream2));
+    hipFree(h->d_part);
+    h->d_part = nullptr;
+    h->part_cap = 0;
+    const size_t MM = (size_t)h->Mpad * h->Mpad;
+    TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)splits * h->nA2 * 2 * MM));
+    h->part_cap = splits;
+    return TH_OK;
+}
+
 // real-axis energy nodes of the current omega to the device
 int th_upload_nodes(th_handle *h, int n_x, const double *x2, const double *p2)
 {
@@ -288,6 +303,10 @@ int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double 
         th_set_error("th_set_real_kmesh must be called before th_chi0 at omega != 0");
         return TH_E_STATE;
     }
+    if (*real_axis && (long)n_x * h->nk_re >= (1L << 31)) {
+        th_set_error("th_chi0: n_x * real k-points must stay below 2^31");
+        return TH_E_ARG;
+    }
     int rc;
     if (*real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
     {
@@ -299,6 +318,7 @@ int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double 
     if ((rc = ensure_bcache(h, Ef, etap, prec, tm, have_b))) return rc;
     long need = batch_pairs(h, h->n_im);
     if (*real_axis) need = std::max(need, batch_pairs(h, (long)n_x * h->nk_re));
+    if ((rc = th_ensure_part(h, th_contract_splits(h, need, prec)))) return rc;
     return ensure_panels(h, need);
 }
 
@@ -338,6 +358,10 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
         a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
         a.omega = omega; a.omega_p = omega_p; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
         a.stream = gs;
+        if (axis == 1) {   // real axis in k-major pair order (runs of n_x pairs per k-point)
+            a.kmajor_re = h->pair_order;
+            a.nx = n_x;
+        }
         a.mode = mode_a;
         a.panel = pA[slot];
         int id = begin(0, gs);
@@ -531,6 +555,7 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
     TH_HIP(hipMemcpy(h->d_h0, h0, sizeof(double2) * (size_t)d * d, hipMemcpyHostToDevice));
     const size_t MM = (size_t)h->Mpad * h->Mpad;
     TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)h->splits * h->nA2 * 2 * MM));
+    h->part_cap = h->splits;
     TH_HIP(hipMalloc(&h->d_acc, sizeof(double) * (size_t)2 * h->nA2 * 2 * MM));
     TH_HIP(hipMalloc(&h->d_chi0, sizeof(double2) * (size_t)h->D * h->D));
     TH_HIP(hipMalloc(&h->d_chi, sizeof(double2) * (size_t)h->D * h->D));
@@ -541,21 +566,15 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
 int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, const double *w)
 {
     if (!h || n < 1 || bad(k) || bad(y) || bad(w)) { th_set_error("th_set_imag_points: bad arguments"); return TH_E_ARG; }
+    if (n >= (1L << 31)) { th_set_error("th_set_imag_points: at most 2^31 - 1 pairs"); return TH_E_ARG; }
     hipFree(h->d_kim); hipFree(h->d_yim); hipFree(h->d_wim); hipFree(h->d_bcache);
     h->d_kim = h->d_yim = h->d_wim = nullptr;
     h->d_bcache = nullptr;
     h->bcache_prec = -1;
-    TH_HIP(hipMalloc(&h->d_kim, sizeof(double) * 3 * n));
-    TH_HIP(hipMalloc(&h->d_yim, sizeof(double) * n));
-    TH_HIP(hipMalloc(&h->d_wim, sizeof(double) * n));
-    TH_HIP(hipMemcpy(h->d_kim, k, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-    TH_HIP(hipMemcpy(h->d_yim, y, sizeof(double) * n, hipMemcpyHostToDevice));
-    TH_HIP(hipMemcpy(h->d_wim, w, sizeof(double) * n, hipMemcpyHostToDevice));
-    h->n_im = n;
     // unique k-points of the list
     std::unordered_map<kkey, int, kkey_hash> ids;
     h->kim_uniq.clear();
-    h->kid_im_local.assign(n, 0);
+    std::vector<int> kid(n);
     for (long p = 0; p < n; ++p) {
         auto key = make_key(k + 3 * p);
         auto it = ids.find(key);
@@ -563,11 +582,54 @@ int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, c
             const int id = (int)(h->kim_uniq.size() / 3);
             ids.emplace(key, id);
             h->kim_uniq.insert(h->kim_uniq.end(), k + 3 * p, k + 3 * p + 3);
-            h->kid_im_local[p] = id;
+            kid[p] = id;
         } else {
-            h->kid_im_local[p] = it->second;
+            kid[p] = it->second;
         }
     }
+    // pair order on the device: as given, or in runs of TH_RUN pairs of one
+    // k-point: the r-th pair of a k-point (in the given order) goes to block
+    // r / TH_RUN, and the list is sorted by (block, k-point, r).  Each run
+    // shares one eigen-decomposition (the spectral kernel loads V once per
+    // run) and consecutive runs belong to different k-points, so that a
+    // batch holds few pairs of any one k-point.  chi0 is a sum over the
+    // pairs; the order changes only the rounding of that sum.
+    std::vector<long> order(n);
+    for (long p = 0; p < n; ++p) order[p] = p;
+    if (h->pair_order) {
+        const size_t nu = h->kim_uniq.size() / 3;
+        std::vector<long> seen(nu, 0), blk(n);
+        long nblk = 0;
+        for (long p = 0; p < n; ++p) {
+            blk[p] = seen[kid[p]]++ / TH_RUN;
+            nblk = std::max(nblk, blk[p] + 1);
+        }
+        // counting sort by k-point, then (stable) by block
+        std::vector<long> start(nu + 1, 0), byk(n);
+        for (long p = 0; p < n; ++p) ++start[kid[p] + 1];
+        for (size_t u = 1; u < start.size(); ++u) start[u] += start[u - 1];
+        for (long p = 0; p < n; ++p) byk[start[kid[p]]++] = p;
+        std::vector<long> bstart(nblk + 1, 0);
+        for (long p = 0; p < n; ++p) ++bstart[blk[p] + 1];
+        for (long b = 1; b <= nblk; ++b) bstart[b] += bstart[b - 1];
+        for (long i = 0; i < n; ++i) order[bstart[blk[byk[i]]]++] = byk[i];
+    }
+    std::vector<double> kk(3 * n), yy(n), ww(n);
+    h->kid_im_local.assign(n, 0);
+    for (long s = 0; s < n; ++s) {
+        const long p = order[s];
+        kk[3 * s] = k[3 * p]; kk[3 * s + 1] = k[3 * p + 1]; kk[3 * s + 2] = k[3 * p + 2];
+        yy[s] = y[p];
+        ww[s] = w[p];
+        h->kid_im_local[s] = kid[p];
+    }
+    TH_HIP(hipMalloc(&h->d_kim, sizeof(double) * 3 * n));
+    TH_HIP(hipMalloc(&h->d_yim, sizeof(double) * n));
+    TH_HIP(hipMalloc(&h->d_wim, sizeof(double) * n));
+    TH_HIP(hipMemcpy(h->d_kim, kk.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    TH_HIP(hipMemcpy(h->d_yim, yy.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    TH_HIP(hipMemcpy(h->d_wim, ww.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    h->n_im = n;
     return rebuild_kunion(h);
 }
 
@@ -786,16 +848,17 @@ int th_tune(th_handle *h, const char *key, long value)
         h->kernel_variant = (int)value;
     } else if (k == "splits") {
         if (value < 1 || value > 4096) { th_set_error("th_tune: splits in 1..4096"); return TH_E_ARG; }
-        if (value > h->splits) {
-            const size_t MM = (size_t)h->Mpad * h->Mpad;
-            hipFree(h->d_part);
-            h->d_part = nullptr;
-            TH_HIP(hipMalloc(&h->d_part, sizeof(double) * (size_t)value * h->nA2 * 2 * MM));
-        }
         h->splits = (int)value;
+        if (int rc = th_ensure_part(h, (int)value)) return rc;
     } else if (k == "spectral") {
         if (value < 0 || value > 1) { th_set_error("th_tune: spectral is 0 or 1"); return TH_E_ARG; }
         h->spectral = (int)value;
+    } else if (k == "spectral_chunk") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: spectral_chunk is 0 or 1"); return TH_E_ARG; }
+        h->spectral_chunk = (int)value;
+    } else if (k == "pair_order") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: pair_order is 0 or 1"); return TH_E_ARG; }
+        h->pair_order = (int)value;   // applies from the next th_set_imag_points
     } else if (k == "overlap") {
         if (value < 0 || value > 1) { th_set_error("th_tune: overlap is 0 or 1"); return TH_E_ARG; }
         h->overlap = (int)value;
@@ -868,9 +931,14 @@ int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, d
     if (rc) return rc;
     if (mode >= TH_G_REAL_A && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
     if ((rc = ensure_panels(h, np))) return rc;
+    if ((rc = th_ensure_eig(h, q))) return rc;   // the spectral path where th_chi0 takes it
     th_green_args a{};
     a.mode = mode; a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
     a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec; a.panel = h->d_pA;
+    if (mode >= TH_G_REAL_A) {
+        a.kmajor_re = h->pair_order;
+        a.nx = n_x;
+    }
     if ((rc = th_launch_green(h, a))) return rc;
     const size_t n = (size_t)np * h->nA2 * 2 * h->Mpad;
     if (prec == TH_PREC_FP64) {
@@ -889,6 +957,7 @@ int thx_contract(th_handle *h, const double *A, const double *B, long np, int pr
     if (!h || np < 1 || bad(A) || bad(B) || bad(acc_out)) { th_set_error("thx_contract: bad arguments"); return TH_E_ARG; }
     int rc = ensure_panels(h, np);
     if (rc) return rc;
+    if ((rc = th_ensure_part(h, th_contract_splits(h, np, prec)))) return rc;
     const size_t n = (size_t)np * h->nA2 * 2 * h->Mpad;
     if (prec == TH_PREC_FP64) {
         TH_HIP(hipMemcpy(h->d_pA, A, sizeof(double) * n, hipMemcpyHostToDevice));

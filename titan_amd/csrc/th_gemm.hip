@@ -57,7 +57,7 @@ template <> struct vec16<float> { typedef float4 type; static constexpr int n = 
 template <typename P, bool K3M>
 __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, const P *__restrict__ B,
                                                        long np, int Mpad, int nA2, int ntile, int splits,
-                                                       long kper, double *__restrict__ part)
+                                                       long kper, unsigned nrun, double *__restrict__ part)
 {
     typedef typename vec16<P>::type V;
     constexpr int VN = vec16<P>::n;
@@ -95,6 +95,19 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
     for (int j = 0; j < (K3M ? 7 : 1); ++j)
         for (int q = 0; q < 4; ++q) acc3[j][q] = 0;
 
+    // Row of the k-th summand.  With nrun > 0 the batch holds runs of
+    // TH_RUN pairs of one k-point (pairs grouped by k-point) and the sum
+    // walks it transposed, row (k mod nrun) * TH_RUN + k / nrun over the
+    // first nrun * TH_RUN rows: a split's FP32 partial sum then collects
+    // pairs of many k-points rather than whole runs of one, whose terms add
+    // coherently and would grow the partial sums (and their rounding).
+    auto rowof = [&](long k) -> long {
+        if (nrun == 0u) return k;
+        const unsigned uk = (unsigned)k;
+        if (uk >= nrun * (unsigned)TH_RUN) return k;
+        const unsigned qd = uk / nrun;
+        return (long)(uk - qd * nrun) * TH_RUN + qd;
+    };
     V ra[PER_T], rb[PER_T];
     auto gload = [&](long k0) {
 #pragma unroll
@@ -106,8 +119,9 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
                 int kk = v / (2 * ROWV), rem = v - kk * 2 * ROWV, comp = rem / ROWV, col = rem - comp * ROWV;
                 long kp = k0 + kk;
                 if (kp < ke) {
-                    za = *reinterpret_cast<const V *>(Ab + kp * pstride + (size_t)comp * Mpad + col * VN);
-                    zb = *reinterpret_cast<const V *>(Bb + kp * pstride + (size_t)comp * Mpad + col * VN);
+                    const size_t row = (size_t)rowof(kp);
+                    za = *reinterpret_cast<const V *>(Ab + row * pstride + (size_t)comp * Mpad + col * VN);
+                    zb = *reinterpret_cast<const V *>(Bb + row * pstride + (size_t)comp * Mpad + col * VN);
                 }
             }
             ra[u] = za;
@@ -383,20 +397,44 @@ __global__ void finalize_kernel(const double *__restrict__ acc, double2 *__restr
     (void)nA2;
 }
 
+}  // namespace
+
+int th_contract_splits(const th_handle *h, long np, int prec)
+{
+    long s = h->splits;
+    if (prec == TH_PREC_MIXED) {
+        // measured on the 54 x 54 bcc supercell: the rounding of a split's
+        // FP32 partial sums grows with its length (1.6e-7 relative at 7282
+        // pairs, 8.8e-7 at 29127)
+        const long lmax = 8192;
+        s = std::max(s, (np + lmax - 1) / lmax);
+    }
+    return (int)std::min<long>(s, 4096);
+}
+
+namespace {
+
 template <typename P>
 int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int axis)
 {
     const int ntile = h->Mpad / TH_TILE;
-    const int splits = h->splits;
+    const int splits = th_contract_splits(h, np, sizeof(P) == sizeof(double) ? TH_PREC_FP64 : TH_PREC_MIXED);
+    if (splits > h->part_cap) {
+        th_set_error("contraction: partial buffer holds " + std::to_string(h->part_cap) + " splits, " +
+                     std::to_string(splits) + " needed");
+        return TH_E_STATE;
+    }
     long kper = (np + splits - 1) / splits;
     kper = ((kper + TH_KC - 1) / TH_KC) * TH_KC;
     const int nwg = ntile * ntile * splits * h->nA2;
+    // pairs grouped by k-point: sum the runs transposed (rows permuted within [0, np))
+    const unsigned nrun = h->pair_order ? (unsigned)(np / TH_RUN) : 0u;
     if (h->contract_3m)
         hipLaunchKernelGGL((contract_kernel<P, true>), dim3(nwg), dim3(NTH), 0, h->stream, (const P *)pA,
-                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, h->d_part);
+                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, nrun, h->d_part);
     else
         hipLaunchKernelGGL((contract_kernel<P, false>), dim3(nwg), dim3(NTH), 0, h->stream, (const P *)pA,
-                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, h->d_part);
+                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, nrun, h->d_part);
     TH_HIP(hipGetLastError());
     const size_t n = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
     double *acc = h->d_acc + (size_t)axis * n;

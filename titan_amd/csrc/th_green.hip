@@ -24,6 +24,8 @@ template <typename R> __device__ inline cplx<R> cmul(cplx<R> a, cplx<R> b)
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 template <typename R> __device__ inline cplx<R> csub(cplx<R> a, cplx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R> __device__ inline cplx<R> cadd(cplx<R> a, cplx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R> __device__ inline cplx<R> conj(cplx<R> a) { return {a.x, -a.y}; }
 template <typename R> __device__ inline cplx<R> cinv(cplx<R> a)
 {
     R n = a.x * a.x + a.y * a.y;
@@ -34,7 +36,8 @@ template <typename R> __device__ inline R cabs1(cplx<R> a) { return fabs(a.x) + 
 struct green_params {
     int mode, prec;
     int nA, nOrb, bs, d, nhop, Mpad, nA2;
-    long p0, nk_re;
+    long p0, np, nk_re;
+    int nx, kmajor_re;       // real axis: nx nodes; pair p = ik * nx + ie when kmajor_re, else ie * nk_re + ik
     double q0, q1, q2, omega, Ef, eta, etap;
     const double *omega_p;   // when set, omega is read from device memory (graph replays)
     const double *R;
@@ -124,6 +127,30 @@ __device__ void gj_invert(cplx<R> *M, int d, cplx<R> *colk, cplx<R> *prow, int *
 
 
 // (k, z, weight) of pair p for the kernel mode
+// node ie and k-point ik of real-axis pair p (pair counts below 2^31,
+// checked on the host, so the divisions are 32-bit).  Node-major:
+// p = ie * nk_re + ik.  Grouped (kmajor_re): the nodes split into blocks of
+// TH_RUN (the last one shorter), block b holds its nodes for every k-point,
+// k-point by k-point: p = b TH_RUN nk_re + ik * len(b) + (ie - b TH_RUN).
+__device__ inline void real_split(const green_params &g, long p, long &ie, long &ik)
+{
+    const unsigned up = (unsigned)p, nk = (unsigned)g.nk_re;
+    if (!g.kmajor_re) {
+        const unsigned a = up / nk;
+        ie = a;
+        ik = up - a * nk;
+        return;
+    }
+    const unsigned nx = (unsigned)g.nx, full = (unsigned)TH_RUN * nk, nb = (nx + TH_RUN - 1) / TH_RUN;
+    unsigned b = up / full;
+    if (b >= nb) b = nb - 1;
+    const unsigned off = up - b * full;
+    const unsigned rest = nx - (unsigned)TH_RUN * b, bl = rest < (unsigned)TH_RUN ? rest : (unsigned)TH_RUN;
+    const unsigned a = off / bl;
+    ik = a;
+    ie = (unsigned)TH_RUN * b + (off - a * bl);
+}
+
 __device__ inline void pair_point(const green_params &g, long p, double &k0, double &k1, double &k2, double &er,
                                   double &ei, double &w)
 {
@@ -138,7 +165,8 @@ __device__ inline void pair_point(const green_params &g, long p, double &k0, dou
             er = g.Ef; ei = g.yim[p] + g.etap;
         }
     } else {
-        long ie = p / g.nk_re, ik = p - ie * g.nk_re;
+        long ie, ik;
+        real_split(g, p, ie, ik);
         k0 = g.kre[3 * ik]; k1 = g.kre[3 * ik + 1]; k2 = g.kre[3 * ik + 2];
         if (g.mode == TH_G_REAL_A) {
             k0 += g.q0; k1 += g.q1; k2 += g.q2;
@@ -787,29 +815,42 @@ __global__ __launch_bounds__(NT) void panel_from_global_kernel(green_params g, c
 // Hermitian H (checked when the model is uploaded).
 
 // Eigen-decomposition of H(k) for a list of k-points by cyclic two-sided
-// Jacobi in parallel (round-robin) order, one workgroup per k-point, A and V
-// in LDS.  Rotation of the pair (p,q): with apq = |apq| e^{i phi} the
-// 2 x 2 block is made real by diag(1, e^{-i phi}) and diagonalised by the
-// symmetric Schur rotation (c, s) of tau = (aqq - app) / (2 |apq|); J =
-// [[c, s], [-s e^{-i phi}, c e^{-i phi}]], A <- J^H A J, V <- V J.
-// Sweeps until the off-diagonal Frobenius norm is below 1e-15 of the
-// whole (at most 30 sweeps).  Output V row-major (V[r][m] = component r of
+// Jacobi in parallel (round-robin) order, one workgroup per k-point.
+// Rotation of the pair (p,q): with apq = |apq| e^{i phi} the 2 x 2 block is
+// made real by diag(1, e^{-i phi}) and diagonalised by the symmetric Schur
+// rotation (c, s) of tau = (aqq - app) / (2 |apq|); J = [[c, s], [-s e^{-i
+// phi}, c e^{-i phi}]], A <- J^H A J, V <- V J.  Written per column m of a
+// round: (M J)[:,m] = alpha_m M[:,m] + beta_m M[:,pi_m] with pi_m the partner
+// of m, alpha_p = c, beta_p = -s e^{-i phi}, alpha_q = c e^{-i phi},
+// beta_q = s; the rows of J^H A use the conjugate coefficients.
+//
+// The NP pairs of a round cover every index once, so A splits into NP x NP
+// disjoint 2 x 2 blocks (rows of pair j1, columns of pair j2); each thread
+// transforms whole blocks (rows, then columns, in registers), so one pass
+// over A per round with one read and one write per entry.  A lives in LDS.
+// V lives in registers: lane m of every wave holds column m of V for the
+// rows r = wave + NW i, and the partner column comes from lane pi_m by a
+// cross-lane shuffle.  Two barriers per round (coefficients, then update).
+// Sweeps until the off-diagonal Frobenius norm is below 1e-15 of the whole
+// (at most 30 sweeps).  Output V row-major (V[r][m] = component r of
 // eigenvector m) and lambda.
-template <int DD>
-__global__ __launch_bounds__(256) void eig_jacobi_kernel(green_params g, const double *klist, double q0, double q1,
-                                                         double q2, double2 *Vout, double *Lout)
+template <int DD, int NT>
+__global__ __launch_bounds__(NT) void eig_jacobi_kernel(green_params g, const double *klist, double q0, double q1,
+                                                        double q2, double2 *Vout, double *Lout)
 {
-    static_assert(DD % 2 == 0 && DD <= 64, "even DD <= 64");
-    constexpr int NP = DD / 2, NR = DD - 1;
+    static_assert(DD % 2 == 0 && DD <= 64, "even DD <= 64 (one lane per column)");
+    constexpr int NP = DD / 2, NR = DD - 1, NW = NT / 64, RPW = (DD + NW - 1) / NW;
+    constexpr int TPC = NT / NP;                  // threads per column pair in the block update
+    constexpr int NA = TPC * NP;                  // threads that take part in it
+    constexpr int BPT = (NP + TPC - 1) / TPC;     // row pairs per such thread
+    static_assert(TPC >= 1 && NT % 64 == 0, "block shape");
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<double> *ph = reinterpret_cast<cplx<double> *>(smem);
-    __shared__ cplx<double> A[DD * DD], V[DD * DD];
-    __shared__ double rc[NP], rs[NP];
-    __shared__ cplx<double> re[NP];
-    __shared__ int rp[NP], rq[NP];
-    __shared__ double red[256];
-    __shared__ int s_more;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    __shared__ cplx<double> A[DD * DD];
+    __shared__ cplx<double> al[DD], be[DD];
+    __shared__ int pt[DD];
+    __shared__ double red[2 * NW];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const long kb = blockIdx.x;
     const double k0 = klist[3 * kb] + q0, k1 = klist[3 * kb + 1] + q1, k2 = klist[3 * kb + 2] + q2;
     for (int n = tid; n < g.nhop; n += NT) {
@@ -836,15 +877,20 @@ __global__ __launch_bounds__(256) void eig_jacobi_kernel(green_params g, const d
             }
         }
         A[e] = x;
-        V[e] = {r == c ? 1.0 : 0.0, 0.0};
     }
+    cplx<double> v[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) v[i] = {wave + NW * i == lane ? 1.0 : 0.0, 0.0};
+    // pair j of round rnd
+    auto pp = [](int j, int rnd) { return j == 0 ? NR : (rnd + j) % NR; };
+    auto qq = [](int j, int rnd) { return j == 0 ? rnd : (rnd - j + NR) % NR; };
+    const int j2 = tid % NP, jr0 = tid / NP;     // column pair and first row pair of this thread
+    const int vm = lane < DD ? lane : 0;         // column of V held by this lane
     __syncthreads();
     for (int sweep = 0; sweep < 30; ++sweep) {
         for (int rnd = 0; rnd < NR; ++rnd) {
             if (tid < NP) {
-                int p, q;
-                if (tid == 0) { p = NR; q = rnd; }
-                else { p = (rnd + tid) % NR; q = (rnd - tid + NR) % NR; }
+                const int p = pp(tid, rnd), q = qq(tid, rnd);
                 const cplx<double> apq = A[p * DD + q];
                 const double app = A[p * DD + p].x, aqq = A[q * DD + q].x;
                 const double mag = sqrt(apq.x * apq.x + apq.y * apq.y);
@@ -858,33 +904,47 @@ __global__ __launch_bounds__(256) void eig_jacobi_kernel(green_params g, const d
                     sv = t * c;
                     e = {apq.x / mag, apq.y / mag};
                 }
-                rp[tid] = p; rq[tid] = q; rc[tid] = c; rs[tid] = sv; re[tid] = e;
+                al[p] = {c, 0.0};
+                be[p] = {-sv * e.x, sv * e.y};
+                pt[p] = q;
+                al[q] = {c * e.x, -c * e.y};
+                be[q] = {sv, 0.0};
+                pt[q] = p;
             }
             __syncthreads();
-            // rows: A[p,:] = c A[p,:] - s e A[q,:],  A[q,:] = s A[p,:] + c e A[q,:]
-            for (int it = tid; it < NP * DD; it += NT) {
-                const int j = it / DD, col = it - (it / DD) * DD;
-                const int p = rp[j], q = rq[j];
-                const double c = rc[j], sv = rs[j];
-                const cplx<double> e = re[j];
-                const cplx<double> x = A[p * DD + col], y = A[q * DD + col];
-                const cplx<double> ey = cmul(e, y);
-                A[p * DD + col] = {c * x.x - sv * ey.x, c * x.y - sv * ey.y};
-                A[q * DD + col] = {sv * x.x + c * ey.x, sv * x.y + c * ey.y};
+            if (tid < NA) {
+                const int p2 = pp(j2, rnd), q2 = qq(j2, rnd);
+                const cplx<double> a2p = al[p2], b2p = be[p2], a2q = al[q2], b2q = be[q2];
+#pragma unroll
+                for (int b = 0; b < BPT; ++b) {
+                    const int j1 = jr0 + TPC * b;
+                    if (j1 < NP) {
+                        const int p1 = pp(j1, rnd), q1 = qq(j1, rnd);
+                        const cplx<double> a1p = conj(al[p1]), b1p = conj(be[p1]);
+                        const cplx<double> a1q = conj(al[q1]), b1q = conj(be[q1]);
+                        const cplx<double> x00 = A[p1 * DD + p2], x01 = A[p1 * DD + q2];
+                        const cplx<double> x10 = A[q1 * DD + p2], x11 = A[q1 * DD + q2];
+                        // rows p1, q1 of J^H A at columns p2, q2
+                        const cplx<double> r0p = cadd(cmul(a1p, x00), cmul(b1p, x10));
+                        const cplx<double> r0q = cadd(cmul(a1p, x01), cmul(b1p, x11));
+                        const cplx<double> r1p = cadd(cmul(a1q, x10), cmul(b1q, x00));
+                        const cplx<double> r1q = cadd(cmul(a1q, x11), cmul(b1q, x01));
+                        // then columns p2, q2 of (J^H A) J
+                        A[p1 * DD + p2] = cadd(cmul(a2p, r0p), cmul(b2p, r0q));
+                        A[p1 * DD + q2] = cadd(cmul(a2q, r0q), cmul(b2q, r0p));
+                        A[q1 * DD + p2] = cadd(cmul(a2p, r1p), cmul(b2p, r1q));
+                        A[q1 * DD + q2] = cadd(cmul(a2q, r1q), cmul(b2q, r1p));
+                    }
+                }
             }
-            __syncthreads();
-            // columns of A and V: M[:,p] = c M[:,p] - s conj(e) M[:,q],  M[:,q] = s M[:,p] + c conj(e) M[:,q]
-            for (int it = tid; it < 2 * NP * DD; it += NT) {
-                const int which = it / (NP * DD), rem = it - which * NP * DD;
-                const int j = rem / DD, row = rem - (rem / DD) * DD;
-                cplx<double> *M = which ? V : A;
-                const int p = rp[j], q = rq[j];
-                const double c = rc[j], sv = rs[j];
-                const cplx<double> ec = {re[j].x, -re[j].y};
-                const cplx<double> x = M[row * DD + p], y = M[row * DD + q];
-                const cplx<double> ey = cmul(ec, y);
-                M[row * DD + p] = {c * x.x - sv * ey.x, c * x.y - sv * ey.y};
-                M[row * DD + q] = {sv * x.x + c * ey.x, sv * x.y + c * ey.y};
+            {
+                const cplx<double> am = al[vm], bm = be[vm];
+                const int src = lane < DD ? pt[vm] : lane;
+#pragma unroll
+                for (int i = 0; i < RPW; ++i) {
+                    const cplx<double> y = {__shfl(v[i].x, src, 64), __shfl(v[i].y, src, 64)};
+                    v[i] = cadd(cmul(am, v[i]), cmul(bm, y));
+                }
             }
             __syncthreads();
         }
@@ -896,26 +956,25 @@ __global__ __launch_bounds__(256) void eig_jacobi_kernel(green_params g, const d
             tot += m2;
             if (e / DD != e - (e / DD) * DD) off += m2;
         }
-        red[tid] = off;
-        __syncthreads();
-        for (int o = NT / 2; o > 0; o >>= 1) {
-            if (tid < o) red[tid] += red[tid + o];
-            __syncthreads();
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o, 64);
+            tot += __shfl_xor(tot, o, 64);
         }
-        const double offs = red[0];
+        if (lane == 0) { red[wave] = off; red[NW + wave] = tot; }
         __syncthreads();
-        red[tid] = tot;
-        __syncthreads();
-        for (int o = NT / 2; o > 0; o >>= 1) {
-            if (tid < o) red[tid] += red[tid + o];
-            __syncthreads();
-        }
-        if (tid == 0) s_more = offs > 1e-30 * red[0];
-        __syncthreads();
-        if (!s_more) break;
+        double so = 0.0, st = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) { so += red[w]; st += red[NW + w]; }
+        __syncthreads();   // red is read by every thread before the next sweep writes it
+        if (!(so > 1e-30 * st)) break;
     }
     double2 *vo = Vout + (size_t)kb * DD * DD;
-    for (int e = tid; e < DD * DD; e += NT) vo[e] = make_double2(V[e].x, V[e].y);
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+        const int r = wave + NW * i;
+        if (r < DD && lane < DD) vo[(size_t)r * DD + lane] = make_double2(v[i].x, v[i].y);
+    }
     for (int m = tid; m < DD; m += NT) Lout[(size_t)kb * DD + m] = A[m * DD + m].x;
 }
 
@@ -958,7 +1017,9 @@ __global__ __launch_bounds__(256) void green_spectral_kernel(green_params g)
     double k0, k1, k2, er, ei, w;
     pair_point(g, p, k0, k1, k2, er, ei, w);
     const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
-    const long kb = imag ? g.kid_im[p] : g.kid_re[p % g.nk_re];
+    long ie, ik;
+    if (!imag) real_split(g, p, ie, ik);
+    const long kb = imag ? g.kid_im[p] : g.kid_re[ik];
     const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
     for (int e = tid; e < DD * DD; e += 256) {
         const double2 v = Vk[e];
@@ -1015,6 +1076,131 @@ __global__ __launch_bounds__(256) void green_spectral_kernel(green_params g)
         for (int e = tid; e < DD * DD; e += 256) o[e] = make_double2(S[e].x, S[e].y);
     }
     write_panel<P, 256>(g, pl, S, ident, DD, w);
+}
+
+// Spectral Green functions for runs of pairs that share a k-point (the pair
+// lists are ordered k-major): workgroup b takes the CH pairs b*CH ..
+// b*CH+CH-1 of the batch, loads V^H of a k-point into LDS and V into
+// registers once per run, and for each pair forms M = V diag(d) V^H on the
+// matrix cores with the diagonal of its panel mode:
+//   imag A  w f        (w Gq)
+//   imag B  f          (G, stored as the (j,i) block of panel (i,j))
+//   real A  2 w Im f   (-i w (Gq - Gq^H) = 2 w V Im(f) V^H, Hermitian)
+//   real B  conj(f)    (G^H, stored as the (j,i) block of panel (i,j))
+// f_m = 1/(z - lambda_m).  Results go from the accumulators straight into
+// the panel (no staging of G), padding entries are written as zeros.
+template <typename P, int DD, int CH>
+__global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_chunk_kernel(green_params g)
+{
+    typedef P R;
+    constexpr int NB = (DD + 15) / 16, NT = 64 * NB, KS = (DD + 3) / 4, KP = 4 * KS, NC = 16 * NB;
+    __shared__ cplx<R> Vh[KP * NC];      // Vh[m][c] = conj(V[c][m]); zero for m or c >= DD
+    __shared__ double lam[KP];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const long s0 = (long)blockIdx.x * CH;
+    const long s1 = s0 + CH < g.np ? s0 + CH : g.np;
+    const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
+    const bool bside = g.mode == TH_G_IMAG_B || g.mode == TH_G_REAL_B;
+    const int bs = g.bs, nA = g.nA, Mpad = g.Mpad, Mb = bs * bs;
+    for (int e = tid; e < KP * NC; e += NT) Vh[e] = {R(0), R(0)};
+    // panel offsets of this lane's accumulator entries: row r = 16 wave +
+    // row(q), column c = 16 j + (lane & 15); atom blocks r / bs, c / bs
+    int rI[4], rR[4], cI[NB], cR[NB];
+    bool rv[4], cv[NB];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 16 * wave + smfma<R>::row(lane >> 4, q);
+        rv[q] = r < DD;
+        rI[q] = r / bs;
+        rR[q] = (r - rI[q] * bs) * bs;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int c = 16 * j + (lane & 15);
+        cv[j] = c < DD;
+        cI[j] = c / bs;
+        cR[j] = c - cI[j] * bs;
+    }
+    const int arow = 16 * wave + (lane & 15);   // row of V in this lane's A operand
+    long cur = -1;
+    cplx<double> av[KS];
+    for (long pl = s0; pl < s1; ++pl) {
+        const long p = g.p0 + pl;
+        double k0, k1, k2, er, ei, w;
+        pair_point(g, p, k0, k1, k2, er, ei, w);
+        long kb;
+        if (imag) {
+            kb = g.kid_im[p];
+        } else {
+            long ie, ik;
+            real_split(g, p, ie, ik);
+            kb = g.kid_re[ik];
+        }
+        if (kb != cur) {                        // same kb in every thread
+            __syncthreads();                    // the previous run's reads of Vh are done
+            const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
+            for (int e = tid; e < DD * DD; e += NT) {
+                const int c = e / DD, m = e - (e / DD) * DD;
+                const double2 x = Vk[e];
+                Vh[m * NC + c] = {(R)x.x, (R)(-x.y)};
+            }
+            for (int m = tid; m < KP; m += NT) lam[m] = m < DD ? g.eigL[(size_t)kb * DD + m] : 0.0;
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int m = 4 * ks + (lane >> 4);
+                cplx<double> v = {0.0, 0.0};
+                if (arow < DD && m < DD) {
+                    const double2 x = Vk[(size_t)arow * DD + m];
+                    v = {x.x, x.y};
+                }
+                av[ks] = v;
+            }
+            cur = kb;
+        }
+        // diagonal d = (ca Re f + cb Im f) + i cc Im f for the mode
+        double ca = 1.0, cb = 0.0, cc = 1.0;                  // imag B: f
+        if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }        // w f
+        if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }   // 2 w Im f
+        if (g.mode == TH_G_REAL_B) cc = -1.0;                 // conj(f)
+        typename smfma<R>::acc accr[NB], acci[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            for (int q = 0; q < 4; ++q) { accr[j][q] = 0; acci[j][q] = 0; }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int m = 4 * ks + (lane >> 4);
+            const double a = er - lam[m], n = a * a + ei * ei;
+            const double fr = a / n, fi = -ei / n;
+            const cplx<double> dg = {ca * fr + cb * fi, cc * fi};
+            const cplx<double> x = cmul(av[ks], dg);
+            const R ar = (R)x.x, ai = (R)x.y, nai = -ai;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const cplx<R> b = Vh[m * NC + 16 * j + (lane & 15)];
+                accr[j] = smfma<R>::mac(ar, b.x, accr[j]);
+                accr[j] = smfma<R>::mac(nai, b.y, accr[j]);
+                acci[j] = smfma<R>::mac(ar, b.y, acci[j]);
+                acci[j] = smfma<R>::mac(ai, b.x, acci[j]);
+            }
+        }
+        P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (rv[q] && cv[j]) {
+                    const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
+                    const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
+                    out[o] = (P)accr[j][q];
+                    out[o + Mpad] = (P)acci[j][q];
+                }
+        const int npad = Mpad - Mb;
+        for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+            const int part = e / npad;
+            out[(size_t)part * Mpad + Mb + (e - part * npad)] = P(0);
+        }
+    }
 }
 
 
@@ -1173,7 +1359,13 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
 template <typename P, int DD>
 int launch_green_spectral(th_handle *h, const th_green_args &a, const green_params &g)
 {
-    hipLaunchKernelGGL((green_spectral_kernel<P, DD>), dim3((unsigned)a.np), dim3(256), 0, gstream(h, a), g);
+    if (!a.raw && h->spectral_chunk > 0) {   // runs of pairs per k-point; G is not kept
+        constexpr int CH = TH_RUN, NT = 64 * ((DD + 15) / 16);
+        const unsigned nblk = (unsigned)((a.np + CH - 1) / CH);
+        hipLaunchKernelGGL((green_spectral_chunk_kernel<P, DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
+    } else {
+        hipLaunchKernelGGL((green_spectral_kernel<P, DD>), dim3((unsigned)a.np), dim3(256), 0, gstream(h, a), g);
+    }
     TH_HIP(hipGetLastError());
     return TH_OK;
 }
@@ -1247,7 +1439,12 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     green_params g;
     g.mode = a.mode; g.prec = a.prec;
     g.nA = h->nA; g.nOrb = h->nOrb; g.bs = h->bs; g.d = h->d; g.nhop = h->nhop;
-    g.Mpad = h->Mpad; g.nA2 = h->nA2; g.p0 = a.p0; g.nk_re = h->nk_re;
+    g.Mpad = h->Mpad; g.nA2 = h->nA2; g.p0 = a.p0; g.np = a.np; g.nk_re = h->nk_re;
+    g.nx = a.nx; g.kmajor_re = a.kmajor_re;
+    if (a.kmajor_re && a.nx < 1) {
+        th_set_error("th_launch_green: k-major real pairs need the node count");
+        return TH_E_ARG;
+    }
     g.q0 = a.q[0]; g.q1 = a.q[1]; g.q2 = a.q[2];
     g.omega = a.omega; g.Ef = a.Ef; g.eta = a.eta; g.etap = a.etap;
     g.omega_p = a.omega_p;
@@ -1285,15 +1482,15 @@ int th_launch_eig(th_handle *h, int which, const double *q)
     const size_t lds = sizeof(double2) * std::max(1, h->nhop);
     switch (h->d) {
     case 18:
-        hipLaunchKernelGGL((eig_jacobi_kernel<18>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
+        hipLaunchKernelGGL((eig_jacobi_kernel<18, 128>), dim3((unsigned)h->n_kuniq), dim3(128), lds, h->stream, g,
                            (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
         break;
     case 36:
-        hipLaunchKernelGGL((eig_jacobi_kernel<36>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
+        hipLaunchKernelGGL((eig_jacobi_kernel<36, 256>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
                            (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
         break;
     case 54:
-        hipLaunchKernelGGL((eig_jacobi_kernel<54>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
+        hipLaunchKernelGGL((eig_jacobi_kernel<54, 256>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
                            (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
         break;
     default:

@@ -14,6 +14,7 @@
 // 112-wide output tile.
 constexpr int TH_TILE = 112;        // output tile edge (7 MFMA blocks of 16)
 constexpr int TH_KC = 8;            // pairs per LDS stage of the GEMM
+constexpr int TH_RUN = 16;          // pairs of one k-point per workgroup of the run-per-k spectral kernel
 constexpr int TH_GEMM_WAVES = 7;    // one wave per 16-row block of the tile
 constexpr int TH_MAX_NORB = 9;
 constexpr int TH_MAX_D_LDS = 64;    // largest d of the generic LDS-resident inversion
@@ -56,7 +57,8 @@ struct th_handle {
     void *d_pA = nullptr, *d_pB = nullptr;    // panels [batch][nA2][2][Mpad]
     size_t panel_bytes = 0;
     int splits = 0;
-    double *d_part = nullptr;    // [splits][nA2][2][Mpad][Mpad]
+    double *d_part = nullptr;    // [part_cap][nA2][2][Mpad][Mpad]
+    int part_cap = 0;            // splits the partial buffer holds
     double *d_acc = nullptr;     // [2 axes][nA2][2][Mpad][Mpad]
     double2 *d_chi0 = nullptr;   // D*D
     double2 *d_chi = nullptr;    // D*D
@@ -78,6 +80,8 @@ struct th_handle {
     // spectral Green functions (Hermitian H, d = 18, 36, 54)
     int hermitian = 0;           // H(k) is Hermitian for every k (checked at th_create)
     int spectral = 1;            // use them when possible
+    int spectral_chunk = 1;      // spectral G for runs of pairs per k-point (V loaded once per run)
+    int pair_order = 1;          // imaginary list sorted by k-point, real axis k-major (runs per k-point)
     long n_kuniq = 0;            // unique k-points of the imaginary list and the real mesh
     double *d_kuniq = nullptr;   // [n_kuniq][3]
     int *d_kid_im = nullptr, *d_kid_re = nullptr;   // pair / mesh point -> unique k-point
@@ -174,6 +178,7 @@ struct th_green_args {
     double2 *raw = nullptr;   // optional: G itself, [np][d][d] row-major FP64
     const double *omega_p = nullptr;   // optional: omega in device memory (graph replays)
     hipStream_t stream = nullptr;      // optional: launch stream (default: the handle's)
+    int kmajor_re = 0, nx = 0;         // real axis: pairs k-major (p = ik * nx + ie) instead of node-major
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
 int th_launch_eig(th_handle *h, int which, const double *q);
@@ -181,6 +186,10 @@ int th_launch_eig(th_handle *h, int which, const double *q);
 // ---- contraction (th_gemm.hip) ----
 // acc[axis] += sum over the batch of A (x) B, one complex GEMM per atom pair.
 int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis);
+// K-splits the contraction uses for a batch of np pairs: the tuned value,
+// raised in FP32 so that no partial sum runs over more than 8192 pairs
+int th_contract_splits(const th_handle *h, long np, int prec);
+int th_ensure_part(th_handle *h, int splits);
 int th_launch_zero(th_handle *h, double *p, size_t n);
 // chi0 = (T_imag + T_imag^dagger + T_real) / 2pi in the chi basis.
 int th_launch_finalize(th_handle *h, double2 *chi0);
