@@ -176,6 +176,11 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    eliminations of a batch run while the MFMA contraction
  *                    of the previous one runs (default; it applies when the
  *                    spectral path is not taken); 0 = one stream
+ *   "contract_f16"   TH_PREC_MIXED: 1 = the contraction runs on the fp16 matrix
+ *                    cores, each FP32 panel value split into two fp16 parts
+ *                    (hi + lo, 22 bits) under a power-of-two scale per
+ *                    workgroup, three fp16 products per real product,
+ *                    FP32 accumulation (default); 0 = FP32 MFMA
  *   "contract_3m"    1 = the contraction forms each complex product from three
  *                    real MFMA products (Ar Br, Ai Bi, (Ar+Ai)(Br+Bi)) instead
  *                    of four; 0 = four (default)

@@ -182,6 +182,40 @@ def test_contraction_matches_numpy(fe, prec, tol, npair):
     h.close()
 
 
+@pytest.mark.parametrize("f16", [0, 1])
+@pytest.mark.parametrize("npair", [37, 5000, 40000])
+def test_mixed_contraction_wide_range(fe, f16, npair):
+    """Mixed contraction (FP32 MFMA, or fp16 MFMA on two-term splits with a
+    per-workgroup power-of-two scale) on panels whose magnitudes span 1e-8 ..
+    1e3 across pairs, with a jump by 1e6 half-way through (the scale must
+    shrink and the partial sums be rescaled), against numpy in FP64; the
+    split path must be as accurate as the FP32 one (measured 1.1e-6 against
+    1.5e-6 at 40000 pairs)."""
+    m, _ = fe
+    L = _hooks()
+    h = m.handle()
+    h.tune(contract_f16=f16, splits=3)
+    Mpad = 336
+    rng = np.random.default_rng(npair + 11)
+    mag = 10.0 ** rng.uniform(-8, -3, size=(npair, 1, 1))
+    mag[npair // 2:] *= 1e6
+    A = (rng.normal(size=(npair, 2, Mpad)) * mag).astype(np.float32).astype(np.float64)
+    Bm = (rng.normal(size=(npair, 2, Mpad)) * 10.0 ** rng.uniform(-2, 2, size=(npair, 1, 1))).astype(np.float32).astype(np.float64)
+    A[:, :, 324:] = 0
+    Bm[:, :, 324:] = 0
+    acc = np.zeros(2 * Mpad * Mpad)
+    rc = L.thx_contract(h._h, np.ascontiguousarray(A).ctypes.data_as(_dp),
+                        np.ascontiguousarray(Bm).ctypes.data_as(_dp), npair, B.PREC_MIXED, acc.ctypes.data_as(_dp))
+    assert rc == 0, L.th_last_error()
+    acc = acc.reshape(2, Mpad, Mpad)
+    T = acc[0] + 1j * acc[1]
+    ref = (A[:, 0] + 1j * A[:, 1]).T @ (Bm[:, 0] + 1j * Bm[:, 1])
+    print("f16", f16, "npair", npair, "rel %.2e" % rel(T, ref), flush=True)
+    # random terms: FP32 partial sums of up to 8192 pairs (both paths ~1e-6 at 40000)
+    assert rel(T, ref) < 3e-6
+    h.close()
+
+
 # --------------------------------------------------------------------------
 # chi0 and Dyson against the oracle (FP64 path, <= 1e-10)
 # --------------------------------------------------------------------------

@@ -9,7 +9,10 @@ of the library's tuning knobs (th_tune):
                  matrix, 3 two waves per matrix, where built for d)
   overlap        eliminations of one batch concurrent with the contraction
                  of the previous one on a second stream
+  contract_f16   mixed precision: contraction on fp16 MFMA with two-term
+                 splits of the FP32 panels (1) or on FP32 MFMA (0)
   contract_3m    three instead of four real MFMA products per complex one
+                 (FP32/FP64 MFMA contraction)
   splits         K-splits of the contraction (workgroups per output tile)
   batch_mb       panel memory per batch (pairs per launch)
 
@@ -42,6 +45,7 @@ def _knobs(h, base, setting):
     if not setting["spectral"]:
         yield "green_variant", list(VARIANTS.get(h.d, (0,)))
         yield "overlap", [1, 0]
+    yield "contract_f16", [1, 0]
     yield "contract_3m", [0, 1]
     yield "splits", sorted({max(1, s0 // 2), s0, 2 * s0})
     yield "batch_mb", [256, 512, 1024]
@@ -56,7 +60,8 @@ def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, r
     t0 = time.perf_counter()
     base = h.tuning()
     setting = {"spectral": 1 if h.d in SPECTRAL_D else 0, "green_variant": base["green_variant"],
-               "overlap": 1, "contract_3m": 0, "splits": base["splits"], "batch_mb": base["batch_mb"]}
+               "overlap": 1, "contract_f16": 1, "contract_3m": 0, "splits": base["splits"],
+               "batch_mb": base["batch_mb"]}
     table = []
 
     def timed(st):

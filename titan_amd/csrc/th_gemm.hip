@@ -199,6 +199,238 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
         }
 }
 
+// The contraction for FP32 panels on the half-precision matrix cores
+// (v_mfma_f32_16x16x32_f16: 32 pairs per instruction at 16x the FP32 MFMA
+// rate).  Each FP32 value x is split as x 2^e = hi + lo with hi = fp16(x 2^e)
+// and lo = fp16(x 2^e - hi) (22 significant bits together), and each real
+// product is hi.hi + hi.lo + lo.hi accumulated in FP32 (the dropped lo.lo is
+// 2^-22 of the product): three half-precision MFMAs instead of eight FP32 ones
+// per 32 pairs.  e is a power of two per workgroup and operand: the largest
+// |x| of every stage of 32 pairs is found first (LDS atomic maximum), e keeps
+// the running maximum below 2^14 (fp16 holds 65504), and when a stage needs a
+// smaller e the accumulators are rescaled by the exact power of two.  The
+// partial sums leave scaled back by 2^-(eA+eB) in FP64.
+//
+// LDS holds two stages (double buffer) of both operands as [operand][re|im]
+// [hi|lo][112 rows][32 pairs + 8] fp16 (rows padded to 80 B so that the
+// 16-byte fragment reads of 8 lanes fall on distinct banks): 143,360 B, one
+// workgroup per CU.  Stage s + 1 is split into the free buffer in the same
+// basic block as the MFMAs of stage s (so that the scheduler interleaves the
+// conversion with the matrix work), the loads run two stages ahead, one
+// barrier per stage.  Lane l of wave w
+// reads the 8 pairs 8 (l >> 4) .. +7 of row 16 w + (l & 15) (A) or of column
+// row 16 j + (l & 15) (B); results land in rows 4 (l >> 4) + q.
+constexpr int HK = 32;          // pairs per stage (K of one MFMA)
+constexpr int HST = HK + 8;     // LDS row stride in halves
+typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
+
+__device__ inline int pow2_exp_for(unsigned maxbits)
+{
+    // exponent e with (max |x|) 2^e < 2^14; a zero maximum asks for nothing
+    if (maxbits == 0u) return 100;
+    const int E = (int)((maxbits >> 23) & 0xffu) - 126;   // max = m 2^E, m in [0.5, 1)
+    const int e = 14 - E;
+    return e > 100 ? 100 : (e < -100 ? -100 : e);
+}
+
+// Largest of a non-negative 32-bit value over the wave (DPP row shifts, then
+// row broadcasts 15 and 31; lane 63 ends with the maximum).
+__device__ inline unsigned wave_max_u32(unsigned v)
+{
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));   // row_shr:1
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));   // row_shr:2
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));   // row_shr:4
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));   // row_shr:8
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restrict__ A, const float *__restrict__ B,
+                                                          long np, int Mpad, int nA2, int ntile, int splits,
+                                                          long kper, unsigned nrun, double *__restrict__ part)
+{
+    // [buffer][operand][re|im][hi|lo][row * HST + pair]
+    __shared__ __align__(16) _Float16 Ls[2][2][2][2][TH_TILE * HST];
+    __shared__ unsigned smax[3][2];
+    const int nwg = gridDim.x;
+    const int b = blockIdx.x;
+    const int n8 = nwg & ~7;
+    const int lid = (b < n8) ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
+    const int tile = lid % (ntile * ntile);
+    const int rest = lid / (ntile * ntile);
+    const int split = rest % splits;
+    const int ij = rest / splits;
+    const int tm = tile / ntile, tn = tile - tm * ntile;
+    const int r0 = tm * TH_TILE, c0 = tn * TH_TILE;
+    // pair indices of a batch and byte offsets within its panels fit 32 bits
+    // (checked on the host)
+    const unsigned kb = (unsigned)(split * kper);
+    const unsigned ke = (unsigned)min(np, (long)kb + kper);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const unsigned pstride = (unsigned)nA2 * 2u * (unsigned)Mpad;
+    const char *Abc = reinterpret_cast<const char *>(A + (size_t)ij * 2 * Mpad + r0);
+    const char *Bbc = reinterpret_cast<const char *>(B + (size_t)ij * 2 * Mpad + c0);
+    if (tid < 6) smax[tid >> 1][tid & 1] = 0u;
+
+    // staging: thread tid holds column e = tid % 112 of the pairs 8 g .. 8 g + 7
+    // (g = tid / 112) of a stage, both components, both operands
+    const int se = tid % TH_TILE, sg = tid / TH_TILE;
+    const unsigned ntr = nrun * (unsigned)TH_RUN;   // rows summed transposed
+    float va[2][8], vb[2][8];
+    auto gload = [&](unsigned k0) {   // issue the loads of a stage
+        // k-th summand -> row (k mod nrun) TH_RUN + k / nrun for k < ntr:
+        // one division per stage, then stepping
+        const unsigned kp0 = k0 + 8u * (unsigned)sg;
+        const unsigned nr = nrun != 0u ? nrun : 1u;   // (unused when nrun = 0: ntr = 0)
+        unsigned qd = kp0 / nr, rm = kp0 - qd * nr;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const unsigned kp = kp0 + (unsigned)i;
+            const bool ok = kp < ke;
+            unsigned row = (kp < ntr) ? rm * (unsigned)TH_RUN + qd : kp;
+            row = ok ? row : 0u;   // in-bounds address; the value is replaced by 0
+            const unsigned off = (row * pstride + (unsigned)se) * 4u, offi = off + (unsigned)Mpad * 4u;
+            const float a0 = *reinterpret_cast<const float *>(Abc + off);
+            const float a1 = *reinterpret_cast<const float *>(Abc + offi);
+            const float b0 = *reinterpret_cast<const float *>(Bbc + off);
+            const float b1 = *reinterpret_cast<const float *>(Bbc + offi);
+            va[0][i] = ok ? a0 : 0.0f;
+            va[1][i] = ok ? a1 : 0.0f;
+            vb[0][i] = ok ? b0 : 0.0f;
+            vb[1][i] = ok ? b1 : 0.0f;
+            ++rm;
+            qd += rm == nr ? 1u : 0u;
+            rm = rm == nr ? 0u : rm;
+        }
+    };
+    auto gmax = [&](int slot) {   // the loaded stage's largest |x| per operand
+        float ma = 0.0f, mb = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                ma = fmaxf(ma, fabsf(va[c][i]));
+                mb = fmaxf(mb, fabsf(vb[c][i]));
+            }
+        const unsigned wa = wave_max_u32(__float_as_uint(ma)), wb = wave_max_u32(__float_as_uint(mb));
+        if (lane == 0) {
+            atomicMax(&smax[slot][0], wa);
+            atomicMax(&smax[slot][1], wb);
+        }
+    };
+    auto split_store = [&](int buf, float scA, float scB) {
+#pragma unroll
+        for (int op = 0; op < 2; ++op)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                hv8 hi, lo;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float x = (op ? vb[c][i] : va[c][i]) * (op ? scB : scA);
+                    const _Float16 h = (_Float16)x;
+                    hi[i] = h;
+                    lo[i] = (_Float16)(x - (float)h);
+                }
+                *reinterpret_cast<hv8 *>(&Ls[buf][op][c][0][se * HST + 8 * sg]) = hi;
+                *reinterpret_cast<hv8 *>(&Ls[buf][op][c][1][se * HST + 8 * sg]) = lo;
+            }
+    };
+
+    // accumulators: Ar Br, Ai Bi (subtracted at the end), Ar Bi + Ai Br
+    fv4 acp[7], acn[7], aci[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+        for (int q = 0; q < 4; ++q) { acp[j][q] = 0.0f; acn[j][q] = 0.0f; aci[j][q] = 0.0f; }
+    // scale exponents: of the accumulators, of the stage in the current
+    // buffer, of the stage in the next buffer (running minima; 1000 = unset)
+    int accA = 1000, accB = 1000, curA = 1000, curB = 1000, nxtA = 1000, nxtB = 1000;
+
+    const unsigned nstage = (ke > kb) ? (ke - kb + HK - 1) / HK : 0u;
+    __syncthreads();   // smax cleared
+    if (nstage > 0) {
+        gload(kb);
+        gmax(0);
+        __syncthreads();
+        curA = min(1000, pow2_exp_for(smax[0][0]));
+        curB = min(1000, pow2_exp_for(smax[0][1]));
+        split_store(0, ldexpf(1.0f, curA), ldexpf(1.0f, curB));
+        if (nstage > 1) {
+            gload(kb + HK);
+            gmax(1);
+        }
+        __syncthreads();
+    }
+    accA = curA;   // the accumulators start at the scale of the first stage
+    accB = curB;
+    for (unsigned s = 0; s < nstage; ++s) {
+        const int buf = (int)(s & 1u);
+        // The next stage's split, the loads two stages ahead and their maxima
+        // run unconditionally, so that the loop body is one basic block with
+        // the MFMAs: past the last stage they read row 0 (values replaced by
+        // 0), write the buffer that is no longer read and a maximum slot that
+        // is no longer read.
+        {
+            const int slot = (int)((s + 1) % 3);
+            nxtA = min(curA, pow2_exp_for(smax[slot][0]));
+            nxtB = min(curB, pow2_exp_for(smax[slot][1]));
+            split_store(buf ^ 1, ldexpf(1.0f, nxtA), ldexpf(1.0f, nxtB));
+        }
+        gload(kb + (s + 2) * HK);
+        const int fo = (16 * wave + (lane & 15)) * HST + 8 * (lane >> 4);
+        const hv8 arh = *reinterpret_cast<const hv8 *>(&Ls[buf][0][0][0][fo]);
+        const hv8 arl = *reinterpret_cast<const hv8 *>(&Ls[buf][0][0][1][fo]);
+        const hv8 aih = *reinterpret_cast<const hv8 *>(&Ls[buf][0][1][0][fo]);
+        const hv8 ail = *reinterpret_cast<const hv8 *>(&Ls[buf][0][1][1][fo]);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const int bo = (16 * j + (lane & 15)) * HST + 8 * (lane >> 4);
+            const hv8 brh = *reinterpret_cast<const hv8 *>(&Ls[buf][1][0][0][bo]);
+            const hv8 brl = *reinterpret_cast<const hv8 *>(&Ls[buf][1][0][1][bo]);
+            const hv8 bih = *reinterpret_cast<const hv8 *>(&Ls[buf][1][1][0][bo]);
+            const hv8 bil = *reinterpret_cast<const hv8 *>(&Ls[buf][1][1][1][bo]);
+            acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh, acp[j], 0, 0, 0);
+            acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih, acn[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih, aci[j], 0, 0, 0);
+            acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl, acp[j], 0, 0, 0);
+            acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil, acn[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bil, aci[j], 0, 0, 0);
+            acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh, acp[j], 0, 0, 0);
+            acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih, acn[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bih, aci[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh, aci[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl, aci[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh, aci[j], 0, 0, 0);
+        }
+        gmax((int)((s + 2) % 3));   // after the MFMA phase, which hid the loads
+        if (tid < 2) smax[s % 3][tid] = 0u;   // stage s's slot, read one barrier ago
+        // accumulators to the scale of the next stage (rare)
+        if (__builtin_expect(nxtA != accA || nxtB != accB, 0)) {
+            const float f = ldexpf(1.0f, (nxtA - accA) + (nxtB - accB));
+#pragma unroll
+            for (int j = 0; j < 7; ++j)
+                for (int q = 0; q < 4; ++q) { acp[j][q] *= f; acn[j][q] *= f; aci[j][q] *= f; }
+            accA = nxtA;
+            accB = nxtB;
+        }
+        __syncthreads();   // next buffer complete, this one free; stage maxima visible
+        curA = nxtA;
+        curB = nxtB;
+    }
+    const double back = (accA == 1000 || accB == 1000) ? 0.0 : ldexp(1.0, -(accA + accB));
+    double *pr_ = part + (((size_t)split * nA2 + ij) * 2 + 0) * Mpad * Mpad;
+    double *pi_ = pr_ + (size_t)Mpad * Mpad;
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = r0 + wave * 16 + 4 * (lane >> 4) + q;
+            const int col = c0 + j * 16 + (lane & 15);
+            pr_[(size_t)row * Mpad + col] = ((double)acp[j][q] - (double)acn[j][q]) * back;
+            pi_[(size_t)row * Mpad + col] = (double)aci[j][q] * back;
+        }
+}
+
 // Adjoint of the contraction (reverse mode).  With a cotangent That of one
 // axis's accumulator, T = sum_p A_p B_p^T gives
 //     Ahat_p = That conj(B_p),   Bhat_p = That^T conj(A_p)
@@ -429,7 +661,13 @@ int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int
     const int nwg = ntile * ntile * splits * h->nA2;
     // pairs grouped by k-point: sum the runs transposed (rows permuted within [0, np))
     const unsigned nrun = h->pair_order ? (unsigned)(np / TH_RUN) : 0u;
-    if (h->contract_3m)
+    if (sizeof(P) == sizeof(float) && h->contract_f16 && np < (1l << 31) &&
+        (unsigned long)np * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 32)) {
+        long kh = (np + splits - 1) / splits;
+        kh = ((kh + HK - 1) / HK) * HK;
+        hipLaunchKernelGGL(contract_h2_kernel, dim3(nwg), dim3(NTH), 0, h->stream, (const float *)pA,
+                           (const float *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, nrun, h->d_part);
+    } else if (h->contract_3m)
         hipLaunchKernelGGL((contract_kernel<P, true>), dim3(nwg), dim3(NTH), 0, h->stream, (const P *)pA,
                            (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, nrun, h->d_part);
     else
