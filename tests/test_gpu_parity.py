@@ -154,6 +154,43 @@ def test_green_panels_match_numpy(which, order, path, fe, cfg2):
     h.close()
 
 
+@pytest.mark.parametrize("f16", [0, 1])
+@pytest.mark.parametrize("which", ["fe", "cfg2"])
+def test_mixed_spectral_panels_match_fp64(which, f16, fe, cfg2):
+    """Mixed-precision panels of the run kernel (FP32 MFMA, or fp16 MFMA on
+    two-term splits with the per-pair resolvent-bound scale) against the FP64
+    panels of the same pairs, every mode, both axes, eta down to 5e-4."""
+    m, _ = {"fe": fe, "cfg2": cfg2}[which]
+    L = _hooks()
+    h = m.handle()
+    h.tune(spectral=1, spectral_chunk=1, green_f16=f16)
+    (K, Y, W), (kp, wk) = _work(m, 2, parts=3, n1gl=8)
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    bs, nA = 2 * m.nOrb, m.nA
+    Mpad = ((bs * bs + 111) // 112) * 112
+    q = np.array([0.013, -0.021, 0.007])
+    x2, p2 = Ct.real_epoints(m.Ef, 3e-3, 2, 10)
+    npair = 64
+    worst = 0.0
+    for eta in (5e-4, 5e-3):
+        for mode in range(4):
+            out = {}
+            for prec in (B.PREC_FP64, B.PREC_MIXED):
+                o = np.zeros(npair * nA * nA * 2 * Mpad)
+                rc = L.thx_green_panel(h._h, mode, 0, npair, q.ctypes.data_as(_dp), 3e-3, m.Ef, eta, eta, prec,
+                                       len(x2), x2.ctypes.data_as(_dp), p2.ctypes.data_as(_dp), o.ctypes.data_as(_dp))
+                assert rc == 0, L.th_last_error()
+                out[prec] = o.reshape(npair, nA * nA, 2, Mpad)
+            assert np.abs(out[B.PREC_MIXED][..., bs * bs:]).max() == 0.0
+            a, b = out[B.PREC_MIXED], out[B.PREC_FP64]
+            for p in range(npair):       # relative to each pair's panel
+                worst = max(worst, np.linalg.norm(a[p] - b[p]) / np.linalg.norm(b[p]))
+    print(which, "f16", f16, "worst per-pair panel rel err %.2e" % worst, flush=True)
+    assert worst < 2e-6
+    h.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(B.PREC_FP64, 1e-13), (B.PREC_MIXED, 2e-6)])
 @pytest.mark.parametrize("npair", [1, 37, 5000])
 def test_contraction_matches_numpy(fe, prec, tol, npair):

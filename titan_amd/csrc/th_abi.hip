@@ -856,6 +856,9 @@ int th_tune(th_handle *h, const char *key, long value)
     } else if (k == "spectral_chunk") {
         if (value < 0 || value > 1) { th_set_error("th_tune: spectral_chunk is 0 or 1"); return TH_E_ARG; }
         h->spectral_chunk = (int)value;
+    } else if (k == "green_f16") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: green_f16 is 0 or 1"); return TH_E_ARG; }
+        h->green_f16 = (int)value;
     } else if (k == "contract_f16") {
         if (value < 0 || value > 1) { th_set_error("th_tune: contract_f16 is 0 or 1"); return TH_E_ARG; }
         h->contract_f16 = (int)value;

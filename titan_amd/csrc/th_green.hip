@@ -134,21 +134,17 @@ __device__ void gj_invert(cplx<R> *M, int d, cplx<R> *colk, cplx<R> *prow, int *
 // k-point by k-point: p = b TH_RUN nk_re + ik * len(b) + (ie - b TH_RUN).
 __device__ inline void real_split(const green_params &g, long p, long &ie, long &ik)
 {
+    // both layouts computed, the result selected by value (no branch-dependent
+    // stores through the references, which the compiler would keep in memory)
     const unsigned up = (unsigned)p, nk = (unsigned)g.nk_re;
-    if (!g.kmajor_re) {
-        const unsigned a = up / nk;
-        ie = a;
-        ik = up - a * nk;
-        return;
-    }
-    const unsigned nx = (unsigned)g.nx, full = (unsigned)TH_RUN * nk, nb = (nx + TH_RUN - 1) / TH_RUN;
-    unsigned b = up / full;
-    if (b >= nb) b = nb - 1;
-    const unsigned off = up - b * full;
-    const unsigned rest = nx - (unsigned)TH_RUN * b, bl = rest < (unsigned)TH_RUN ? rest : (unsigned)TH_RUN;
-    const unsigned a = off / bl;
-    ik = a;
-    ie = (unsigned)TH_RUN * b + (off - a * bl);
+    const unsigned a0 = up / nk, ie0 = a0, ik0 = up - a0 * nk;
+    const unsigned nx = g.nx > 0 ? (unsigned)g.nx : 1u, full = (unsigned)TH_RUN * nk;
+    const unsigned nb = (nx + TH_RUN - 1) / TH_RUN, bq = up / full, b = bq < nb ? bq : nb - 1;
+    const unsigned off = up - b * full, rest = nx - (unsigned)TH_RUN * b;
+    const unsigned bl = rest < (unsigned)TH_RUN ? rest : (unsigned)TH_RUN;
+    const unsigned a1 = off / bl, ik1 = a1, ie1 = (unsigned)TH_RUN * b + (off - a1 * bl);
+    ie = (long)(g.kmajor_re ? ie1 : ie0);
+    ik = (long)(g.kmajor_re ? ik1 : ik0);
 }
 
 __device__ inline void pair_point(const green_params &g, long p, double &k0, double &k1, double &k2, double &er,
@@ -1203,6 +1199,174 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_chunk_ke
     }
 }
 
+// Mixed precision variant of the run kernel on the fp16 matrix cores
+// (v_mfma_f32_16x16x32_f16, K = 32 eigen-indices per instruction).  The
+// operands are split in two fp16 parts (hi + lo, 22 significant bits):
+// V^H once per run (|V| <= 1 needs no scale), V diag(d) per pair under a
+// power-of-two scale from |d_m| <= max(|ca|, |cb|, |cc|) / Im z (the
+// resolvent bound of a Hermitian H), and each real product is hi.hi +
+// hi.lo + lo.hi accumulated in FP32 (Ar Br and Ai Bi in separate
+// accumulators).  The diagonal d_m is computed once per pair (one eigenvalue
+// per thread, double-buffered in LDS by pair parity, one barrier per pair).
+typedef _Float16 ghv8 __attribute__((ext_vector_type(8)));
+template <int DD, int CH>
+__global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kernel(green_params g)
+{
+    constexpr int NB = (DD + 15) / 16, NT = 64 * NB, KT = (DD + 31) / 32, KP = 32 * KT, NC = 16 * NB;
+    constexpr int VST = KP + 8;                  // LDS row stride in halves (16-byte reads on distinct banks)
+    __shared__ __align__(16) _Float16 Vs[2][2][NC * VST];   // [re|im][hi|lo][c][m] = conj(V[c][m])
+    __shared__ float2 dv[2][KP];                 // diagonal of the pair, by pair parity
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const long s0 = (long)blockIdx.x * CH;
+    const long s1 = s0 + CH < g.np ? s0 + CH : g.np;
+    const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
+    const bool bside = g.mode == TH_G_IMAG_B || g.mode == TH_G_REAL_B;
+    const int bs = g.bs, nA = g.nA, Mpad = g.Mpad, Mb = bs * bs;
+    int rI[4], rR[4], cI[NB], cR[NB];
+    bool rv[4], cv[NB];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 16 * wave + 4 * (lane >> 4) + q;
+        rv[q] = r < DD;
+        rI[q] = r / bs;
+        rR[q] = (r - rI[q] * bs) * bs;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int c = 16 * j + (lane & 15);
+        cv[j] = c < DD;
+        cI[j] = c / bs;
+        cR[j] = c - cI[j] * bs;
+    }
+    const int arow = 16 * wave + (lane & 15);   // row of V in this lane's A operand
+    const int kofs = 8 * (lane >> 4);            // first of the lane's 8 eigen-indices per K step
+    long cur = -1;
+    float avr[KT][8], avi[KT][8];
+    for (long pl = s0; pl < s1; ++pl) {
+        const long p = g.p0 + pl;
+        const int par = (int)(pl & 1);
+        double k0, k1, k2, er, ei, w;
+        pair_point(g, p, k0, k1, k2, er, ei, w);
+        long kb;
+        if (imag) {
+            kb = g.kid_im[p];
+        } else {
+            long ie, ik;
+            real_split(g, p, ie, ik);
+            kb = g.kid_re[ik];
+        }
+        // diagonal coefficients: d = (ca Re f + cb Im f) + i cc Im f
+        double ca = 1.0, cb = 0.0, cc = 1.0;
+        if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }
+        if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
+        if (g.mode == TH_G_REAL_B) cc = -1.0;
+        const double* Lk = g.eigL + (size_t)kb * DD;
+        if (kb != cur) {                        // same kb in every thread
+            __syncthreads();                    // the previous run's reads of Vs are done
+            const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
+            for (int e = tid; e < NC * KP; e += NT) {
+                const int c = e / KP, m = e - (e / KP) * KP;
+                float xr = 0.0f, xi = 0.0f;
+                if (c < DD && m < DD) {
+                    const double2 x = Vk[(size_t)c * DD + m];
+                    xr = (float)x.x;
+                    xi = (float)(-x.y);
+                }
+                const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+                Vs[0][0][c * VST + m] = hr;
+                Vs[0][1][c * VST + m] = (_Float16)(xr - (float)hr);
+                Vs[1][0][c * VST + m] = hi_;
+                Vs[1][1][c * VST + m] = (_Float16)(xi - (float)hi_);
+            }
+#pragma unroll
+            for (int t = 0; t < KT; ++t)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int m = 32 * t + kofs + i;
+                    double2 x = make_double2(0.0, 0.0);
+                    if (arow < DD && m < DD) x = Vk[(size_t)arow * DD + m];
+                    avr[t][i] = (float)x.x;
+                    avi[t][i] = (float)x.y;
+                }
+            cur = kb;
+        }
+        for (int m = tid; m < KP; m += NT) {
+            float2 d = make_float2(0.0f, 0.0f);
+            if (m < DD) {
+                const double a = er - Lk[m], n = a * a + ei * ei;
+                const double fr = a / n, fi = -ei / n;
+                d = make_float2((float)(ca * fr + cb * fi), (float)(cc * fi));
+            }
+            dv[par][m] = d;
+        }
+        __syncthreads();                        // the pair's diagonal (and a new run's V) in LDS
+        // power-of-two scale of V diag(d): |d_m| <= max(|ca|, |cb|, |cc|) / Im z
+        const double bound = fmax(fmax(fabs(ca), fabs(cb)), fabs(cc)) / fabs(ei);
+        // bound = m 2^ex with m in [0.5, 1) (exponent bits; an infinite or zero
+        // bound clamps to the smallest / largest scale)
+        const int ex = (int)((__double_as_longlong(bound) >> 52) & 0x7ff) - 1022;
+        const int es = min(100, max(-100, 14 - ex));
+        const float sc = ldexpf(1.0f, es);
+        ghv8 arh[KT], arl[KT], aih[KT], ail[KT];
+#pragma unroll
+        for (int t = 0; t < KT; ++t)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float2 d = dv[par][32 * t + kofs + i];
+                const float xr = (avr[t][i] * d.x - avi[t][i] * d.y) * sc;
+                const float xi = (avr[t][i] * d.y + avi[t][i] * d.x) * sc;
+                const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+                arh[t][i] = hr;
+                arl[t][i] = (_Float16)(xr - (float)hr);
+                aih[t][i] = hi_;
+                ail[t][i] = (_Float16)(xi - (float)hi_);
+            }
+        sfv4 acp[NB], acn[NB], aci[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            for (int q = 0; q < 4; ++q) { acp[j][q] = 0.0f; acn[j][q] = 0.0f; aci[j][q] = 0.0f; }
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int bo = (16 * j + (lane & 15)) * VST + 32 * t + kofs;
+                const ghv8 brh = *reinterpret_cast<const ghv8 *>(&Vs[0][0][bo]);
+                const ghv8 brl = *reinterpret_cast<const ghv8 *>(&Vs[0][1][bo]);
+                const ghv8 bih = *reinterpret_cast<const ghv8 *>(&Vs[1][0][bo]);
+                const ghv8 bil = *reinterpret_cast<const ghv8 *>(&Vs[1][1][bo]);
+                acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], brh, acp[j], 0, 0, 0);
+                acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], bih, acn[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], bih, aci[j], 0, 0, 0);
+                acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], brl, acp[j], 0, 0, 0);
+                acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], bil, acn[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], bil, aci[j], 0, 0, 0);
+                acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[t], brh, acp[j], 0, 0, 0);
+                acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], bih, acn[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[t], bih, aci[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], brh, aci[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], brl, aci[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], brh, aci[j], 0, 0, 0);
+            }
+        const float back = ldexpf(1.0f, -es);
+        float *out = reinterpret_cast<float *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (rv[q] && cv[j]) {
+                    const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
+                    const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
+                    out[o] = (acp[j][q] - acn[j][q]) * back;
+                    out[o + Mpad] = aci[j][q] * back;
+                }
+        const int npad = Mpad - Mb;
+        for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+            const int part = e / npad;
+            out[(size_t)part * Mpad + Mb + (e - part * npad)] = 0.0f;
+        }
+    }
+}
+
 
 template <typename P, int NT>
 __global__ __launch_bounds__(NT) void green_kernel(green_params g)
@@ -1362,7 +1526,10 @@ int launch_green_spectral(th_handle *h, const th_green_args &a, const green_para
     if (!a.raw && h->spectral_chunk > 0) {   // runs of pairs per k-point; G is not kept
         constexpr int CH = TH_RUN, NT = 64 * ((DD + 15) / 16);
         const unsigned nblk = (unsigned)((a.np + CH - 1) / CH);
-        hipLaunchKernelGGL((green_spectral_chunk_kernel<P, DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
+        if (sizeof(P) == sizeof(float) && h->green_f16)
+            hipLaunchKernelGGL((green_spectral_h2_kernel<DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
+        else
+            hipLaunchKernelGGL((green_spectral_chunk_kernel<P, DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
     } else {
         hipLaunchKernelGGL((green_spectral_kernel<P, DD>), dim3((unsigned)a.np), dim3(256), 0, gstream(h, a), g);
     }

@@ -81,7 +81,8 @@ struct th_handle {
     int hermitian = 0;           // H(k) is Hermitian for every k (checked at th_create)
     int spectral = 1;            // use them when possible
     int spectral_chunk = 1;      // spectral G for runs of pairs per k-point (V loaded once per run)
-    int contract_f16 = 1;        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
+    int contract_f16 = 1;
+    int green_f16 = 1;           // mixed: spectral products on fp16 MFMA with two-term splits        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
     int pair_order = 1;          // imaginary list sorted by k-point, real axis k-major (runs per k-point)
     long n_kuniq = 0;            // unique k-points of the imaginary list and the real mesh
     double *d_kuniq = nullptr;   // [n_kuniq][3]
