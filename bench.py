@@ -123,7 +123,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default kernel shapes")
     ap.add_argument("--tuning", default=None,
-                    help="JSON of th_tune settings to use instead of tuning (e.g. for profiler runs)")
+                    help="JSON of th_tune settings to use instead of tuning (e.g. for profiler runs); "
+                         "either one setting or {\"mixed\": {...}, \"fp64\": {...}}")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the chi(omega) path is built for a single MI355X "
@@ -149,9 +150,14 @@ def main():
     P = wl.pairs(om_time)
     tuning = None
     if args.tuning:
+        # one setting for both precisions, or {"mixed": {...}, "fp64": {...}}
         fixed = json.loads(args.tuning)
-        wl.h.tune(**fixed)
-        tuning = {"best": fixed, "fixed": True}
+        if "mixed" in fixed or "fp64" in fixed:
+            per = {k: {"best": fixed.get(k, {})} for k in ("mixed", "fp64")}
+            tuning = {"best": per["mixed"]["best"], "fixed": True, "per_precision": per}
+        else:
+            wl.h.tune(**fixed)
+            tuning = {"best": fixed, "fixed": True}
     elif not args.no_tune:
         # kernel shapes tuned on every 64th k-point (the autotuner of the library,
         # titan_amd/autotune.py), then the full work list is restored
