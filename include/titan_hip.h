@@ -127,6 +127,35 @@ int th_chi_adjoint(th_handle *h, const double *q, double omega, double Ef, doubl
                    double *t_hat, double *h0_hat, double *U_hat);
 
 /* Releases every device buffer of the handle. */
+/* Orbital traces of a susceptibility and the spin-orbit cross responses
+ * (replaces the post-processing in calculate_chi.F90:91-185 that follows the
+ * Dyson step for every frequency).
+ *   chi     D x D complex128 column-major, or NULL for the chi of the last
+ *           th_dyson / th_chi_sweep on this handle (no transfer)
+ *   schi    out, 4nA x 4nA: schi[(s,i),(t,j)] = sum_{mu,nu}
+ *           chi[(s,i,mu,mu),(t,j,nu,nu)], index s*nA + i (sigmai2i), s in
+ *           (+, up, down, -); may be NULL
+ *   angles  NULL, or per atom (theta, phi) in degrees of its magnetisation:
+ *           schi is then rotated to the local frames, block (i,j) -> A_i
+ *           block B_j with TITAN's rotation_matrices_chi at (180 - theta,
+ *           180 + phi) (calculate_chi.F90:143-185; needs schi)
+ *   lvec    NULL, or nOrb x nOrb x 3 complex128, lvec[mu + nOrb*(gamma +
+ *           nOrb*s)] = <mu|L_s|gamma> (TITAN's Types%lvec)
+ *   ls, sl, ll  out (need lvec), 3nA x 3nA each, index s*nA + i, s = x, y, z:
+ *           <<L,S>>, <<S,L>>, <<L,L>> (calculate_chi.F90:109-140); may be NULL
+ * Outputs are column-major complex128. */
+int th_chi_traces(th_handle *h, const double *chi, const double *lvec, const double *angles, double *schi,
+                  double *ls, double *sl, double *ll);
+
+/* Local spin density <S^{mu nu}_ii>_k = <c+_{i mu} sigma_k c_{i nu}> of the
+ * ground state by an imaginary-axis sweep over the handle's imaginary list
+ * (replaces calcSmunu, mod_sumrule.F90:162-245, which the sum rule check
+ * sumrule() of mod_sumrule.F90:10-109 consumes): S = sum_p w_p (G + G^H)/2 pi
+ * over z = E_F + i(y_p + eta) plus 1/2 on the diagonal of the up-up and
+ * down-down blocks.  S_out: nOrb x nOrb x 3 x nA complex128, S_out[mu +
+ * nOrb*(nu + nOrb*(k + 3*i))] (TITAN's Smunuiivec), k = x, y, z. */
+int th_spin_density(th_handle *h, double Ef, double eta, double *S_out);
+
 int th_destroy(th_handle *h);
 
 /* Text of the last error on this thread (never NULL). */

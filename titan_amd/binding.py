@@ -20,7 +20,7 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_chi_adjoint", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
+           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
            "th_tune", "th_get_tuning")
 
 
@@ -46,6 +46,8 @@ def load():
                                C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]
     L.th_chi_adjoint.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                  _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.th_chi_traces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.th_spin_density.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     L.th_destroy.argtypes = [C.c_void_p]
     L.th_last_error.restype = C.c_char_p
     L.th_limits.argtypes = [_ip, _ip, _ip]
@@ -165,6 +167,29 @@ class Handle:
         s0 = s0.transpose(0, 2, 1).copy()
         s = s.transpose(0, 2, 1).copy()
         return (s0, s, cond, chi) if full else (s0, s, cond)
+
+    def traces(self, chi=None, lvec=None, angles=None):
+        """Spin traces (4nA x 4nA, rotated to the local frames when angles =
+        [(theta_i, phi_i)] in degrees are given) and, with lvec [nOrb, nOrb, 3]
+        (<mu|L_s|gamma>), the <<L,S>>, <<S,L>>, <<L,L>> responses (3nA x 3nA
+        each).  chi None: the chi of the last Dyson step on this handle."""
+        n4, n3 = 4 * self.nA, 3 * self.nA
+        c = _c128(chi) if chi is not None else None
+        lv = _c128(lvec) if lvec is not None else None
+        an = np.ascontiguousarray(angles, dtype=np.float64).ravel() if angles is not None else None
+        schi = np.zeros((n4, n4), dtype=np.complex128, order="F")
+        out = [np.zeros((n3, n3), dtype=np.complex128, order="F") for _ in range(3)] if lvec is not None else [None] * 3
+        ptr = lambda a: a.ctypes.data_as(_dp) if a is not None else None
+        _check(load().th_chi_traces(self._h, ptr(c), ptr(lv), ptr(an), ptr(schi), *[ptr(o) for o in out]),
+               "th_chi_traces")
+        return (np.ascontiguousarray(schi),) + tuple(np.ascontiguousarray(o) if o is not None else None for o in out)
+
+    def spin_density(self, Ef, eta):
+        """<S^{mu nu}_ii> of the imaginary list: array [nA, 3, nOrb, nOrb] (k = x,
+        y, z) of S[mu, nu] (TITAN's Smunuiivec(mu, nu, k, i))."""
+        out = np.zeros(self.nOrb * self.nOrb * 3 * self.nA, dtype=np.complex128)
+        _check(load().th_spin_density(self._h, Ef, eta, out.ctypes.data_as(_dp)), "th_spin_density")
+        return out.reshape(self.nA, 3, self.nOrb, self.nOrb).transpose(0, 1, 3, 2).copy()
 
     def adjoint(self, q, omega, Ef, eta, etap, chi0, chi_hat, U=None, chi=None, x2=None, p2=None,
                 precision=PREC_FP64, want_U=True):
