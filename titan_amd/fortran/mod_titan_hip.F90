@@ -60,6 +60,20 @@ module mod_titan_hip
       complex(c_double_complex), intent(out) :: chi(*)
       real(c_double), intent(out) :: cond
     end function
+    ! traces, cross responses and local-frame rotation (calculate_chi.F90:91-185);
+    ! pass c_null_ptr for chi (the last Dyson result), lvec / angles (not
+    ! wanted) or any output, c_loc(array) otherwise
+    integer(c_int) function th_chi_traces(h, chi, lvec, angles, schi, ls, sl, ll) bind(C, name="th_chi_traces")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, chi, lvec, angles, schi, ls, sl, ll
+    end function
+    ! <S^{mu nu}_ii> of the ground state (calcSmunu, mod_sumrule.F90:162-245)
+    integer(c_int) function th_spin_density(h, Ef, eta, S) bind(C, name="th_spin_density")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), value :: Ef, eta
+      complex(c_double_complex), intent(out) :: S(*)
+    end function
     integer(c_int) function th_destroy(h) bind(C, name="th_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: h
