@@ -222,12 +222,12 @@ def test_contraction_matches_numpy(fe, prec, tol, npair):
 @pytest.mark.parametrize("f16", [0, 1])
 @pytest.mark.parametrize("npair", [37, 5000, 40000])
 def test_mixed_contraction_wide_range(fe, f16, npair):
-    """Mixed contraction (FP32 MFMA, or fp16 MFMA on two-term splits with a
-    per-workgroup power-of-two scale) on panels whose magnitudes span 1e-8 ..
-    1e3 across pairs, with a jump by 1e6 half-way through (the scale must
-    shrink and the partial sums be rescaled), against numpy in FP64; the
-    split path must be as accurate as the FP32 one (measured 1.1e-6 against
-    1.5e-6 at 40000 pairs)."""
+    """Mixed contraction (FP32 MFMA, or fp16 MFMA on two-term splits with one
+    power-of-two scale per operand from its largest entry) on panels whose
+    magnitudes span 1e-8 .. 1e3 across pairs, with a jump by 1e6 half-way
+    through, against numpy in FP64: entries far below the scale keep an
+    absolute error below 2^-39 of it, so the split path must be as accurate
+    as the FP32 one (both about 1e-6 at 40000 pairs)."""
     m, _ = fe
     L = _hooks()
     h = m.handle()

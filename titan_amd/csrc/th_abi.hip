@@ -249,7 +249,10 @@ long batch_pairs(th_handle *h, long total)
 int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &tm, bool *have)
 {
     *have = false;
-    if (h->d_bcache && h->bcache_prec == prec && h->bcache_key[0] == Ef && h->bcache_key[1] == etap) {
+    // stored per batch of the imaginary contraction, rows in its summation order
+    const long bp = batch_pairs(h, h->n_im), geo = h->pair_order ? bp : 0;
+    if (h->d_bcache && h->bcache_prec == prec && h->bcache_key[0] == Ef && h->bcache_key[1] == etap &&
+        h->bcache_bp == geo) {
         *have = true;
         return TH_OK;
     }
@@ -266,7 +269,6 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
         (void)hipGetLastError();
         return TH_OK;
     }
-    const long bp = batch_pairs(h, h->n_im) * 4;
     for (long p0 = 0; p0 < h->n_im; p0 += bp) {
         th_green_args a{};
         a.mode = TH_G_IMAG_B;
@@ -275,6 +277,7 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
         a.Ef = Ef;
         a.etap = etap;
         a.prec = prec;
+        a.sum_order = 1;
         a.panel = (char *)h->d_bcache + (size_t)p0 * h->nA2 * 2 * h->Mpad * prec_size(prec);
         int id = tm.begin(0);
         int rc = th_launch_green(h, a);
@@ -284,6 +287,7 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
     h->bcache_prec = prec;
     h->bcache_key[0] = Ef;
     h->bcache_key[1] = etap;
+    h->bcache_bp = geo;
     *have = true;
     return TH_OK;
 }
@@ -322,6 +326,38 @@ int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double 
     return ensure_panels(h, need);
 }
 
+// Scales of the FP32 panels for the half-precision contraction from the
+// resolvent bound |(z - H)^-1| <= 1/Im z of a Hermitian H: imaginary axis
+// |w G_q| <= |w|/(y + eta) and |G| <= 1/(y + eta'); real axis
+// |w (G_q - G_q^H)| <= 2 |p2| |w_k|/eta (the node weight p2 from device
+// memory) and |G^H| <= 1/eta'.  False when a bound is not finite.
+bool contract_scales(th_handle *h, double eta, double etap, int n_x, th_contract_scale *im, th_contract_scale *re)
+{
+    if (!h->hermitian || !(eta > 0.0) || !(etap > 0.0) || !std::isfinite(eta) || !std::isfinite(etap)) return false;
+    if (h->im_bound_key[0] != eta || h->im_bound_key[1] != etap) {
+        double ba = 0.0, bb = 0.0;
+        for (size_t p = 0; p < h->yim_host.size(); ++p) {
+            ba = std::max(ba, std::fabs(h->wim_host[p]) / (h->yim_host[p] + eta));
+            bb = std::max(bb, 1.0 / (h->yim_host[p] + etap));
+        }
+        h->im_bound[0] = ba;
+        h->im_bound[1] = bb;
+        h->im_bound_key[0] = eta;
+        h->im_bound_key[1] = etap;
+    }
+    for (double b : h->im_bound)
+        if (!(b >= 0.0) || !std::isfinite(b)) return false;
+    im->eA = th_pow2_scale(h->im_bound[0]);
+    im->eB = th_pow2_scale(h->im_bound[1]);
+    re->areal = 2.0 * h->wre_absmax / eta;
+    if (!std::isfinite(re->areal)) return false;
+    re->eA = 0;
+    re->eB = th_pow2_scale(1.0 / etap);
+    re->p2 = h->d_p2;
+    re->n_x = n_x;
+    return true;
+}
+
 // Kernel launches of one omega of chi0 into h->d_chi0 (everything on the
 // handle's stream, no host synchronisation, no allocation: capturable).
 // omega_p, when set, points to omega in device memory.
@@ -349,6 +385,8 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
         TH_HIP(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
     }
     long nb = 0;   // batches issued in this call
+    th_contract_scale sc[2];
+    const bool scaled = prec == TH_PREC_MIXED && contract_scales(h, eta, etap, n_x, &sc[0], &sc[1]);
     auto begin = [&](int st, hipStream_t s) { return tm ? tm->begin(st, s) : -1; };
     auto end = [&](int id) { if (tm) tm->end(id); };
     auto batch = [&](int mode_a, int mode_b, long p0, long np, int axis) -> int {
@@ -358,6 +396,7 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
         a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
         a.omega = omega; a.omega_p = omega_p; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = prec;
         a.stream = gs;
+        a.sum_order = 1;   // panel rows in the contraction's summation order
         if (axis == 1) {   // real axis in k-major pair order (runs of n_x pairs per k-point)
             a.kmajor_re = h->pair_order;
             a.nx = n_x;
@@ -382,7 +421,7 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
             TH_HIP(hipStreamWaitEvent(h->stream, h->ev_green[slot], 0));
         }
         id = begin(1, h->stream);
-        if ((r = th_launch_contract(h, pA[slot], b, np, prec, axis))) return r;
+        if ((r = th_launch_contract(h, pA[slot], b, np, prec, axis, scaled ? &sc[axis] : nullptr))) return r;
         end(id);
         if (ov) TH_HIP(hipEventRecord(h->ev_contract[slot], h->stream));
         ++nb;
@@ -630,6 +669,9 @@ int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, c
     TH_HIP(hipMemcpy(h->d_yim, yy.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(h->d_wim, ww.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     h->n_im = n;
+    h->yim_host = std::move(yy);
+    h->wim_host = std::move(ww);
+    h->im_bound_key[0] = h->im_bound_key[1] = -1.0;
     return rebuild_kunion(h);
 }
 
@@ -644,6 +686,8 @@ int th_set_real_kmesh(th_handle *h, long nk, const double *k, const double *w)
     TH_HIP(hipMemcpy(h->d_wre, w, sizeof(double) * nk, hipMemcpyHostToDevice));
     h->nk_re = nk;
     h->kre_host.assign(k, k + 3 * nk);
+    h->wre_absmax = 0.0;
+    for (long i = 0; i < nk; ++i) h->wre_absmax = std::max(h->wre_absmax, std::fabs(w[i]));
     return rebuild_kunion(h);
 }
 
@@ -976,7 +1020,18 @@ int thx_contract(th_handle *h, const double *A, const double *B, long np, int pr
     }
     const size_t nacc = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
     if ((rc = th_launch_zero(h, h->d_acc, nacc))) return rc;
-    if ((rc = th_launch_contract(h, h->d_pA, h->d_pB, np, prec, 0))) return rc;
+    // scales of the half-precision contraction from the panels' largest entries
+    th_contract_scale sc;
+    if (prec != TH_PREC_FP64) {
+        float ma = 0.0f, mb = 0.0f;
+        for (size_t e = 0; e < n; ++e) {
+            ma = std::max(ma, std::fabs((float)A[e]));
+            mb = std::max(mb, std::fabs((float)B[e]));
+        }
+        sc.eA = th_pow2_scale(ma);
+        sc.eB = th_pow2_scale(mb);
+    }
+    if ((rc = th_launch_contract(h, h->d_pA, h->d_pB, np, prec, 0, &sc))) return rc;
     TH_HIP(hipMemcpyAsync(acc_out, h->d_acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, h->stream));
     TH_HIP(hipStreamSynchronize(h->stream));
     return TH_OK;

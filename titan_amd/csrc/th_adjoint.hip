@@ -469,7 +469,8 @@ __global__ __launch_bounds__(512) void hhat_fused_kernel(assemble_params a, cons
 // raw G of the imaginary-axis B side from the omega-independent panel cache:
 // B(i,j) = vec G(j,i), so G[(j bs + rr) d + i bs + cc] = B[p][i nA + j][rr bs + cc]
 template <typename P>
-__global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, int bs, int Mpad, double2 *raw)
+__global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, int bs, int Mpad, th_rowmap rows,
+                                double2 *raw)
 {
     const int d = nA * bs;
     const size_t dd = (size_t)d * d;
@@ -479,7 +480,7 @@ __global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, in
     const int f = (int)(e - pl * dd);
     const int r = f / d, c = f - (f / d) * d;
     const int j = r / bs, rr = r - j * bs, i = c / bs, cc = c - i * bs;
-    const P *b = panel + ((size_t)pl * nA * nA + (size_t)i * nA + j) * 2 * Mpad;
+    const P *b = panel + ((size_t)th_map_row(rows.pg0 + pl, rows) * nA * nA + (size_t)i * nA + j) * 2 * Mpad;
     raw[e] = make_double2((double)b[rr * bs + cc], (double)b[Mpad + rr * bs + cc]);
 }
 
@@ -871,16 +872,21 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             // the imaginary-axis B side does not depend on omega: when the forward's
             // panel cache holds it (same E_F, eta', precision), read G from there
             const void *pB = h->d_pB;
+            th_rowmap brows;   // rows of pB: this batch's pairs in order
             if (axis == 0 && use_bcache) {
-                const size_t psz = precision == TH_PREC_FP64 ? sizeof(double) : sizeof(float);
-                pB = (const char *)h->d_bcache + (size_t)p0 * h->nA2 * 2 * h->Mpad * psz;
+                // the cache holds the whole list, per batch of the forward's
+                // contraction in its summation order
+                pB = h->d_bcache;
+                brows.pg0 = p0;
+                brows.bpc = h->bcache_bp;
+                brows.ntot = h->n_im;
                 const unsigned g = nblk((size_t)np * dd, 256);
                 if (precision == TH_PREC_FP64)
                     hipLaunchKernelGGL(raw_from_bpanel<double>, dim3(g), dim3(256), 0, h->stream, (const double *)pB,
-                                       np, h->nA, h->bs, h->Mpad, rawB);
+                                       np, h->nA, h->bs, h->Mpad, brows, rawB);
                 else
                     hipLaunchKernelGGL(raw_from_bpanel<float>, dim3(g), dim3(256), 0, h->stream, (const float *)pB, np,
-                                       h->nA, h->bs, h->Mpad, rawB);
+                                       h->nA, h->bs, h->Mpad, brows, rawB);
                 TH_HIP(hipGetLastError());
             } else {
                 a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
@@ -890,7 +896,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             }
             tm.end(id);
             id = tm.begin(1);
-            if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA))) return rc;
+            if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA, brows))) return rc;
             if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB))) return rc;
             h->n_launch[1] += 2;
             tm.end(id);

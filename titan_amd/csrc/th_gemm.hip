@@ -17,6 +17,9 @@
 // stream through LDS in stages of TH_KC pairs, double buffered with a
 // register prefetch.  Partial tiles go to a split buffer and are summed in
 // a fixed order (bitwise reproducible).
+#include <algorithm>
+#include <cmath>
+
 #include "th_internal.h"
 
 namespace {
@@ -57,7 +60,7 @@ template <> struct vec16<float> { typedef float4 type; static constexpr int n = 
 template <typename P, bool K3M>
 __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, const P *__restrict__ B,
                                                        long np, int Mpad, int nA2, int ntile, int splits,
-                                                       long kper, unsigned nrun, double *__restrict__ part)
+                                                       long kper, double *__restrict__ part)
 {
     typedef typename vec16<P>::type V;
     constexpr int VN = vec16<P>::n;
@@ -95,19 +98,6 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
     for (int j = 0; j < (K3M ? 7 : 1); ++j)
         for (int q = 0; q < 4; ++q) acc3[j][q] = 0;
 
-    // Row of the k-th summand.  With nrun > 0 the batch holds runs of
-    // TH_RUN pairs of one k-point (pairs grouped by k-point) and the sum
-    // walks it transposed, row (k mod nrun) * TH_RUN + k / nrun over the
-    // first nrun * TH_RUN rows: a split's FP32 partial sum then collects
-    // pairs of many k-points rather than whole runs of one, whose terms add
-    // coherently and would grow the partial sums (and their rounding).
-    auto rowof = [&](long k) -> long {
-        if (nrun == 0u) return k;
-        const unsigned uk = (unsigned)k;
-        if (uk >= nrun * (unsigned)TH_RUN) return k;
-        const unsigned qd = uk / nrun;
-        return (long)(uk - qd * nrun) * TH_RUN + qd;
-    };
     V ra[PER_T], rb[PER_T];
     auto gload = [&](long k0) {
 #pragma unroll
@@ -119,7 +109,7 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
                 int kk = v / (2 * ROWV), rem = v - kk * 2 * ROWV, comp = rem / ROWV, col = rem - comp * ROWV;
                 long kp = k0 + kk;
                 if (kp < ke) {
-                    const size_t row = (size_t)rowof(kp);
+                    const size_t row = (size_t)kp;
                     za = *reinterpret_cast<const V *>(Ab + row * pstride + (size_t)comp * Mpad + col * VN);
                     zb = *reinterpret_cast<const V *>(Bb + row * pstride + (size_t)comp * Mpad + col * VN);
                 }
@@ -205,11 +195,14 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
 // and lo = fp16(x 2^e - hi) (22 significant bits together), and each real
 // product is hi.hi + hi.lo + lo.hi accumulated in FP32 (the dropped lo.lo is
 // 2^-22 of the product): three half-precision MFMAs instead of eight FP32 ones
-// per 32 pairs.  e is a power of two per workgroup and operand: the largest
-// |x| of every stage of 32 pairs is found first (LDS atomic maximum), e keeps
-// the running maximum below 2^14 (fp16 holds 65504), and when a stage needs a
-// smaller e the accumulators are rescaled by the exact power of two.  The
-// partial sums leave scaled back by 2^-(eA+eB) in FP64.
+// per 32 pairs.  e is one power of two per operand and launch, from the
+// resolvent bound |G| <= 1/Im z of a Hermitian H (imaginary axis: |w G_q| <=
+// |w|/(y + eta), |G| <= 1/(y + eta'); real axis: |w (G_q - G_q^H)| <=
+// 2|w|/eta, |G^H| <= 1/eta'), so that every scaled value stays below 2^14
+// (fp16 holds 65504); a value far below the bound keeps an absolute error
+// below 2^-39 of it (half the fp16 subnormal spacing in lo), so the bound may
+// be loose by orders of magnitude.  The partial sums leave scaled back by
+// 2^-(eA+eB) in FP64.  H must be Hermitian (else the FP32 kernel runs).
 //
 // LDS holds two stages (double buffer) of both operands as [operand][re|im]
 // [hi|lo][112 rows][32 pairs + 8] fp16 (rows padded to 80 B so that the
@@ -217,42 +210,21 @@ __global__ __launch_bounds__(NTH) void contract_kernel(const P *__restrict__ A, 
 // workgroup per CU.  Stage s + 1 is split into the free buffer in the same
 // basic block as the MFMAs of stage s (so that the scheduler interleaves the
 // conversion with the matrix work), the loads run two stages ahead, one
-// barrier per stage.  Lane l of wave w
-// reads the 8 pairs 8 (l >> 4) .. +7 of row 16 w + (l & 15) (A) or of column
-// row 16 j + (l & 15) (B); results land in rows 4 (l >> 4) + q.
+// barrier per stage.  Lane l of wave w reads the 8 pairs 8 (l >> 4) .. +7 of
+// row 16 w + (l & 15) (A) or of column row 16 j + (l & 15) (B); results land
+// in rows 4 (l >> 4) + q.
 constexpr int HK = 32;          // pairs per stage (K of one MFMA)
 constexpr int HST = HK + 8;     // LDS row stride in halves
 typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
 
-__device__ inline int pow2_exp_for(unsigned maxbits)
-{
-    // exponent e with (max |x|) 2^e < 2^14; a zero maximum asks for nothing
-    if (maxbits == 0u) return 100;
-    const int E = (int)((maxbits >> 23) & 0xffu) - 126;   // max = m 2^E, m in [0.5, 1)
-    const int e = 14 - E;
-    return e > 100 ? 100 : (e < -100 ? -100 : e);
-}
-
-// Largest of a non-negative 32-bit value over the wave (DPP row shifts, then
-// row broadcasts 15 and 31; lane 63 ends with the maximum).
-__device__ inline unsigned wave_max_u32(unsigned v)
-{
-    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));   // row_shr:1
-    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));   // row_shr:2
-    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));   // row_shr:4
-    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));   // row_shr:8
-    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
-    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
-    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
-}
-
 __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restrict__ A, const float *__restrict__ B,
                                                           long np, int Mpad, int nA2, int ntile, int splits,
-                                                          long kper, unsigned nrun, double *__restrict__ part)
+                                                          long kper, int eA, int eB,
+                                                          const double *__restrict__ p2, int n_x, double areal,
+                                                          double *__restrict__ part)
 {
     // [buffer][operand][re|im][hi|lo][row * HST + pair]
     __shared__ __align__(16) _Float16 Ls[2][2][2][2][TH_TILE * HST];
-    __shared__ unsigned smax[3][2];
     const int nwg = gridDim.x;
     const int b = blockIdx.x;
     const int n8 = nwg & ~7;
@@ -268,58 +240,45 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
     const unsigned kb = (unsigned)(split * kper);
     const unsigned ke = (unsigned)min(np, (long)kb + kper);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const unsigned pstride = (unsigned)nA2 * 2u * (unsigned)Mpad;
-    const char *Abc = reinterpret_cast<const char *>(A + (size_t)ij * 2 * Mpad + r0);
-    const char *Bbc = reinterpret_cast<const char *>(B + (size_t)ij * 2 * Mpad + c0);
-    if (tid < 6) smax[tid >> 1][tid & 1] = 0u;
+    // raw buffer loads over the split's rows: an offset at or past row ke
+    // reads 0 without touching memory (no select waits on loaded values).
+    // The B resource starts (c0 - r0) floats off B so that one offset
+    // serves both operands; the imaginary halves sit Mpad floats on (scalar
+    // offset).  Panel bytes + 96 rows < 2^31 (checked on the host).
+    const unsigned ps4 = (unsigned)nA2 * 2u * (unsigned)Mpad * 4u;
+    const int nrec = (int)(ke * ps4);
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, nrec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(reinterpret_cast<const char *>(B) + (long)(c0 - r0) * 4), (short)0, nrec, 0x00020000);
+    if (p2) {   // real axis: |A| <= 2 max|p2| max|w_k| / eta = areal max|p2| (omega-dependent, from device memory)
+        double m = 0.0;
+        for (int x = 0; x < n_x; ++x) m = fmax(m, fabs(p2[x]));
+        const double bound = areal * m;
+        const int ex = (int)((__double_as_longlong(bound) >> 52) & 0x7ff) - 1022;
+        eA = bound > 0.0 ? min(100, max(-100, 14 - ex)) : 0;
+    }
+    const float scA = ldexpf(1.0f, eA), scB = ldexpf(1.0f, eB);
 
     // staging: thread tid holds column e = tid % 112 of the pairs 8 g .. 8 g + 7
     // (g = tid / 112) of a stage, both components, both operands
     const int se = tid % TH_TILE, sg = tid / TH_TILE;
-    const unsigned ntr = nrun * (unsigned)TH_RUN;   // rows summed transposed
+    const unsigned base = ((unsigned)ij * 2u * (unsigned)Mpad + (unsigned)(r0 + se)) * 4u;
+    const int imoff = Mpad * 4;
     float va[2][8], vb[2][8];
     auto gload = [&](unsigned k0) {   // issue the loads of a stage
-        // k-th summand -> row (k mod nrun) TH_RUN + k / nrun for k < ntr:
-        // one division per stage, then stepping
-        const unsigned kp0 = k0 + 8u * (unsigned)sg;
-        const unsigned nr = nrun != 0u ? nrun : 1u;   // (unused when nrun = 0: ntr = 0)
-        unsigned qd = kp0 / nr, rm = kp0 - qd * nr;
+        const unsigned v0 = base + (k0 + 8u * (unsigned)sg) * ps4;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const unsigned kp = kp0 + (unsigned)i;
-            const bool ok = kp < ke;
-            unsigned row = (kp < ntr) ? rm * (unsigned)TH_RUN + qd : kp;
-            row = ok ? row : 0u;   // in-bounds address; the value is replaced by 0
-            const unsigned off = (row * pstride + (unsigned)se) * 4u, offi = off + (unsigned)Mpad * 4u;
-            const float a0 = *reinterpret_cast<const float *>(Abc + off);
-            const float a1 = *reinterpret_cast<const float *>(Abc + offi);
-            const float b0 = *reinterpret_cast<const float *>(Bbc + off);
-            const float b1 = *reinterpret_cast<const float *>(Bbc + offi);
-            va[0][i] = ok ? a0 : 0.0f;
-            va[1][i] = ok ? a1 : 0.0f;
-            vb[0][i] = ok ? b0 : 0.0f;
-            vb[1][i] = ok ? b1 : 0.0f;
-            ++rm;
-            qd += rm == nr ? 1u : 0u;
-            rm = rm == nr ? 0u : rm;
+            const int vo = (int)(v0 + (unsigned)i * ps4);
+            va[0][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rA, vo, 0, 0));
+            va[1][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rA, vo, imoff, 0));
+            vb[0][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rB, vo, 0, 0));
+            vb[1][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rB, vo, imoff, 0));
         }
     };
-    auto gmax = [&](int slot) {   // the loaded stage's largest |x| per operand
-        float ma = 0.0f, mb = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                ma = fmaxf(ma, fabsf(va[c][i]));
-                mb = fmaxf(mb, fabsf(vb[c][i]));
-            }
-        const unsigned wa = wave_max_u32(__float_as_uint(ma)), wb = wave_max_u32(__float_as_uint(mb));
-        if (lane == 0) {
-            atomicMax(&smax[slot][0], wa);
-            atomicMax(&smax[slot][1], wb);
-        }
-    };
-    auto split_store = [&](int buf, float scA, float scB) {
+    // x 2^e = hi + lo: hi = fp16(x 2^e), lo = fp16(x 2^e - hi) with one fused
+    // multiply-add on the fp32 value and the fp16 hi
+    auto split_store = [&](int buf) {
 #pragma unroll
         for (int op = 0; op < 2; ++op)
 #pragma unroll
@@ -327,10 +286,10 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
                 hv8 hi, lo;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const float x = (op ? vb[c][i] : va[c][i]) * (op ? scB : scA);
-                    const _Float16 h = (_Float16)x;
+                    const float x = op ? vb[c][i] : va[c][i], sc = op ? scB : scA;
+                    const _Float16 h = (_Float16)(x * sc);
                     hi[i] = h;
-                    lo[i] = (_Float16)(x - (float)h);
+                    lo[i] = (_Float16)fmaf(x, sc, -(float)h);
                 }
                 *reinterpret_cast<hv8 *>(&Ls[buf][op][c][0][se * HST + 8 * sg]) = hi;
                 *reinterpret_cast<hv8 *>(&Ls[buf][op][c][1][se * HST + 8 * sg]) = lo;
@@ -342,40 +301,21 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
 #pragma unroll
     for (int j = 0; j < 7; ++j)
         for (int q = 0; q < 4; ++q) { acp[j][q] = 0.0f; acn[j][q] = 0.0f; aci[j][q] = 0.0f; }
-    // scale exponents: of the accumulators, of the stage in the current
-    // buffer, of the stage in the next buffer (running minima; 1000 = unset)
-    int accA = 1000, accB = 1000, curA = 1000, curB = 1000, nxtA = 1000, nxtB = 1000;
 
     const unsigned nstage = (ke > kb) ? (ke - kb + HK - 1) / HK : 0u;
-    __syncthreads();   // smax cleared
     if (nstage > 0) {
         gload(kb);
-        gmax(0);
-        __syncthreads();
-        curA = min(1000, pow2_exp_for(smax[0][0]));
-        curB = min(1000, pow2_exp_for(smax[0][1]));
-        split_store(0, ldexpf(1.0f, curA), ldexpf(1.0f, curB));
-        if (nstage > 1) {
-            gload(kb + HK);
-            gmax(1);
-        }
+        split_store(0);
+        gload(kb + HK);   // past the end: reads 0
         __syncthreads();
     }
-    accA = curA;   // the accumulators start at the scale of the first stage
-    accB = curB;
     for (unsigned s = 0; s < nstage; ++s) {
         const int buf = (int)(s & 1u);
-        // The next stage's split, the loads two stages ahead and their maxima
-        // run unconditionally, so that the loop body is one basic block with
-        // the MFMAs: past the last stage they read row 0 (values replaced by
-        // 0), write the buffer that is no longer read and a maximum slot that
-        // is no longer read.
-        {
-            const int slot = (int)((s + 1) % 3);
-            nxtA = min(curA, pow2_exp_for(smax[slot][0]));
-            nxtB = min(curB, pow2_exp_for(smax[slot][1]));
-            split_store(buf ^ 1, ldexpf(1.0f, nxtA), ldexpf(1.0f, nxtB));
-        }
+        // the next stage's split and the loads two stages ahead run
+        // unconditionally, so that the loop body is one basic block with the
+        // MFMAs; past the last stage they read 0 and write the buffer that is
+        // no longer read
+        split_store(buf ^ 1);
         gload(kb + (s + 2) * HK);
         const int fo = (16 * wave + (lane & 15)) * HST + 8 * (lane >> 4);
         const hv8 arh = *reinterpret_cast<const hv8 *>(&Ls[buf][0][0][0][fo]);
@@ -402,22 +342,9 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
             aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl, aci[j], 0, 0, 0);
             aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh, aci[j], 0, 0, 0);
         }
-        gmax((int)((s + 2) % 3));   // after the MFMA phase, which hid the loads
-        if (tid < 2) smax[s % 3][tid] = 0u;   // stage s's slot, read one barrier ago
-        // accumulators to the scale of the next stage (rare)
-        if (__builtin_expect(nxtA != accA || nxtB != accB, 0)) {
-            const float f = ldexpf(1.0f, (nxtA - accA) + (nxtB - accB));
-#pragma unroll
-            for (int j = 0; j < 7; ++j)
-                for (int q = 0; q < 4; ++q) { acp[j][q] *= f; acn[j][q] *= f; aci[j][q] *= f; }
-            accA = nxtA;
-            accB = nxtB;
-        }
-        __syncthreads();   // next buffer complete, this one free; stage maxima visible
-        curA = nxtA;
-        curB = nxtB;
+        __syncthreads();   // next buffer complete, this one free
     }
-    const double back = (accA == 1000 || accB == 1000) ? 0.0 : ldexp(1.0, -(accA + accB));
+    const double back = ldexp(1.0, -(eA + eB));
     double *pr_ = part + (((size_t)split * nA2 + ij) * 2 + 0) * Mpad * Mpad;
     double *pi_ = pr_ + (size_t)Mpad * Mpad;
 #pragma unroll
@@ -444,7 +371,7 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
 template <typename P, bool K3M>
 __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict__ X, const double *__restrict__ That,
                                                             long np, int Mpad, int nA2, int ntile, int trans,
-                                                            double *__restrict__ out)
+                                                            double *__restrict__ out, th_rowmap rows)
 {
     __shared__ __align__(16) P Xs[2][TH_KC][2][TH_TILE];
     __shared__ __align__(16) P Ys[2][TH_KC][2][TH_TILE];
@@ -471,6 +398,7 @@ __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict_
         for (int q = 0; q < 4; ++q) acc3[j][q] = 0;
 
     P rx[PER_T], ry[PER_T];
+    long xrow[PER_T];   // panel row of each staged pair (fixed over the stages)
     // Stage element v of X: kk fastest (8 consecutive k of one pair row are
     // one contiguous segment of the panel), then the 112 rows, then re/im.
     // Y follows its memory order: kk fastest for trans = 1 (rows n of That),
@@ -492,8 +420,7 @@ __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict_
             if (v < STAGE) {
                 int kk, m, comp;
                 xidx(v, kk, m, comp);
-                const long pp = p0 + m;
-                if (pp < np) x = Xb[pp * pstride + (size_t)comp * Mpad + k0 + kk];
+                if (p0 + m < np) x = Xb[(size_t)xrow[u] * pstride + (size_t)comp * Mpad + k0 + kk];
                 yidx(v, kk, m, comp);
                 const int n = n0 + m, k = k0 + kk;
                 y = (P)(trans ? Tb[(size_t)comp * MM + (size_t)n * Mpad + k] : Tb[(size_t)comp * MM + (size_t)k * Mpad + n]);
@@ -515,6 +442,12 @@ __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict_
             }
         }
     };
+#pragma unroll
+    for (int u = 0; u < PER_T; ++u) {
+        int kk, m, comp;
+        xidx(tid + u * NTH, kk, m, comp);
+        xrow[u] = th_map_row(rows.pg0 + p0 + m, rows);
+    }
     const int nstage = Mpad / TH_KC;
     gload(0);
     for (int s = 0; s < nstage; ++s) {
@@ -647,7 +580,7 @@ int th_contract_splits(const th_handle *h, long np, int prec)
 namespace {
 
 template <typename P>
-int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int axis)
+int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int axis, const th_contract_scale *sc)
 {
     const int ntile = h->Mpad / TH_TILE;
     const int splits = th_contract_splits(h, np, sizeof(P) == sizeof(double) ? TH_PREC_FP64 : TH_PREC_MIXED);
@@ -659,20 +592,19 @@ int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int
     long kper = (np + splits - 1) / splits;
     kper = ((kper + TH_KC - 1) / TH_KC) * TH_KC;
     const int nwg = ntile * ntile * splits * h->nA2;
-    // pairs grouped by k-point: sum the runs transposed (rows permuted within [0, np))
-    const unsigned nrun = h->pair_order ? (unsigned)(np / TH_RUN) : 0u;
-    if (sizeof(P) == sizeof(float) && h->contract_f16 && np < (1l << 31) &&
-        (unsigned long)np * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 32)) {
+    if (sizeof(P) == sizeof(float) && sc && h->contract_f16 &&
+        (unsigned long)(np + 96) * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 31)) {
         long kh = (np + splits - 1) / splits;
         kh = ((kh + HK - 1) / HK) * HK;
         hipLaunchKernelGGL(contract_h2_kernel, dim3(nwg), dim3(NTH), 0, h->stream, (const float *)pA,
-                           (const float *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, nrun, h->d_part);
+                           (const float *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, sc->eA, sc->eB,
+                           sc->p2, sc->n_x, sc->areal, h->d_part);
     } else if (h->contract_3m)
         hipLaunchKernelGGL((contract_kernel<P, true>), dim3(nwg), dim3(NTH), 0, h->stream, (const P *)pA,
-                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, nrun, h->d_part);
+                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, h->d_part);
     else
         hipLaunchKernelGGL((contract_kernel<P, false>), dim3(nwg), dim3(NTH), 0, h->stream, (const P *)pA,
-                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, nrun, h->d_part);
+                           (const P *)pB, np, h->Mpad, h->nA2, ntile, splits, kper, h->d_part);
     TH_HIP(hipGetLastError());
     const size_t n = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
     double *acc = h->d_acc + (size_t)axis * n;
@@ -686,11 +618,19 @@ int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int
 
 }  // namespace
 
-int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis)
+int th_pow2_scale(double bound)
+{
+    if (!(bound > 0.0) || !std::isfinite(bound)) return 0;
+    const int E = std::ilogb(bound) + 1;   // bound < 2^E
+    return std::min(100, std::max(-100, 14 - E));
+}
+
+int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis,
+                       const th_contract_scale *sc)
 {
     if (np <= 0) return TH_OK;
-    if (prec == TH_PREC_FP64) return launch_contract_t<double>(h, pA, pB, np, axis);
-    return launch_contract_t<float>(h, pA, pB, np, axis);
+    if (prec == TH_PREC_FP64) return launch_contract_t<double>(h, pA, pB, np, axis, nullptr);
+    return launch_contract_t<float>(h, pA, pB, np, axis, sc);
 }
 
 int th_launch_zero(th_handle *h, double *p, size_t n)
@@ -711,7 +651,7 @@ int th_launch_finalize(th_handle *h, double2 *chi0)
 }
 
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
-                            double *out)
+                            double *out, const th_rowmap &rows)
 {
     if (np <= 0) return TH_OK;
     const int ntile = h->Mpad / TH_TILE;
@@ -721,17 +661,17 @@ int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const dou
     if (prec == TH_PREC_FP64) {
         if (h->contract_3m)
             hipLaunchKernelGGL((panel_adjoint_kernel<double, true>), grid, blk, 0, h->stream, (const double *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
         else
             hipLaunchKernelGGL((panel_adjoint_kernel<double, false>), grid, blk, 0, h->stream, (const double *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
     } else {
         if (h->contract_3m)
             hipLaunchKernelGGL((panel_adjoint_kernel<float, true>), grid, blk, 0, h->stream, (const float *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
         else
             hipLaunchKernelGGL((panel_adjoint_kernel<float, false>), grid, blk, 0, h->stream, (const float *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
     }
     TH_HIP(hipGetLastError());
     return TH_OK;

@@ -45,7 +45,9 @@ struct green_params {
     const double2 *t, *h0;
     const int *hoff;
     const double *kim, *yim, *wim, *kre, *wre, *x2, *p2;
-    void *panel;
+    void *panel;                 // panel rows of the batch
+    long slot0;                  // batch index of this launch's first pair
+    unsigned nrun_out;           // summation order: see panel_row
     double2 *raw;
     int *status;
     // spectral mode: eigenpairs of H(k) per k-point of the current sets and
@@ -54,6 +56,18 @@ struct green_params {
     const double *eigL;
     const int *kid_im, *kid_re;
 };
+
+// Panel row of batch pair b = slot0 + pl.  With nrun_out = n > 0 the first
+// TH_RUN n pairs (n runs of TH_RUN pairs of one k-point each) are stored
+// transposed, pair j of run r in row j n + r, so that consecutive rows, the
+// order in which the contraction sums them, belong to different k-points
+// (sums over runs of one k-point round coherently in FP32); the rest keep
+// their order.
+__device__ inline size_t panel_row(const green_params &g, long pl)
+{
+    const unsigned b = (unsigned)(g.slot0 + pl), n = g.nrun_out;
+    return (n != 0u && b < (unsigned)TH_RUN * n) ? (size_t)((b % TH_RUN) * n + b / TH_RUN) : (size_t)b;
+}
 
 // In-place Gauss-Jordan inverse of the d x d row-major matrix M in LDS.
 // Row interchanges are recorded; the final column permutation is returned
@@ -180,7 +194,7 @@ __device__ void write_panel(const green_params &g, long pl, const cplx<double> *
 {
     const int tid = threadIdx.x;
     const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
-    P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+    P *out = reinterpret_cast<P *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
     for (int ij = 0; ij < g.nA2; ++ij) {
         const int i = ij / nA, j = ij - (ij / nA) * nA;
         P *o = out + (size_t)ij * 2 * Mpad;
@@ -341,7 +355,7 @@ __global__ __launch_bounds__(NT) void green_reg_kernel(green_params g)
         __syncthreads();
         const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
         cplx<double> *Sij = S, *Sji = S + Mb;          // blocks (i,j) and (j,i), row-major bs x bs
-        P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+        P *out = reinterpret_cast<P *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
         for (int ij = 0; ij < g.nA2; ++ij) {
             const int i = ij / nA, j = ij - (ij / nA) * nA;
 #pragma unroll
@@ -583,7 +597,7 @@ __global__ __launch_bounds__(TY *TX) void green_2d_kernel(green_params g)
     } else {
         const int bs = g.bs, Mb = bs * bs, Mpad = g.Mpad, nA = g.nA;
         cplx<double> *Sij = S, *Sji = S + Mb;
-        P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+        P *out = reinterpret_cast<P *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
         int drow[RI];
 #pragma unroll
         for (int i = 0; i < RI; ++i) drow[i] = (ty + i * TY < DD) ? qinv[ty + i * TY] : -1;
@@ -1180,7 +1194,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_chunk_ke
                 acci[j] = smfma<R>::mac(ai, b.x, acci[j]);
             }
         }
-        P *out = reinterpret_cast<P *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+        P *out = reinterpret_cast<P *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
@@ -1348,7 +1362,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
                 aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], brh, aci[j], 0, 0, 0);
             }
         const float back = ldexpf(1.0f, -es);
-        float *out = reinterpret_cast<float *>(g.panel) + (size_t)pl * g.nA2 * 2 * Mpad;
+        float *out = reinterpret_cast<float *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
@@ -1483,7 +1497,6 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
     double2 *M = reinterpret_cast<double2 *>(h->d_blk);
     double2 *G = M + CHUNK * dd;
     double2 *Ai = G + CHUNK * dd, *X = Ai + CHUNK * bb, *Y = X + CHUNK * bb, *S = Y + CHUNK * bb;
-    const size_t pstride = (size_t)h->nA2 * 2 * h->Mpad * (a.prec == TH_PREC_FP64 ? sizeof(double) : sizeof(float));
     const int nt = (b + 31) / 32;
     auto gemm = [&](long np, double alpha, const double2 *A_, long sa, int lda, const double2 *B_, long sb, int ldb,
                     double beta, const double2 *C_, long sc, int ldc, double2 *O_, long so, int ldo) {
@@ -1494,7 +1507,7 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
         const long np = std::min(CHUNK, a.np - c0);
         green_params g = g0;
         g.p0 = a.p0 + c0;
-        g.panel = (char *)a.panel + (size_t)c0 * pstride;
+        g.slot0 = c0;
         g.raw = a.raw ? a.raw + (size_t)c0 * dd : nullptr;
         hipLaunchKernelGGL(assemble_m_kernel, dim3((unsigned)np), dim3(256), sizeof(double2) * std::max(1, h->nhop),
                            gstream(h, a), g, M);
@@ -1619,6 +1632,8 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.kim = h->d_kim; g.yim = h->d_yim; g.wim = h->d_wim;
     g.kre = h->d_kre; g.wre = h->d_wre; g.x2 = h->d_x2; g.p2 = h->d_p2;
     g.panel = a.panel;
+    g.slot0 = 0;
+    g.nrun_out = (a.sum_order && h->pair_order) ? (unsigned)(a.np / TH_RUN) : 0u;
     g.raw = a.raw;
     g.status = th_status_word(h);
     // spectral Green functions when the eigenpairs of this side's k-points are ready

@@ -66,6 +66,7 @@ struct th_handle {
     void *d_bcache = nullptr;
     int bcache_prec = -1;
     double bcache_key[2] = {0, 0};   // (Ef, etap)
+    long bcache_bp = 0;              // stored in summation order per batch of this many pairs (0: pair order)
     // Dyson work space
     int *d_sup = nullptr;        // support of U
     double2 *d_U = nullptr, *d_W = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_X = nullptr;
@@ -81,8 +82,8 @@ struct th_handle {
     int hermitian = 0;           // H(k) is Hermitian for every k (checked at th_create)
     int spectral = 1;            // use them when possible
     int spectral_chunk = 1;      // spectral G for runs of pairs per k-point (V loaded once per run)
-    int contract_f16 = 1;
-    int green_f16 = 1;           // mixed: spectral products on fp16 MFMA with two-term splits        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
+    int contract_f16 = 1;        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
+    int green_f16 = 1;           // mixed: spectral products on fp16 MFMA with two-term splits
     int pair_order = 1;          // imaginary list sorted by k-point, real axis k-major (runs per k-point)
     long n_kuniq = 0;            // unique k-points of the imaginary list and the real mesh
     double *d_kuniq = nullptr;   // [n_kuniq][3]
@@ -91,6 +92,10 @@ struct th_handle {
     int eig_valid[2] = {0, 0};
     std::vector<int> kid_im_local;        // imaginary point -> index in kim_uniq
     std::vector<double> kim_uniq, kre_host;
+    std::vector<double> yim_host, wim_host;   // imaginary list in stored order
+    double wre_absmax = 0.0;     // max |w| of the real k-mesh
+    double im_bound_key[2] = {-1.0, -1.0};    // (eta, etap) of the cached bounds below
+    double im_bound[2] = {0.0, 0.0};          // max |w|/(y + eta), max 1/(y + etap) over the imaginary list
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // [omega == 0, omega != 0]
     int graph_prec[2] = {-1, -1};
     double *d_omega = nullptr;   // omega of the current replay
@@ -181,13 +186,46 @@ struct th_green_args {
     const double *omega_p = nullptr;   // optional: omega in device memory (graph replays)
     hipStream_t stream = nullptr;      // optional: launch stream (default: the handle's)
     int kmajor_re = 0, nx = 0;         // real axis: pairs k-major (p = ik * nx + ie) instead of node-major
+    // panel rows in the contraction's summation order (with pair_order: the
+    // runs of TH_RUN pairs per k-point stored transposed, see panel_row)
+    int sum_order = 0;
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
 int th_launch_eig(th_handle *h, int which, const double *q);
 
+// Row of global pair b in panels stored per batch of bpc pairs with each
+// batch's rows in the contraction's summation order (the runs of TH_RUN pairs
+// per k-point transposed, as the Green function kernels write them with
+// sum_order); bpc = 0: row b.  pg0 is added to batch-local pair indices.
+struct th_rowmap {
+    long pg0 = 0, bpc = 0, ntot = 0;
+};
+__host__ __device__ inline long th_map_row(long b, const th_rowmap &m)
+{
+    if (m.bpc <= 0) return b;
+    const long cb = b / m.bpc, l = b - cb * m.bpc;
+    const long nb = (m.ntot - cb * m.bpc) < m.bpc ? (m.ntot - cb * m.bpc) : m.bpc;
+    const long n = nb / TH_RUN;
+    return cb * m.bpc + (l < TH_RUN * n ? (l % TH_RUN) * n + l / TH_RUN : l);
+}
+
 // ---- contraction (th_gemm.hip) ----
+// Power-of-two scales of the two FP32 panels for the half-precision
+// contraction: every |entry| 2^e of a panel stays below 2^14.  On the real axis
+// the A exponent is taken on the device from areal max_x |p2[x]| (the node
+// weights of the current frequency, so that graph replays see new ones).
+struct th_contract_scale {
+    int eA = 0, eB = 0;
+    const double *p2 = nullptr;
+    int n_x = 0;
+    double areal = 0.0;
+};
+// Exponent e with bound 2^e < 2^14 (0 for a bound that is not positive and finite)
+int th_pow2_scale(double bound);
 // acc[axis] += sum over the batch of A (x) B, one complex GEMM per atom pair.
-int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis);
+// sc: the panels' scales (FP32 only); without them the FP32 kernel runs.
+int th_launch_contract(th_handle *h, const void *pA, const void *pB, long np, int prec, int axis,
+                       const th_contract_scale *sc = nullptr);
 // K-splits the contraction uses for a batch of np pairs: the tuned value,
 // raised in FP32 so that no partial sum runs over more than 8192 pairs
 int th_contract_splits(const th_handle *h, long np, int prec);
@@ -199,7 +237,7 @@ int th_launch_finalize(th_handle *h, double2 *chi0);
 // (trans = 1) or conj(panel[p]) . That (trans = 0) per atom pair, FP64 out
 // [np][nA2][2][Mpad]; That is [nA2][2][Mpad][Mpad] FP64.
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
-                            double *out);
+                            double *out, const th_rowmap &rows = th_rowmap());
 
 // ---- Dyson (th_dyson.hip) ----
 int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,
