@@ -12,6 +12,7 @@ synthetic systems (titan_amd.configs), resident on the device before the
 timed region starts.  Prints one JSON line.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -112,6 +113,13 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
     return out
 
 
+def _last_split(h):
+    """1 when the last chi0 of h had the Green function kernels write split fp16 panels."""
+    L = B.load()
+    L.thx_last_split.argtypes = [ctypes.c_void_p]
+    return int(L.thx_last_split(h._h))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +200,8 @@ def main():
         s0, s, cond, chi = wl.sweep(om_time, precision=prec, full=True)
         dt = time.perf_counter() - t0
         tm = wl.h.last_timing()
-        res[name] = {"dt": dt, "timing": tm, "schi": s, "schi0": s0, "chi": chi, "cond": cond}
+        res[name] = {"dt": dt, "timing": tm, "schi": s, "schi0": s0, "chi": chi, "cond": cond,
+                     "split": _last_split(wl.h)}
         log(name, "%.3f s, %.4g pairs/s" % (dt, P / dt), tm)
     mx = res["mixed"]
     out = {
@@ -224,13 +233,18 @@ def main():
     c3m = int(best.get("contract_3m", 0))
     if f16:
         # fp16 MFMA on two-term splits: 4 real products x 3 split products per complex entry
-        kernel, peak = "contract_h2_kernel (fp16 MFMA 16x16x32 on hi+lo splits of the FP32 panels)", PEAK["fp16"]
+        if mx["split"]:
+            kernel = "contract_h2p_kernel (fp16 MFMA 16x16x32 on hi+lo splits written by the Green function kernels)"
+        else:
+            kernel = "contract_h2_kernel (fp16 MFMA 16x16x32 on hi+lo splits of the FP32 panels)"
+        peak = PEAK["fp16"]
         per_pair = 2.0 * 12 * Mpad * Mpad
     else:
         kernel, peak = "contract_kernel<float> (FP32 MFMA 16x16x4)", PEAK["mixed"]
         per_pair = 2.0 * (3 if c3m else 4) * Mpad * Mpad
     executed = (n_imag + n_real) * m.nA ** 2 * per_pair
-    out["dtype"] = ("f32 panels, contraction on fp16 MFMA with two-term splits" if f16 else "f32")
+    out["dtype"] = (("fp16 hi+lo split panels" if mx["split"] else "f32 panels") +
+                    ", contraction on fp16 MFMA with two-term splits" if f16 else "f32")
     out["roofline"] = {"bound": "mfma", "achieved": alg / ctime / 1e12, "peak": peak, "unit": "TFLOP/s",
                        "frac": alg / ctime / 1e12 / peak, "traffic": None,
                        "kernel": kernel,

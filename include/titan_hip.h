@@ -232,10 +232,13 @@ int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_m
  *   Takes the path th_chi0 takes under the handle's tuning (spectral and
  *   pair_order: pairs in the stored order, real axis grouped when set).
  * thx_contract: uploads panels A, B of that layout, runs the contraction and
- *   returns the accumulator [nA*nA][2][Mpad][Mpad] (re block, im block). */
+ *   returns the accumulator [nA*nA][2][Mpad][Mpad] (re block, im block).
+ * thx_last_split: 1 when the last chi0 had the Green function kernels write
+ *   split half-precision panels, else 0. */
 int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, double omega, double Ef,
                     double eta, double etap, int prec, int n_x, const double *x2, const double *p2, double *out);
 int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out);
+int thx_last_split(th_handle *h);
 
 #ifdef __cplusplus
 }

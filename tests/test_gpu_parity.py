@@ -28,6 +28,7 @@ def _hooks():
     L.thx_green_panel.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_long, _dp, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
     L.thx_contract.argtypes = [C.c_void_p, _dp, _dp, C.c_long, C.c_int, _dp]
+    L.thx_last_split.argtypes = [C.c_void_p]
     return L
 
 
@@ -642,3 +643,34 @@ def test_mixed_error_with_full_batches(order):
     print("order", order, "mixed vs fp64 %.3e" % rel(gmx, g64), flush=True)
     assert rel(gmx, g64) < 5e-7
     h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("q,omega", [((0.0, 0.0, 0.0), 0.0), ((0.0, 0.0, 0.0), 0.00078125),
+                                     ((0.1, 0.0, 0.05), 0.00078125)])
+def test_split_panels_are_bitwise_the_fp32_panel_path(q, omega):
+    """Mixed chi0 on the 54 x 54 bcc supercell with the half-precision
+    spectral kernels writing the fp16 splits (copied as they are by the
+    contraction) equals, bit for bit, the path that writes FP32 panels and
+    splits them in the contraction: the same scaled values summed in the same
+    order.  Small batches (a ragged last one on both axes), q = 0 and q != 0,
+    the cached imaginary B panels on the second call."""
+    L = _hooks()
+    spec, run = CF.system(2), CF.run_params(2)
+    m = TBModel(spec)
+    kp, wk = Ct.uniform_kmesh(m.b, 6, 6, 6)
+    y, wy = Ct.imag_epoints(1, 12)
+    x2, p2 = Ct.real_epoints(m.Ef, 0.00078125, 1, 8)
+    out = {}
+    for split in (1, 0):
+        h = m.handle()
+        h.tune(batch_mb=16, presplit=split)
+        h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+        h.set_real_kmesh(kp, wk)
+        out[split] = [h.chi0(q, omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=B.PREC_MIXED)
+                      for _ in range(2)]
+        assert L.thx_last_split(h._h) == split
+        h.close()
+    for a, b in zip(out[1], out[0]):
+        print("q", q, "omega", omega, "max |split - fp32 panels| %.3e" % np.abs(a - b).max(), flush=True)
+        assert np.array_equal(a, b)

@@ -246,13 +246,13 @@ long batch_pairs(th_handle *h, long total)
 
 // imaginary-axis B panels G(k, Ef + i(y+etap)) do not depend on omega:
 // computed once per (Ef, etap, precision) and kept while they fit
-int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &tm, bool *have)
+int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit, stage_timer &tm, bool *have)
 {
     *have = false;
     // stored per batch of the imaginary contraction, rows in its summation order
     const long bp = batch_pairs(h, h->n_im), geo = h->pair_order ? bp : 0;
     if (h->d_bcache && h->bcache_prec == prec && h->bcache_key[0] == Ef && h->bcache_key[1] == etap &&
-        h->bcache_bp == geo) {
+        h->bcache_bp == geo && h->bcache_presplit == (presplit ? 1 : 0)) {
         *have = true;
         return TH_OK;
     }
@@ -278,6 +278,12 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
         a.etap = etap;
         a.prec = prec;
         a.sum_order = 1;
+        if (presplit) {   // |G| <= 1/(y + eta')
+            double bb = 0.0;
+            for (double y : h->yim_host) bb = std::max(bb, 1.0 / (y + etap));
+            a.presplit = 1;
+            a.pexp = th_pow2_scale(bb);
+        }
         a.panel = (char *)h->d_bcache + (size_t)p0 * h->nA2 * 2 * h->Mpad * prec_size(prec);
         int id = tm.begin(0);
         int rc = th_launch_green(h, a);
@@ -288,6 +294,7 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, stage_timer &t
     h->bcache_key[0] = Ef;
     h->bcache_key[1] = etap;
     h->bcache_bp = geo;
+    h->bcache_presplit = presplit ? 1 : 0;
     *have = true;
     return TH_OK;
 }
@@ -319,7 +326,11 @@ int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double 
         tm.end(id);
         if (rc) return rc;
     }
-    if ((rc = ensure_bcache(h, Ef, etap, prec, tm, have_b))) return rc;
+    // split half-precision panels from the Green function kernels (also for
+    // the cache) when both sides take the half-precision spectral kernel
+    const bool presplit = prec == TH_PREC_MIXED && h->contract_f16 && h->hermitian && etap > 0.0 &&
+                          std::isfinite(etap) && th_green_presplit_ok(h, q);
+    if ((rc = ensure_bcache(h, Ef, etap, prec, presplit, tm, have_b))) return rc;
     long need = batch_pairs(h, h->n_im);
     if (*real_axis) need = std::max(need, batch_pairs(h, (long)n_x * h->nk_re));
     if ((rc = th_ensure_part(h, th_contract_splits(h, need, prec)))) return rc;
@@ -387,6 +398,12 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
     long nb = 0;   // batches issued in this call
     th_contract_scale sc[2];
     const bool scaled = prec == TH_PREC_MIXED && contract_scales(h, eta, etap, n_x, &sc[0], &sc[1]);
+    // both sides' Green function kernels write fp16 splits (with the
+    // contraction's exponents); a cache of the other form is not used
+    const bool presplit = scaled && h->contract_f16 && th_green_presplit_ok(h, q);
+    sc[0].presplit = sc[1].presplit = presplit;
+    h->last_presplit = presplit ? 1 : 0;
+    if (have_b && h->bcache_presplit != (presplit ? 1 : 0)) have_b = false;
     auto begin = [&](int st, hipStream_t s) { return tm ? tm->begin(st, s) : -1; };
     auto end = [&](int id) { if (tm) tm->end(id); };
     auto batch = [&](int mode_a, int mode_b, long p0, long np, int axis) -> int {
@@ -403,6 +420,14 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
         }
         a.mode = mode_a;
         a.panel = pA[slot];
+        if (presplit) {
+            a.presplit = 1;
+            a.pexp = sc[axis].eA;
+            if (axis == 1) {   // exponent from the node weights in device memory
+                a.pareal = sc[1].areal;
+                a.pnx = n_x;
+            }
+        }
         int id = begin(0, gs);
         int r = th_launch_green(h, a);
         if (r) return r;
@@ -412,6 +437,8 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
         } else {
             a.mode = mode_b;
             a.panel = pB[slot];
+            a.pexp = sc[axis].eB;
+            a.pareal = 0.0;
             if ((r = th_launch_green(h, a))) return r;
             b = pB[slot];
         }
@@ -900,6 +927,9 @@ int th_tune(th_handle *h, const char *key, long value)
     } else if (k == "spectral_chunk") {
         if (value < 0 || value > 1) { th_set_error("th_tune: spectral_chunk is 0 or 1"); return TH_E_ARG; }
         h->spectral_chunk = (int)value;
+    } else if (k == "presplit") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: presplit is 0 or 1"); return TH_E_ARG; }
+        h->presplit = (int)value;
     } else if (k == "green_f16") {
         if (value < 0 || value > 1) { th_set_error("th_tune: green_f16 is 0 or 1"); return TH_E_ARG; }
         h->green_f16 = (int)value;
@@ -1001,6 +1031,8 @@ int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, d
     }
     return th_check_status(h);
 }
+
+int thx_last_split(th_handle *h) { return h ? h->last_presplit : 0; }
 
 int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out)
 {

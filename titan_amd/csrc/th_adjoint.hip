@@ -853,7 +853,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     tm.end(id);
     const size_t accsz = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
     const bool use_bcache = h->d_bcache && h->bcache_prec == precision && h->bcache_key[0] == Ef &&
-                            h->bcache_key[1] == etap;
+                            h->bcache_key[1] == etap && !h->bcache_presplit;
     const int ntile = (d + 31) / 32;
     // 3. pair batches of each axis
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {

@@ -250,13 +250,8 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
     const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, nrec, 0x00020000);
     const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(reinterpret_cast<const char *>(B) + (long)(c0 - r0) * 4), (short)0, nrec, 0x00020000);
-    if (p2) {   // real axis: |A| <= 2 max|p2| max|w_k| / eta = areal max|p2| (omega-dependent, from device memory)
-        double m = 0.0;
-        for (int x = 0; x < n_x; ++x) m = fmax(m, fabs(p2[x]));
-        const double bound = areal * m;
-        const int ex = (int)((__double_as_longlong(bound) >> 52) & 0x7ff) - 1022;
-        eA = bound > 0.0 ? min(100, max(-100, 14 - ex)) : 0;
-    }
+    // real axis: |A| <= 2 max|p2| max|w_k| / eta = areal max|p2| (omega-dependent, from device memory)
+    if (p2) eA = th_real_exp(p2, n_x, areal);
     const float scA = ldexpf(1.0f, eA), scB = ldexpf(1.0f, eB);
 
     // staging: thread tid holds column e = tid % 112 of the pairs 8 g .. 8 g + 7
@@ -329,6 +324,163 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
             const hv8 brl = *reinterpret_cast<const hv8 *>(&Ls[buf][1][0][1][bo]);
             const hv8 bih = *reinterpret_cast<const hv8 *>(&Ls[buf][1][1][0][bo]);
             const hv8 bil = *reinterpret_cast<const hv8 *>(&Ls[buf][1][1][1][bo]);
+            acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh, acp[j], 0, 0, 0);
+            acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih, acn[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih, aci[j], 0, 0, 0);
+            acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl, acp[j], 0, 0, 0);
+            acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil, acn[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bil, aci[j], 0, 0, 0);
+            acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh, acp[j], 0, 0, 0);
+            acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih, acn[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bih, aci[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh, aci[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl, aci[j], 0, 0, 0);
+            aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh, aci[j], 0, 0, 0);
+        }
+        __syncthreads();   // next buffer complete, this one free
+    }
+    const double back = ldexp(1.0, -(eA + eB));
+    double *pr_ = part + (((size_t)split * nA2 + ij) * 2 + 0) * Mpad * Mpad;
+    double *pi_ = pr_ + (size_t)Mpad * Mpad;
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = r0 + wave * 16 + 4 * (lane >> 4) + q;
+            const int col = c0 + j * 16 + (lane & 15);
+            pr_[(size_t)row * Mpad + col] = ((double)acp[j][q] - (double)acn[j][q]) * back;
+            pi_[(size_t)row * Mpad + col] = (double)aci[j][q] * back;
+        }
+}
+
+// The same contraction for panels that the Green function kernels already
+// split (rows of [nA2][re|im][Mpad] words holding the fp16 pair (hi, lo) of
+// x 2^e = hi + lo, bitwise the values contract_h2_kernel forms from the FP32
+// panels).  A stage arrives 16 bytes (4 entries) per load and is separated
+// into hi and lo with byte permutes: LDS holds [buffer][operand][re|im]
+// [hi|lo][32 pairs][112 entries] fp16 (114,688 B), and the MFMA operands,
+// which want 8 consecutive pairs of one entry per lane, come from transposing
+// reads (ds_read_b64_tr_b16: within each 16 lanes, lane i receives element
+// i & 3 of the 8-byte loads of lanes 4 j + (i >> 2), j = 0..3; lane t of a
+// group addresses pair 4 h + (t >> 2), entries 4 (t & 3) .. +3, so lane i
+// gets entry i of pairs 4 h .. 4 h + 3).  No conversion work remains in the
+// loop.
+typedef short sv4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+
+__device__ inline hv8 tr_frag(const _Float16 *base)
+{
+    // base: this lane's address for pairs 4 h + (t >> 2) of the first half;
+    // the second half is 4 pairs (4 x 112 entries) on
+    typedef __attribute__((address_space(3))) sv4 lsv4;
+    const sv4 a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lsv4 *)(base));
+    const sv4 b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lsv4 *)(base + 4 * TH_TILE));
+    const hv4 ha = __builtin_bit_cast(hv4, a), hb = __builtin_bit_cast(hv4, b);
+    return hv8{ha[0], ha[1], ha[2], ha[3], hb[0], hb[1], hb[2], hb[3]};
+}
+
+__global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__restrict__ A,
+                                                           const unsigned *__restrict__ B, long np, int Mpad,
+                                                           int nA2, int ntile, int splits, long kper, int eA, int eB,
+                                                           const double *__restrict__ p2, int n_x, double areal,
+                                                           double *__restrict__ part)
+{
+    // [buffer][operand][re|im][hi|lo][pair * 112 + entry]
+    __shared__ __align__(16) _Float16 Ls[2][2][2][2][HK * TH_TILE];
+    const int nwg = gridDim.x;
+    const int b = blockIdx.x;
+    const int n8 = nwg & ~7;
+    const int lid = (b < n8) ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
+    const int tile = lid % (ntile * ntile);
+    const int rest = lid / (ntile * ntile);
+    const int split = rest % splits;
+    const int ij = rest / splits;
+    const int tm = tile / ntile, tn = tile - tm * ntile;
+    const int r0 = tm * TH_TILE, c0 = tn * TH_TILE;
+    const unsigned kb = (unsigned)(split * kper);
+    const unsigned ke = (unsigned)min(np, (long)kb + kper);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // raw buffer loads over the split's rows (an offset at or past row ke
+    // reads 0); the B resource starts (c0 - r0) words off B so that one
+    // offset serves both operands.  Panel bytes + 96 rows < 2^31 (host check).
+    const unsigned ps2 = (unsigned)nA2 * 2u * (unsigned)Mpad * 4u;   // bytes per pair row
+    const int nrec = (int)(ke * ps2);
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, nrec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(reinterpret_cast<const char *>(B) + (long)(c0 - r0) * 4), (short)0, nrec, 0x00020000);
+    if (p2) eA = th_real_exp(p2, n_x, areal);
+
+    // staging: per operand a stage is 32 pairs x 2 components x 28 chunks of
+    // 16 bytes (4 entries); thread tid copies chunks u * 448 + tid, u < 4
+    unsigned goff[4], loff[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int q = u * NTH + tid;
+        const int pr = q / 56, rem = q - pr * 56, c = rem / 28, ch = rem - c * 28;
+        goff[u] = (unsigned)pr * ps2 + (((unsigned)ij * 2u + (unsigned)c) * (unsigned)Mpad + (unsigned)(r0 + 4 * ch)) * 4u;
+        loff[u] = (unsigned)c * (2 * HK * TH_TILE) + (unsigned)(pr * TH_TILE + 4 * ch);
+    }
+    uint4 sa[4], sb[4];
+    auto gload = [&](unsigned k0) {
+        const unsigned st = k0 * ps2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sa[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rA, (int)(goff[u] + st), 0, 0));
+            sb[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rB, (int)(goff[u] + st), 0, 0));
+        }
+    };
+    // words (hi | lo << 16) of 4 entries -> hi[4] and lo[4]
+    auto put = [&](_Float16 *hl, const uint4 v) {
+        const uint2 hi = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x05040100u), __builtin_amdgcn_perm(v.w, v.z, 0x05040100u));
+        const uint2 lo = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x07060302u), __builtin_amdgcn_perm(v.w, v.z, 0x07060302u));
+        *reinterpret_cast<uint2 *>(hl) = hi;
+        *reinterpret_cast<uint2 *>(hl + HK * TH_TILE) = lo;
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            put(&Ls[buf][0][0][0][loff[u]], sa[u]);
+            put(&Ls[buf][1][0][0][loff[u]], sb[u]);
+        }
+    };
+
+    fv4 acp[7], acn[7], aci[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+        for (int q = 0; q < 4; ++q) { acp[j][q] = 0.0f; acn[j][q] = 0.0f; aci[j][q] = 0.0f; }
+
+    // this lane's transposing-read offset within an array: pairs
+    // 8 (lane >> 4) + ((lane & 15) >> 2), entries 4 (lane & 3) of a 16-block
+    const int tro = (8 * (lane >> 4) + ((lane & 15) >> 2)) * TH_TILE + 4 * (lane & 3);
+    const unsigned nstage = (ke > kb) ? (ke - kb + HK - 1) / HK : 0u;
+    if (nstage > 0) {
+        gload(kb);
+        sstore(0);
+        gload(kb + HK);   // past the end: reads 0
+        __syncthreads();
+    }
+    for (unsigned s = 0; s < nstage; ++s) {
+        const int buf = (int)(s & 1u);
+        // the next stage's copy and the loads two stages ahead run
+        // unconditionally (one basic block with the MFMAs); past the last
+        // stage they read 0 and write the buffer that is no longer read
+        sstore(buf ^ 1);
+        gload(kb + (s + 2) * HK);
+        // keep the loads here, ahead of the MFMAs: they then have the whole
+        // stage to arrive before the next copy waits for them
+        __builtin_amdgcn_sched_barrier(0);
+        const int fo = tro + 16 * wave;
+        const hv8 arh = tr_frag(&Ls[buf][0][0][0][fo]);
+        const hv8 arl = tr_frag(&Ls[buf][0][0][1][fo]);
+        const hv8 aih = tr_frag(&Ls[buf][0][1][0][fo]);
+        const hv8 ail = tr_frag(&Ls[buf][0][1][1][fo]);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const int bo = tro + 16 * j;
+            const hv8 brh = tr_frag(&Ls[buf][1][0][0][bo]);
+            const hv8 brl = tr_frag(&Ls[buf][1][0][1][bo]);
+            const hv8 bih = tr_frag(&Ls[buf][1][1][0][bo]);
+            const hv8 bil = tr_frag(&Ls[buf][1][1][1][bo]);
             acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh, acp[j], 0, 0, 0);
             acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih, acn[j], 0, 0, 0);
             aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih, aci[j], 0, 0, 0);
@@ -592,8 +744,18 @@ int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int
     long kper = (np + splits - 1) / splits;
     kper = ((kper + TH_KC - 1) / TH_KC) * TH_KC;
     const int nwg = ntile * ntile * splits * h->nA2;
-    if (sizeof(P) == sizeof(float) && sc && h->contract_f16 &&
-        (unsigned long)(np + 96) * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 31)) {
+    const bool fits = (unsigned long)(np + 96) * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 31);
+    if (sc && sc->presplit) {
+        if (sizeof(P) != sizeof(float) || !fits) {
+            th_set_error("contraction: split half-precision panels need the mixed path and < 2 GiB per batch");
+            return TH_E_STATE;
+        }
+        long kh = (np + splits - 1) / splits;
+        kh = ((kh + HK - 1) / HK) * HK;
+        hipLaunchKernelGGL(contract_h2p_kernel, dim3(nwg), dim3(NTH), 0, h->stream, (const unsigned *)pA,
+                           (const unsigned *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, sc->eA, sc->eB, sc->p2,
+                           sc->n_x, sc->areal, h->d_part);
+    } else if (sizeof(P) == sizeof(float) && sc && h->contract_f16 && fits) {
         long kh = (np + splits - 1) / splits;
         kh = ((kh + HK - 1) / HK) * HK;
         hipLaunchKernelGGL(contract_h2_kernel, dim3(nwg), dim3(NTH), 0, h->stream, (const float *)pA,

@@ -48,6 +48,8 @@ struct green_params {
     void *panel;                 // panel rows of the batch
     long slot0;                  // batch index of this launch's first pair
     unsigned nrun_out;           // summation order: see panel_row
+    int presplit, pexp, pnx;     // fp16 split panels (th_green_args::presplit)
+    double pareal;
     double2 *raw;
     int *status;
     // spectral mode: eigenpairs of H(k) per k-point of the current sets and
@@ -1254,6 +1256,9 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
     }
     const int arow = 16 * wave + (lane & 15);   // row of V in this lane's A operand
     const int kofs = 8 * (lane >> 4);            // first of the lane's 8 eigen-indices per K step
+    // split panels: exponent of the panel (the real-axis A side from the
+    // current node weights)
+    const int eP = (g.presplit && g.pareal > 0.0) ? th_real_exp(g.p2, g.pnx, g.pareal) : g.pexp;
     long cur = -1;
     float avr[KT][8], avi[KT][8];
     for (long pl = s0; pl < s1; ++pl) {
@@ -1362,21 +1367,47 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
                 aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], brh, aci[j], 0, 0, 0);
             }
         const float back = ldexpf(1.0f, -es);
-        float *out = reinterpret_cast<float *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (rv[q] && cv[j]) {
-                    const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
-                    const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
-                    out[o] = (acp[j][q] - acn[j][q]) * back;
-                    out[o + Mpad] = aci[j][q] * back;
-                }
         const int npad = Mpad - Mb;
-        for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
-            const int part = e / npad;
-            out[(size_t)part * Mpad + Mb + (e - part * npad)] = 0.0f;
+        if (g.presplit) {
+            // x 2^eP = hi + lo as the contraction would split the FP32 value
+            // x = acc 2^-es (the power-of-two scalings are exact, x 2^eP - hi
+            // is exact in FP32); hi and lo share one 4-byte word
+            unsigned *out = reinterpret_cast<unsigned *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (rv[q] && cv[j]) {
+                        const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
+                        const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
+                        const float xr = ldexpf(acp[j][q] - acn[j][q], eP - es), xi = ldexpf(aci[j][q], eP - es);
+                        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+                        const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+                        out[o] = (unsigned)__builtin_bit_cast(unsigned short, hr) |
+                                 ((unsigned)__builtin_bit_cast(unsigned short, lr) << 16);
+                        out[o + Mpad] = (unsigned)__builtin_bit_cast(unsigned short, hi_) |
+                                        ((unsigned)__builtin_bit_cast(unsigned short, li) << 16);
+                    }
+            for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+                const int part = e / npad;
+                out[(size_t)part * Mpad + Mb + (e - part * npad)] = 0u;
+            }
+        } else {
+            float *out = reinterpret_cast<float *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (rv[q] && cv[j]) {
+                        const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
+                        const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
+                        out[o] = (acp[j][q] - acn[j][q]) * back;
+                        out[o + Mpad] = aci[j][q] * back;
+                    }
+            for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+                const int part = e / npad;
+                out[(size_t)part * Mpad + Mb + (e - part * npad)] = 0.0f;
+            }
         }
     }
 }
@@ -1536,6 +1567,10 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
 template <typename P, int DD>
 int launch_green_spectral(th_handle *h, const th_green_args &a, const green_params &g)
 {
+    if (a.presplit && !(sizeof(P) == sizeof(float) && !a.raw && h->spectral_chunk > 0 && h->green_f16)) {
+        th_set_error("split half-precision panels come from the half-precision spectral kernel only");
+        return TH_E_STATE;
+    }
     if (!a.raw && h->spectral_chunk > 0) {   // runs of pairs per k-point; G is not kept
         constexpr int CH = TH_RUN, NT = 64 * ((DD + 15) / 16);
         const unsigned nblk = (unsigned)((a.np + CH - 1) / CH);
@@ -1553,6 +1588,10 @@ int launch_green_spectral(th_handle *h, const th_green_args &a, const green_para
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
+    if (a.presplit && (h->kernel_variant != 0 || !g.eigV || (h->d != 18 && h->d != 36 && h->d != 54))) {
+        th_set_error("split half-precision panels come from the half-precision spectral kernel only");
+        return TH_E_STATE;
+    }
     if (h->kernel_variant == 1 && !a.raw) {   // one-dimensional ownership (kept for comparison)
         switch (h->d) {
         case 18: return launch_green_reg<P, 18, 64>(h, a, g);
@@ -1610,6 +1649,23 @@ int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g)
 
 int *th_status_word(th_handle *h);
 
+// Eigenpair set the spectral kernels use for one side (nullptr: inversion)
+static const th_eig_set *side_eig(const th_handle *h, bool qside, const double *q)
+{
+    const bool qzero = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
+    if (h->spectral && h->eig_valid[0] && (!qside || qzero)) return &h->eig[0];
+    if (h->spectral && qside && h->eig_valid[1] && h->eig[1].q[0] == q[0] && h->eig[1].q[1] == q[1] &&
+        h->eig[1].q[2] == q[2])
+        return &h->eig[1];
+    return nullptr;
+}
+
+bool th_green_presplit_ok(const th_handle *h, const double *q)
+{
+    return h->presplit && h->kernel_variant == 0 && h->spectral_chunk > 0 && h->green_f16 &&
+           (h->d == 18 || h->d == 36 || h->d == 54) && side_eig(h, true, q) && side_eig(h, false, q);
+}
+
 int th_launch_green(th_handle *h, const th_green_args &a)
 {
     if (h->d > TH_MAX_D_LDS && h->d != 72 && h->d != 144 && h->d != 288) {
@@ -1634,17 +1690,15 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.panel = a.panel;
     g.slot0 = 0;
     g.nrun_out = (a.sum_order && h->pair_order) ? (unsigned)(a.np / TH_RUN) : 0u;
+    g.presplit = a.presplit;
+    g.pexp = a.pexp;
+    g.pnx = a.pnx;
+    g.pareal = a.pareal;
     g.raw = a.raw;
     g.status = th_status_word(h);
     // spectral Green functions when the eigenpairs of this side's k-points are ready
     const bool qside = a.mode == TH_G_IMAG_A || a.mode == TH_G_REAL_A;
-    const bool qzero = a.q[0] == 0.0 && a.q[1] == 0.0 && a.q[2] == 0.0;
-    const th_eig_set *es = nullptr;
-    if (h->spectral && h->eig_valid[0] && (!qside || qzero))
-        es = &h->eig[0];
-    else if (h->spectral && qside && h->eig_valid[1] && h->eig[1].q[0] == a.q[0] && h->eig[1].q[1] == a.q[1] &&
-             h->eig[1].q[2] == a.q[2])
-        es = &h->eig[1];
+    const th_eig_set *es = side_eig(h, qside, a.q);
     g.eigV = es ? es->V : nullptr;
     g.eigL = es ? es->L : nullptr;
     g.kid_im = h->d_kid_im;

@@ -67,6 +67,7 @@ struct th_handle {
     int bcache_prec = -1;
     double bcache_key[2] = {0, 0};   // (Ef, etap)
     long bcache_bp = 0;              // stored in summation order per batch of this many pairs (0: pair order)
+    int bcache_presplit = 0;         // rows hold fp16 splits (th_green_args::presplit)
     // Dyson work space
     int *d_sup = nullptr;        // support of U
     double2 *d_U = nullptr, *d_W = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_X = nullptr;
@@ -83,6 +84,8 @@ struct th_handle {
     int spectral = 1;            // use them when possible
     int spectral_chunk = 1;      // spectral G for runs of pairs per k-point (V loaded once per run)
     int contract_f16 = 1;        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
+    int presplit = 1;            // mixed spectral: Green function kernels write the fp16 splits themselves
+    int last_presplit = 0;       // the last chi0 did
     int green_f16 = 1;           // mixed: spectral products on fp16 MFMA with two-term splits
     int pair_order = 1;          // imaginary list sorted by k-point, real axis k-major (runs per k-point)
     long n_kuniq = 0;            // unique k-points of the imaginary list and the real mesh
@@ -189,8 +192,18 @@ struct th_green_args {
     // panel rows in the contraction's summation order (with pair_order: the
     // runs of TH_RUN pairs per k-point stored transposed, see panel_row)
     int sum_order = 0;
+    // half-precision panels: rows of [nA2][re|im][Mpad] words, each the fp16
+    // pair (hi, lo) of the two-term split of x 2^pexp, hi in the low half (the
+    // half-precision spectral kernel only); on
+    // the real axis with pareal > 0 the exponent is taken on the device from
+    // pareal max |p2[0 .. pnx)| instead
+    int presplit = 0, pexp = 0, pnx = 0;
+    double pareal = 0.0;
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
+// The Green function kernels of both sides of chi0(q) can write split
+// half-precision panels (th_green_args::presplit)
+bool th_green_presplit_ok(const th_handle *h, const double *q);
 int th_launch_eig(th_handle *h, int which, const double *q);
 
 // Row of global pair b in panels stored per batch of bpc pairs with each
@@ -219,7 +232,19 @@ struct th_contract_scale {
     const double *p2 = nullptr;
     int n_x = 0;
     double areal = 0.0;
+    bool presplit = false;   // panels already split into fp16 hi + lo (th_green_args::presplit)
 };
+// Exponent of the real-axis A panel: bound = areal max |p2[x]|, e = 14 - E
+// with bound = m 2^E, m in [0.5, 1) (the producer and the contraction use
+// this one function so that both see the same e)
+__device__ inline int th_real_exp(const double *p2, int n_x, double areal)
+{
+    double m = 0.0;
+    for (int x = 0; x < n_x; ++x) m = fmax(m, fabs(p2[x]));
+    const double bound = areal * m;
+    const int ex = (int)((__double_as_longlong(bound) >> 52) & 0x7ff) - 1022;
+    return bound > 0.0 ? min(100, max(-100, 14 - ex)) : 0;
+}
 // Exponent e with bound 2^e < 2^14 (0 for a bound that is not positive and finite)
 int th_pow2_scale(double bound);
 // acc[axis] += sum over the batch of A (x) B, one complex GEMM per atom pair.
