@@ -358,23 +358,27 @@ __global__ __launch_bounds__(NTH) void contract_h2_kernel(const float *__restric
 // x 2^e = hi + lo, bitwise the values contract_h2_kernel forms from the FP32
 // panels).  A stage arrives 16 bytes (4 entries) per load and is separated
 // into hi and lo with byte permutes: LDS holds [buffer][operand][re|im]
-// [hi|lo][32 pairs][112 entries] fp16 (114,688 B), and the MFMA operands,
+// [hi|lo][32 pairs][128 entries] fp16 (131,072 B), and the MFMA operands,
 // which want 8 consecutive pairs of one entry per lane, come from transposing
 // reads (ds_read_b64_tr_b16: within each 16 lanes, lane i receives element
 // i & 3 of the 8-byte loads of lanes 4 j + (i >> 2), j = 0..3; lane t of a
 // group addresses pair 4 h + (t >> 2), entries 4 (t & 3) .. +3, so lane i
-// gets entry i of pairs 4 h .. 4 h + 3).  No conversion work remains in the
-// loop.
+// gets entry i of pairs 4 h .. 4 h + 3).  The 8-byte chunks of pair p's row
+// sit XOR-swizzled, chunk c at c ^ (4 (p & 3) + 16 ((p >> 3) & 1)), so that
+// the 32 lanes of two groups read 32 distinct chunk positions (no bank
+// conflicts with 32 or 64 banks).  No conversion work remains in the loop.
 typedef short sv4 __attribute__((ext_vector_type(4)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+
+constexpr int HROW = 128;   // LDS entries per pair row (112 used, 32 swizzled chunks of 4)
 
 __device__ inline hv8 tr_frag(const _Float16 *base)
 {
     // base: this lane's address for pairs 4 h + (t >> 2) of the first half;
-    // the second half is 4 pairs (4 x 112 entries) on
+    // the second half is 4 pairs on (same swizzle: p & 3 and p >> 3 agree)
     typedef __attribute__((address_space(3))) sv4 lsv4;
     const sv4 a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lsv4 *)(base));
-    const sv4 b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lsv4 *)(base + 4 * TH_TILE));
+    const sv4 b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lsv4 *)(base + 4 * HROW));
     const hv4 ha = __builtin_bit_cast(hv4, a), hb = __builtin_bit_cast(hv4, b);
     return hv8{ha[0], ha[1], ha[2], ha[3], hb[0], hb[1], hb[2], hb[3]};
 }
@@ -385,8 +389,8 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
                                                            const double *__restrict__ p2, int n_x, double areal,
                                                            double *__restrict__ part)
 {
-    // [buffer][operand][re|im][hi|lo][pair * 112 + entry]
-    __shared__ __align__(16) _Float16 Ls[2][2][2][2][HK * TH_TILE];
+    // [buffer][operand][re|im][hi|lo][pair * 128 + swizzled entry]
+    __shared__ __align__(16) _Float16 Ls[2][2][2][2][HK * HROW];
     const int nwg = gridDim.x;
     const int b = blockIdx.x;
     const int n8 = nwg & ~7;
@@ -400,6 +404,7 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
     const unsigned kb = (unsigned)(split * kper);
     const unsigned ke = (unsigned)min(np, (long)kb + kper);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    auto swz = [](int p) { return 4 * (p & 3) + 16 * ((p >> 3) & 1); };
     // raw buffer loads over the split's rows (an offset at or past row ke
     // reads 0); the B resource starts (c0 - r0) words off B so that one
     // offset serves both operands.  Panel bytes + 96 rows < 2^31 (host check).
@@ -418,7 +423,7 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
         const int q = u * NTH + tid;
         const int pr = q / 56, rem = q - pr * 56, c = rem / 28, ch = rem - c * 28;
         goff[u] = (unsigned)pr * ps2 + (((unsigned)ij * 2u + (unsigned)c) * (unsigned)Mpad + (unsigned)(r0 + 4 * ch)) * 4u;
-        loff[u] = (unsigned)c * (2 * HK * TH_TILE) + (unsigned)(pr * TH_TILE + 4 * ch);
+        loff[u] = (unsigned)c * (2 * HK * HROW) + (unsigned)(pr * HROW + 4 * (ch ^ swz(pr)));
     }
     uint4 sa[4], sb[4];
     auto gload = [&](unsigned k0) {
@@ -434,7 +439,7 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
         const uint2 hi = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x05040100u), __builtin_amdgcn_perm(v.w, v.z, 0x05040100u));
         const uint2 lo = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x07060302u), __builtin_amdgcn_perm(v.w, v.z, 0x07060302u));
         *reinterpret_cast<uint2 *>(hl) = hi;
-        *reinterpret_cast<uint2 *>(hl + HK * TH_TILE) = lo;
+        *reinterpret_cast<uint2 *>(hl + HK * HROW) = lo;
     };
     auto sstore = [&](int buf) {
 #pragma unroll
@@ -449,9 +454,13 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
     for (int j = 0; j < 7; ++j)
         for (int q = 0; q < 4; ++q) { acp[j][q] = 0.0f; acn[j][q] = 0.0f; aci[j][q] = 0.0f; }
 
-    // this lane's transposing-read offset within an array: pairs
-    // 8 (lane >> 4) + ((lane & 15) >> 2), entries 4 (lane & 3) of a 16-block
-    const int tro = (8 * (lane >> 4) + ((lane & 15) >> 2)) * TH_TILE + 4 * (lane & 3);
+    // this lane's transposing-read offsets within an array: pair
+    // 8 (lane >> 4) + ((lane & 15) >> 2), chunk 4 b + (lane & 3) of the
+    // 16-entry block b, swizzled: (4 b + k) ^ s = (4 b ^ s) + k (s has no
+    // bits below 2)
+    const int trp = (8 * (lane >> 4) + ((lane & 15) >> 2)) * HROW, trk = lane & 3;
+    const int trs = swz(8 * (lane >> 4) + ((lane & 15) >> 2));
+    auto tro = [&](int blk) { return trp + 4 * (((4 * blk) ^ trs) + trk); };
     const unsigned nstage = (ke > kb) ? (ke - kb + HK - 1) / HK : 0u;
     if (nstage > 0) {
         gload(kb);
@@ -469,14 +478,14 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
         // keep the loads here, ahead of the MFMAs: they then have the whole
         // stage to arrive before the next copy waits for them
         __builtin_amdgcn_sched_barrier(0);
-        const int fo = tro + 16 * wave;
+        const int fo = tro(wave);
         const hv8 arh = tr_frag(&Ls[buf][0][0][0][fo]);
         const hv8 arl = tr_frag(&Ls[buf][0][0][1][fo]);
         const hv8 aih = tr_frag(&Ls[buf][0][1][0][fo]);
         const hv8 ail = tr_frag(&Ls[buf][0][1][1][fo]);
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
-            const int bo = tro + 16 * j;
+            const int bo = tro(j);
             const hv8 brh = tr_frag(&Ls[buf][1][0][0][bo]);
             const hv8 brl = tr_frag(&Ls[buf][1][0][1][bo]);
             const hv8 bih = tr_frag(&Ls[buf][1][1][0][bo]);

@@ -26,6 +26,11 @@ template <typename R> __device__ inline cplx<R> cmul(cplx<R> a, cplx<R> b)
 template <typename R> __device__ inline cplx<R> csub(cplx<R> a, cplx<R> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename R> __device__ inline cplx<R> cadd(cplx<R> a, cplx<R> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename R> __device__ inline cplx<R> conj(cplx<R> a) { return {a.x, -a.y}; }
+// r x + z w (r real)
+__device__ inline cplx<double> rcfma(double r, cplx<double> x, cplx<double> z, cplx<double> w)
+{
+    return {fma(r, x.x, fma(z.x, w.x, -z.y * w.y)), fma(r, x.y, fma(z.x, w.y, z.y * w.x))};
+}
 template <typename R> __device__ inline cplx<R> cinv(cplx<R> a)
 {
     R n = a.x * a.x + a.y * a.y;
@@ -847,7 +852,7 @@ __global__ __launch_bounds__(NT) void panel_from_global_kernel(green_params g, c
 // (at most 30 sweeps).  Output V row-major (V[r][m] = component r of
 // eigenvector m) and lambda.
 template <int DD, int NT>
-__global__ __launch_bounds__(NT) void eig_jacobi_kernel(green_params g, const double *klist, double q0, double q1,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void eig_jacobi_kernel(green_params g, const double *klist, double q0, double q1,
                                                         double q2, double2 *Vout, double *Lout)
 {
     static_assert(DD % 2 == 0 && DD <= 64, "even DD <= 64 (one lane per column)");
@@ -859,7 +864,11 @@ __global__ __launch_bounds__(NT) void eig_jacobi_kernel(green_params g, const do
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<double> *ph = reinterpret_cast<cplx<double> *>(smem);
     __shared__ cplx<double> A[DD * DD];
-    __shared__ cplx<double> al[DD], be[DD];
+    // rotation of column m of the pair (p, q): J[p][p] = J[q][q] = c (real),
+    // J[q][p] = -conj(sigma), J[p][q] = sigma, so every column and row update is
+    // c x_m + coef_m x_partner(m) with coef_p = -conj(sigma), coef_q = sigma
+    __shared__ double rr[DD];
+    __shared__ cplx<double> cc[DD];
     __shared__ int pt[DD];
     __shared__ double red[2 * NW];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -902,60 +911,64 @@ __global__ __launch_bounds__(NT) void eig_jacobi_kernel(green_params g, const do
     for (int sweep = 0; sweep < 30; ++sweep) {
         for (int rnd = 0; rnd < NR; ++rnd) {
             if (tid < NP) {
+                // zero A[p][q] = b with dd = (A[q][q] - A[p][p]) / 2, r = sqrt(dd^2 + |b|^2),
+                // u = |dd| + r: c = u / sqrt(2 r u), sigma = sign(dd) b / sqrt(2 r u)
+                // (t = sign(dd) |b| / u is the smaller root of t^2 + (dd / |b|) 2 t - 1)
                 const int p = pp(tid, rnd), q = qq(tid, rnd);
-                const cplx<double> apq = A[p * DD + q];
-                const double app = A[p * DD + p].x, aqq = A[q * DD + q].x;
-                const double mag = sqrt(apq.x * apq.x + apq.y * apq.y);
-                double c = 1.0, sv = 0.0;
-                cplx<double> e = {1.0, 0.0};
-                if (mag > 0.0) {
-                    const double tau = (aqq - app) / (2.0 * mag);
-                    const double at = fabs(tau);
-                    const double t = (at > 1e150 ? 0.5 / at : 1.0 / (at + sqrt(1.0 + tau * tau))) * (tau >= 0.0 ? 1.0 : -1.0);
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    sv = t * c;
-                    e = {apq.x / mag, apq.y / mag};
+                const cplx<double> b = A[p * DD + q];
+                const double m2 = b.x * b.x + b.y * b.y;
+                const double dd = 0.5 * (A[q * DD + q].x - A[p * DD + p].x);
+                double c = 1.0;
+                cplx<double> sg = {0.0, 0.0};
+                if (m2 > 0.0) {
+                    const double r = sqrt(fma(dd, dd, m2)), u = fabs(dd) + r;
+                    const double kk = 1.0 / sqrt(2.0 * r * u), ks = dd >= 0.0 ? kk : -kk;
+                    c = u * kk;
+                    sg = {ks * b.x, ks * b.y};
                 }
-                al[p] = {c, 0.0};
-                be[p] = {-sv * e.x, sv * e.y};
+                rr[p] = c;
+                cc[p] = {-sg.x, sg.y};
                 pt[p] = q;
-                al[q] = {c * e.x, -c * e.y};
-                be[q] = {sv, 0.0};
+                rr[q] = c;
+                cc[q] = sg;
                 pt[q] = p;
             }
             __syncthreads();
             if (tid < NA) {
                 const int p2 = pp(j2, rnd), q2 = qq(j2, rnd);
-                const cplx<double> a2p = al[p2], b2p = be[p2], a2q = al[q2], b2q = be[q2];
+                const double c2 = rr[p2];
+                const cplx<double> B2 = cc[p2], A2 = cc[q2];
 #pragma unroll
                 for (int b = 0; b < BPT; ++b) {
                     const int j1 = jr0 + TPC * b;
                     if (j1 < NP) {
                         const int p1 = pp(j1, rnd), q1 = qq(j1, rnd);
-                        const cplx<double> a1p = conj(al[p1]), b1p = conj(be[p1]);
-                        const cplx<double> a1q = conj(al[q1]), b1q = conj(be[q1]);
+                        const double c1 = rr[p1];
+                        const cplx<double> B1 = conj(cc[p1]), A1 = conj(cc[q1]);
                         const cplx<double> x00 = A[p1 * DD + p2], x01 = A[p1 * DD + q2];
                         const cplx<double> x10 = A[q1 * DD + p2], x11 = A[q1 * DD + q2];
                         // rows p1, q1 of J^H A at columns p2, q2
-                        const cplx<double> r0p = cadd(cmul(a1p, x00), cmul(b1p, x10));
-                        const cplx<double> r0q = cadd(cmul(a1p, x01), cmul(b1p, x11));
-                        const cplx<double> r1p = cadd(cmul(a1q, x10), cmul(b1q, x00));
-                        const cplx<double> r1q = cadd(cmul(a1q, x11), cmul(b1q, x01));
+                        const cplx<double> r0p = rcfma(c1, x00, B1, x10);
+                        const cplx<double> r0q = rcfma(c1, x01, B1, x11);
+                        const cplx<double> r1p = rcfma(c1, x10, A1, x00);
+                        const cplx<double> r1q = rcfma(c1, x11, A1, x01);
                         // then columns p2, q2 of (J^H A) J
-                        A[p1 * DD + p2] = cadd(cmul(a2p, r0p), cmul(b2p, r0q));
-                        A[p1 * DD + q2] = cadd(cmul(a2q, r0q), cmul(b2q, r0p));
-                        A[q1 * DD + p2] = cadd(cmul(a2p, r1p), cmul(b2p, r1q));
-                        A[q1 * DD + q2] = cadd(cmul(a2q, r1q), cmul(b2q, r1p));
+                        A[p1 * DD + p2] = rcfma(c2, r0p, B2, r0q);
+                        A[p1 * DD + q2] = rcfma(c2, r0q, A2, r0p);
+                        A[q1 * DD + p2] = rcfma(c2, r1p, B2, r1q);
+                        A[q1 * DD + q2] = rcfma(c2, r1q, A2, r1p);
                     }
                 }
             }
             {
-                const cplx<double> am = al[vm], bm = be[vm];
+                // column m of V J: c v_m + coef_m v_partner(m)
+                const double rm = rr[vm];
+                const cplx<double> cm = cc[vm];
                 const int src = lane < DD ? pt[vm] : lane;
 #pragma unroll
                 for (int i = 0; i < RPW; ++i) {
                     const cplx<double> y = {__shfl(v[i].x, src, 64), __shfl(v[i].y, src, 64)};
-                    v[i] = cadd(cmul(am, v[i]), cmul(bm, y));
+                    v[i] = rcfma(rm, v[i], cm, y);
                 }
             }
             __syncthreads();
