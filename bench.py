@@ -115,7 +115,8 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
 
 def _last_split(h):
     """1 when the last chi0 of h had the Green function kernels write split fp16 panels."""
-    L = B.load()
+    from titan_amd import binding
+    L = binding.load()
     L.thx_last_split.argtypes = [ctypes.c_void_p]
     return int(L.thx_last_split(h._h))
 
