@@ -186,7 +186,9 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    th_chi_sweep, th_chi0 keeps the eigenpairs of earlier
  *                    calls at the same q) and G(k,z) = V diag(1/(z-lambda))
  *                    V^H is formed on the matrix cores for every z (FP64
- *                    MFMA; FP32 MFMA in TH_PREC_MIXED) (default); 0 = every
+ *                    MFMA; in TH_PREC_MIXED fp16 MFMA on two-term splits with
+ *                    "green_f16" = 1, the default, else FP32 MFMA)
+ *                    (default); 0 = every
  *                    G by Gauss-Jordan elimination of z-H
  *   "spectral_chunk" 1 = the spectral kernel takes runs of up to 16 pairs of
  *                    one k-point per workgroup, loads V once per run and
@@ -197,8 +199,10 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    block r/16 and the list is sorted by (block, k-point,
  *                    r); th_chi0 walks the real axis the same way (nodes in
  *                    blocks of 16, each block k-point by k-point), and the
- *                    contraction sums each batch with its runs transposed
- *                    (default); 0 = the given order.  Applies from the next
+ *                    Green function kernels store each batch's panel rows
+ *                    with its runs transposed, the order in which the
+ *                    contraction sums them (default); 0 = the given order.
+ *                    Applies from the next
  *                    th_set_imag_points; chi0 changes only in the rounding
  *                    of its sum over pairs
  *   "overlap"        1 = th_chi0 pipelines the batches on two streams: the
@@ -207,9 +211,18 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    spectral path is not taken); 0 = one stream
  *   "contract_f16"   TH_PREC_MIXED: 1 = the contraction runs on the fp16 matrix
  *                    cores, each FP32 panel value split into two fp16 parts
- *                    (hi + lo, 22 bits) under a power-of-two scale per
- *                    workgroup, three fp16 products per real product,
- *                    FP32 accumulation (default); 0 = FP32 MFMA
+ *                    (hi + lo, 22 bits) under one power-of-two scale per
+ *                    operand from the resolvent bound of a Hermitian H, three
+ *                    fp16 products per real product, FP32 accumulation
+ *                    (default; FP32 MFMA where H is not Hermitian); 0 = FP32
+ *                    MFMA
+ *   "presplit"       TH_PREC_MIXED with contract_f16, where both sides take the
+ *                    half-precision spectral kernel: 1 = that kernel writes
+ *                    the (hi, lo) fp16 words itself and the contraction copies
+ *                    them (default; the same values as 0); 0 = FP32 panels
+ *                    split in the contraction
+ *   "green_f16"      TH_PREC_MIXED spectral path: 1 = V diag(d) V^H on the fp16
+ *                    matrix cores with two-term splits (default); 0 = FP32 MFMA
  *   "contract_3m"    1 = the contraction forms each complex product from three
  *                    real MFMA products (Ar Br, Ai Bi, (Ar+Ai)(Br+Bi)) instead
  *                    of four; 0 = four (default)
