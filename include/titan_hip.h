@@ -156,6 +156,20 @@ int th_chi_traces(th_handle *h, const double *chi, const double *lvec, const dou
  * nOrb*(nu + nOrb*(k + 3*i))] (TITAN's Smunuiivec), k = x, y, z. */
 int th_spin_density(th_handle *h, double Ef, double eta, double *S_out);
 
+/* Parallel spin-current response to an electric field along ue[3] (replaces
+ * eintshe(q,e), eintshe.F90:2-224, with dtdksub, dtdksub.F90:2-28; callers
+ * calculate_all.F90:137, calculate_dc_limit.F90:134): over the handle's
+ * imaginary list and, for omega != 0 with n_x > 0, the real k-mesh x the nodes
+ * (x2, p2), Gq = G(k+q, z+omega) and G = G(k, z) as th_chi0 takes them,
+ * contracted with the velocity vertex dtdk = sum_n i (ue.R_n) e^{i k.R_n} t_n
+ * in the 16 spin-block terms of the reference, summed with the pair weights,
+ * multiplied by prefactor (D x D complex128 column-major, e.g.
+ * (I + chi_hf U)^-1; NULL = identity) and divided by 2 pi.
+ * out: D x 4 complex128 column-major (TITAN's tchiorbiikl).  FP64; d <= 54. */
+int th_current_response(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
+                        int n_x, const double *x2, const double *p2, const double *ue, const double *prefactor,
+                        double *out);
+
 int th_destroy(th_handle *h);
 
 /* Text of the last error on this thread (never NULL). */

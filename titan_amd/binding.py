@@ -20,7 +20,7 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
+           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_current_response", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
            "th_tune", "th_get_tuning")
 
 
@@ -48,6 +48,8 @@ def load():
                                  _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.th_chi_traces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.th_spin_density.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    L.th_current_response.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      _dp, _dp, _dp, C.c_void_p, _dp]
     L.th_destroy.argtypes = [C.c_void_p]
     L.th_last_error.restype = C.c_char_p
     L.th_limits.argtypes = [_ip, _ip, _ip]
@@ -190,6 +192,28 @@ class Handle:
         out = np.zeros(self.nOrb * self.nOrb * 3 * self.nA, dtype=np.complex128)
         _check(load().th_spin_density(self._h, Ef, eta, out.ctypes.data_as(_dp)), "th_spin_density")
         return out.reshape(self.nA, 3, self.nOrb, self.nOrb).transpose(0, 1, 3, 2).copy()
+
+    def current_response(self, q, omega, Ef, eta, etap, ue, x2=None, p2=None, prefactor=None):
+        """TITAN's tchiorbiikl of eintshe: D x 4 complex, the spin-current
+        response to a field along ue over the imaginary list and (omega != 0,
+        nodes x2/p2 given) the real k-mesh; prefactor D x D (None: identity)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        ue = np.ascontiguousarray(ue, dtype=np.float64)
+        if x2 is None:
+            x2 = np.zeros(1)
+            p2 = np.zeros(1)
+            nx = 0
+        else:
+            x2 = np.ascontiguousarray(x2, dtype=np.float64)
+            p2 = np.ascontiguousarray(p2, dtype=np.float64)
+            nx = len(x2)
+        pre = _c128(prefactor) if prefactor is not None else None
+        out = np.zeros(4 * self.D, dtype=np.complex128)
+        rc = load().th_current_response(self._h, _d(q), omega, Ef, eta, etap, nx, _d(x2), _d(p2), _d(ue),
+                                        pre.ctypes.data_as(C.c_void_p) if pre is not None else None,
+                                        out.ctypes.data_as(_dp))
+        _check(rc, "th_current_response")
+        return out.reshape(4, self.D).T.copy()
 
     def adjoint(self, q, omega, Ef, eta, etap, chi0, chi_hat, U=None, chi=None, x2=None, p2=None,
                 precision=PREC_FP64, want_U=True):

@@ -1,0 +1,293 @@
+// Parallel spin-current response to an electric field (eintshe,
+// eintshe.F90:2-224): per (k, E) pair the Green functions Gq = G(k + q, z + w)
+// and G = G(k, z) come from the Green function kernels' raw output, the
+// velocity vertex T = dH/dk . u (dtdksub.F90:14-26) is assembled from the hop
+// table, and the 16 spin-block terms of the reference are evaluated as
+//     df[(s, i, mu, nu), s'] = sum_{(j,g),(l,x)} P[(i,rn,nu),(j,cg,g)] T[(j,g),(l,x)]
+//                                                 Q[(l,rx,x),(i,cm,mu)]
+// with s -> (rn, cm), s' -> (cg, rx) in {(d,u), (u,u), (d,d), (u,d)}:
+// imaginary axis P = Gq, Q = G plus P = Gq^H, Q = G^H (the conjugated second
+// product of the reference); real axis P = -i (Gq - Gq^H), Q = G^H.  Per pair
+// the product P (1 (x) T) is formed once (d^3 / 2 complex MACs) and only the
+// diagonal atom blocks of its product with Q are taken.  Sums over pairs run
+// per chunk of pairs, then over chunks, then over batches, in a fixed order;
+// the RPA prefactor (a D x D matrix, constant over pairs) is applied once to
+// the sum.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "th_internal.h"
+
+namespace {
+
+constexpr int TH_CU_NT = 256;      // threads per workgroup
+constexpr int TH_CU_RO = 16;       // outputs per thread: D x 4 <= 16 x 256 for d <= 54
+constexpr int TH_CU_CHUNK = 64;    // pairs per workgroup
+
+__device__ inline double2 zmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+
+struct current_params {
+    int nA, nO, d, nhop;
+    long np, p0, nk_re;
+    int real;                        // real-axis pairs (node-major: p = ie * nk_re + ik)
+    double u0, u1, u2;               // field direction
+    const double *R;
+    const double2 *t;                // hops, column-major nO x nO blocks
+    const int *hoff;                 // hops of atom pair (j, l): hoff[j nA + l] ..
+    const double *kim, *wim, *kre, *wre, *p2;
+};
+
+// partial[chunk][o], o = X + D s' (X = 4 ((i nO + mu) nO + nu) + s)
+__global__ __launch_bounds__(TH_CU_NT) void current_pair_kernel(current_params c, const double2 *__restrict__ rawq,
+                                                                const double2 *__restrict__ raw,
+                                                                double2 *__restrict__ partial)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int d = c.d, n = d / 2, nO = c.nO, nA = c.nA;
+    double2 *P = reinterpret_cast<double2 *>(smem);   // Gq, d x d row-major
+    double2 *Q = P + d * d;                           // G
+    double2 *Z = Q + d * d;                           // P~ (1 (x) T): [row][cg n + (l nO + x)]
+    double2 *T = Z + d * d;                           // n x n
+    double2 *ph = T + n * n;                          // i (u.R) e^{i k.R} per hop
+    const int tid = threadIdx.x;
+    const int D = 4 * nA * nO * nO, nout = 4 * D;
+    const long b0 = (long)blockIdx.x * TH_CU_CHUNK;
+    const long b1 = b0 + TH_CU_CHUNK < c.np ? b0 + TH_CU_CHUNK : c.np;
+    double2 acc[TH_CU_RO];
+#pragma unroll
+    for (int r = 0; r < TH_CU_RO; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (long pl = b0; pl < b1; ++pl) {
+        const long p = c.p0 + pl;
+        double k0, k1, k2, w;
+        if (c.real) {
+            const long ie = p / c.nk_re, ik = p - (p / c.nk_re) * c.nk_re;
+            k0 = c.kre[3 * ik]; k1 = c.kre[3 * ik + 1]; k2 = c.kre[3 * ik + 2];
+            w = c.p2[ie] * c.wre[ik];
+        } else {
+            k0 = c.kim[3 * p]; k1 = c.kim[3 * p + 1]; k2 = c.kim[3 * p + 2];
+            w = c.wim[p];
+        }
+        const double2 *gq = rawq + (size_t)pl * d * d, *gg = raw + (size_t)pl * d * d;
+        for (int e = tid; e < d * d; e += TH_CU_NT) {
+            P[e] = gq[e];
+            Q[e] = gg[e];
+        }
+        for (int m = tid; m < c.nhop; m += TH_CU_NT) {
+            const double a = k0 * c.R[3 * m] + k1 * c.R[3 * m + 1] + k2 * c.R[3 * m + 2];
+            const double v = c.u0 * c.R[3 * m] + c.u1 * c.R[3 * m + 1] + c.u2 * c.R[3 * m + 2];
+            double sn, cs;
+            sincos(a, &sn, &cs);
+            ph[m] = make_double2(-v * sn, v * cs);   // i v (cos a + i sin a)
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += TH_CU_NT) {
+            const int rg = e / n, cx = e - (e / n) * n;
+            const int j = rg / nO, g = rg - j * nO, l = cx / nO, x = cx - l * nO;
+            double2 s = make_double2(0.0, 0.0);
+            for (int m = c.hoff[j * nA + l]; m < c.hoff[j * nA + l + 1]; ++m) {
+                const double2 tv = zmul(c.t[(size_t)m * nO * nO + g + nO * x], ph[m]);
+                s.x += tv.x;
+                s.y += tv.y;
+            }
+            T[e] = s;
+        }
+        __syncthreads();
+        const int nterm = c.real ? 1 : 2;
+        for (int term = 0; term < nterm; ++term) {
+            // Z[r][cg n + lx] = sum_{jg} P~[r][(2 j + cg) nO + g] T[jg][lx]
+            for (int e = tid; e < d * d; e += TH_CU_NT) {
+                const int r = e / d, c2 = e - (e / d) * d, cg = c2 / n, lx = c2 - cg * n;
+                double2 s = make_double2(0.0, 0.0);
+                for (int jg = 0; jg < n; ++jg) {
+                    const int j = jg / nO, g = jg - j * nO, col = (2 * j + cg) * nO + g;
+                    double2 pv;
+                    if (c.real) {   // -i (Gq - Gq^H)
+                        const double2 a = P[r * d + col], b = P[col * d + r];
+                        pv = make_double2(a.y + b.y, -(a.x - b.x));
+                    } else {
+                        pv = term == 0 ? P[r * d + col] : zconj(P[col * d + r]);
+                    }
+                    const double2 tv = T[jg * n + lx];
+                    s.x += pv.x * tv.x - pv.y * tv.y;
+                    s.y += pv.x * tv.y + pv.y * tv.x;
+                }
+                Z[e] = s;
+            }
+            __syncthreads();
+            const bool herm = c.real || term == 1;   // Q~ = G^H
+#pragma unroll
+            for (int rr = 0; rr < TH_CU_RO; ++rr) {
+                const int o = tid + TH_CU_NT * rr;
+                if (o < nout) {
+                    const int X = o % D, sp = o / D;
+                    const int s = X & 3, idx = X >> 2;
+                    const int nu = idx % nO, mu = (idx / nO) % nO, i = idx / (nO * nO);
+                    // s, s' -> (first spin, second spin): (d,u), (u,u), (d,d), (u,d)
+                    const int rn = (s == 0 || s == 2) ? 1 : 0, cm = (s >= 2) ? 1 : 0;
+                    const int cg = (sp == 0 || sp == 2) ? 1 : 0, rx = (sp >= 2) ? 1 : 0;
+                    const double2 *zr = Z + (size_t)((2 * i + rn) * nO + nu) * d + cg * n;
+                    const int qc = (2 * i + cm) * nO + mu;
+                    double2 v = make_double2(0.0, 0.0);
+                    for (int lx = 0; lx < n; ++lx) {
+                        const int l = lx / nO, x = lx - l * nO, qr = (2 * l + rx) * nO + x;
+                        const double2 qv = herm ? zconj(Q[qc * d + qr]) : Q[qr * d + qc];
+                        const double2 zv = zr[lx];
+                        v.x += zv.x * qv.x - zv.y * qv.y;
+                        v.y += zv.x * qv.y + zv.y * qv.x;
+                    }
+                    acc[rr].x += w * v.x;
+                    acc[rr].y += w * v.y;
+                }
+            }
+            __syncthreads();   // Z is rewritten by the next term or pair
+        }
+    }
+#pragma unroll
+    for (int rr = 0; rr < TH_CU_RO; ++rr) {
+        const int o = tid + TH_CU_NT * rr;
+        if (o < nout) partial[(size_t)blockIdx.x * nout + o] = acc[rr];
+    }
+}
+
+// acc[o] += sum over chunks of partial (fixed order)
+__global__ void current_sum_kernel(const double2 *partial, long nchunk, int nout, double2 *acc)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nout) return;
+    double2 s = acc[o];
+    for (long k = 0; k < nchunk; ++k) {
+        const double2 v = partial[(size_t)k * nout + o];
+        s.x += v.x;
+        s.y += v.y;
+    }
+    acc[o] = s;
+}
+
+// out[X + D s'] = sum_Y pre[X + D Y] acc[Y + D s'] / 2 pi (pre = NULL: identity)
+__global__ void current_prefactor_kernel(const double2 *pre, const double2 *acc, int D, double2 *out)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= 4 * D) return;
+    const int X = o % D, sp = o / D;
+    double2 s = make_double2(0.0, 0.0);
+    if (pre) {
+        for (int Y = 0; Y < D; ++Y) {
+            const double2 a = pre[(size_t)X + (size_t)D * Y], b = acc[(size_t)Y + (size_t)D * sp];
+            s.x += a.x * b.x - a.y * b.y;
+            s.y += a.x * b.y + a.y * b.x;
+        }
+    } else {
+        s = acc[o];
+    }
+    const double f = 0.15915494309189533577;   // 1 / (2 pi)
+    out[o] = make_double2(s.x * f, s.y * f);
+}
+
+}  // namespace
+
+extern "C" int th_current_response(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
+                                   int n_x, const double *x2, const double *p2, const double *ue,
+                                   const double *prefactor, double *out)
+{
+    if (!h || !q || !ue || !out || !(eta > 0.0) || !(etap > 0.0) || n_x < 0 || (n_x > 0 && (!x2 || !p2))) {
+        th_set_error("th_current_response: bad arguments");
+        return TH_E_ARG;
+    }
+    if (!h->d_kim || h->n_im <= 0) {
+        th_set_error("th_set_imag_points must be called before th_current_response");
+        return TH_E_STATE;
+    }
+    if (h->d > 54) {
+        th_set_error("th_current_response: d = " + std::to_string(h->d) + " > 54 is not supported");
+        return TH_E_UNSUPPORTED;
+    }
+    const bool real_axis = std::fabs(omega) >= 1e-15 && n_x > 0;
+    if (real_axis && (!h->d_kre || h->nk_re <= 0)) {
+        th_set_error("th_set_real_kmesh must be called before th_current_response at omega != 0");
+        return TH_E_STATE;
+    }
+    const int d = h->d, D = h->D, nout = 4 * D;
+    int rc = 0;
+    if (real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
+    // pairs per batch: the two raw G (d^2 complex each) within the panel budget
+    const size_t per = (size_t)2 * d * d * sizeof(double2);
+    const long n_re = real_axis ? (long)n_x * h->nk_re : 0;
+    const long bp = std::max<long>(TH_CU_CHUNK, std::min<long>(std::max(h->n_im, n_re),
+                                                               (long)(((size_t)h->batch_mb << 20) / per)));
+    const long nch_max = (bp + TH_CU_CHUNK - 1) / TH_CU_CHUNK;
+    double2 *d_raw = nullptr, *d_part = nullptr, *d_acc = nullptr, *d_pre = nullptr, *d_out = nullptr;
+    auto cleanup = [&]() { hipFree(d_raw); hipFree(d_part); hipFree(d_acc); hipFree(d_pre); hipFree(d_out); };
+    if (hipMalloc(&d_raw, per * bp) != hipSuccess || hipMalloc(&d_part, sizeof(double2) * nch_max * nout) != hipSuccess ||
+        hipMalloc(&d_acc, sizeof(double2) * nout) != hipSuccess || hipMalloc(&d_out, sizeof(double2) * nout) != hipSuccess ||
+        (prefactor && hipMalloc(&d_pre, sizeof(double2) * (size_t)D * D) != hipSuccess)) {
+        cleanup();
+        (void)hipGetLastError();
+        th_set_error("th_current_response: out of device memory");
+        return TH_E_HIP;
+    }
+    double2 *d_rawq = d_raw, *d_rawg = d_raw + (size_t)bp * d * d;
+    const int n = d / 2;
+    const size_t lds = sizeof(double2) * ((size_t)3 * d * d + (size_t)n * n + (size_t)std::max(1, h->nhop));
+    if (lds > 160 * 1024) {
+        cleanup();
+        th_set_error("th_current_response: workgroup memory exceeds the CU's LDS");
+        return TH_E_UNSUPPORTED;
+    }
+    if (hipFuncSetAttribute((const void *)current_pair_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess) {
+        cleanup();
+        (void)hipGetLastError();
+        th_set_error("th_current_response: workgroup memory attribute refused");
+        return TH_E_HIP;
+    }
+    if (!rc) rc = th_ensure_panels(h, bp);
+    if (!rc) rc = th_reset_status(h);
+    if (!rc) rc = th_launch_zero(h, reinterpret_cast<double *>(d_acc), (size_t)2 * nout);
+    if (!rc && prefactor &&
+        hipMemcpyAsync(d_pre, prefactor, sizeof(double2) * (size_t)D * D, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = TH_E_HIP;
+    current_params cp{};
+    cp.nA = h->nA; cp.nO = h->nOrb; cp.d = d; cp.nhop = h->nhop; cp.nk_re = h->nk_re;
+    cp.u0 = ue[0]; cp.u1 = ue[1]; cp.u2 = ue[2];
+    cp.R = h->d_R; cp.t = h->d_t; cp.hoff = h->d_hoff;
+    cp.kim = h->d_kim; cp.wim = h->d_wim; cp.kre = h->d_kre; cp.wre = h->d_wre; cp.p2 = h->d_p2;
+    for (int axis = 0; axis < (real_axis ? 2 : 1) && !rc; ++axis) {
+        const long total = axis == 0 ? h->n_im : n_re;
+        for (long p0 = 0; p0 < total && !rc; p0 += bp) {
+            const long np = std::min(bp, total - p0);
+            th_green_args a{};
+            a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
+            a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = TH_PREC_FP64;
+            a.mode = axis == 0 ? TH_G_IMAG_A : TH_G_REAL_A;
+            a.panel = h->d_pA;
+            a.raw = d_rawq;
+            if ((rc = th_launch_green(h, a))) break;
+            a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
+            a.panel = h->d_pB;
+            a.raw = d_rawg;
+            if ((rc = th_launch_green(h, a))) break;
+            cp.np = np; cp.p0 = p0; cp.real = axis;
+            const long nch = (np + TH_CU_CHUNK - 1) / TH_CU_CHUNK;
+            hipLaunchKernelGGL(current_pair_kernel, dim3((unsigned)nch), dim3(TH_CU_NT), lds, h->stream, cp,
+                               (const double2 *)d_rawq, (const double2 *)d_rawg, d_part);
+            hipLaunchKernelGGL(current_sum_kernel, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, h->stream,
+                               (const double2 *)d_part, nch, nout, d_acc);
+            if (hipGetLastError() != hipSuccess) { rc = TH_E_HIP; th_set_error("th_current_response: kernel launch"); }
+        }
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(current_prefactor_kernel, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, h->stream,
+                           (const double2 *)d_pre, (const double2 *)d_acc, D, d_out);
+        if (hipGetLastError() != hipSuccess) { rc = TH_E_HIP; th_set_error("th_current_response: kernel launch"); }
+    }
+    if (!rc && hipMemcpyAsync(out, d_out, sizeof(double2) * nout, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+        rc = TH_E_HIP;
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = TH_E_HIP;
+    if (!rc) rc = th_check_status(h);
+    cleanup();
+    return rc;
+}
