@@ -113,6 +113,56 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
     return out
 
 
+def current_response_cost(cfg=2, nk=16, omega=4e-3, reps=2):
+    """Throughput of th_current_response (the spin-current response of
+    eintshe) on the cfg system with an nk^3 k-mesh and the config's energies,
+    FP64, after a th_chi0 at the same q (eigenpairs in place), and the
+    oracle's restatement timed on a few pairs of the same workload."""
+    from oracle import current as OC
+    from oracle import oracle as O
+    from titan_amd import configs as CF
+    from titan_amd import contour as Ct
+    from titan_amd.system import TBModel
+    spec, run = CF.system(cfg), CF.run_params(cfg)
+    m = TBModel(spec)
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    kp, wk = Ct.uniform_kmesh(m.b, nk, nk, nk)
+    x2, p2 = Ct.real_epoints(m.Ef, omega, run["parts3"], run["n3gl"])
+    h = m.handle()
+    h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+    h.set_real_kmesh(kp, wk)
+    q, eta, etap = run["q"], run["eta"], run["etap"]
+    ue = np.array([1.0, 0.0, 0.0])
+    h.chi0(q, omega, m.Ef, eta, etap, x2, p2)
+    ts = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        h.current_response(q, omega, m.Ef, eta, etap, ue, x2, p2)
+        if r:
+            ts.append(time.perf_counter() - t0)
+    h.close()
+    n_im, n_re = len(y) * len(wk), len(x2) * len(wk)
+    d, nO, nA = m.d, m.nOrb, m.nA
+    n, D = d // 2, m.D
+    mac = lambda terms: terms * (d * d * n + 4 * D * n)   # complex MACs per pair of the device formulation
+    flops = 8.0 * (n_im * mac(2) + n_re * mac(1))
+    dt = min(ts)
+    # CPU: the oracle (numpy, one process) on a few pairs of the same workload
+    s = O.OracleSystem(spec)
+    rng = np.random.default_rng(3)
+    ks = kp[rng.choice(len(wk), 2, replace=False)]
+    t0 = time.perf_counter()
+    OC.current_response(s, q, omega, eta, etap, ue, (np.repeat(ks, 2, axis=0), np.tile(y[:2], 2), np.ones(4)),
+                        (ks, np.ones(2)), (x2[:1], p2[:1]))
+    tc = time.perf_counter() - t0
+    return {"workload": "cfg%d: %s, H %dx%d, k-mesh %dx%dx%d, %d imag + %d real energies, one omega, FP64" %
+                        (cfg, spec["name"], d, d, nk, nk, nk, len(y), len(x2)),
+            "pairs": n_im + n_re, "value": (n_im + n_re) / dt, "unit": "(k,E)-points/s", "ms": 1e3 * dt,
+            "alg_fp64_tflops": flops / dt / 1e12,
+            "cpu_baseline": {"value": 6.0 / tc, "unit": "(k,E)-points/s", "cores": 1, "kind": "port",
+                             "sample": "4 imaginary + 2 real pairs of the same workload by the oracle (numpy), %.1f s" % tc}}
+
+
 def _last_split(h):
     """1 when the last chi0 of h had the Green function kernels write split fp16 panels."""
     from titan_amd import binding
@@ -130,6 +180,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-current", action="store_true", help="skip the spin-current response measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default kernel shapes")
     ap.add_argument("--tuning", default=None,
                     help="JSON of th_tune settings to use instead of tuning (e.g. for profiler runs); "
@@ -283,6 +334,13 @@ def main():
         except Exception as e:   # reported, never required
             log("adjoint cost failed:", e)
             out["adjoint_cost"] = None
+    if not args.no_current:
+        try:
+            out["current_response"] = current_response_cost()
+            log("current response", out["current_response"])
+        except Exception as e:   # reported, never required
+            log("current response failed:", e)
+            out["current_response"] = None
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(spec, run, float(om_time[-1]))

@@ -10,7 +10,8 @@
 // product of the reference); real axis P = -i (Gq - Gq^H), Q = G^H.  Per pair
 // the product P (1 (x) T) is formed once (d^3 / 2 complex MACs) and only the
 // diagonal atom blocks of its product with Q are taken.  Sums over pairs run
-// per chunk of pairs, then over chunks, then over batches, in a fixed order;
+// per chunk of pairs (np / 1024 per workgroup), then over chunks, then over
+// batches, in a fixed order;
 // the RPA prefactor (a D x D matrix, constant over pairs) is applied once to
 // the sum.
 #include <hip/hip_runtime.h>
@@ -24,15 +25,43 @@
 namespace {
 
 constexpr int TH_CU_NT = 256;      // threads per workgroup
-constexpr int TH_CU_RO = 16;       // outputs per thread: D x 4 <= 16 x 256 for d <= 54
-constexpr int TH_CU_CHUNK = 64;    // pairs per workgroup
+constexpr int TH_CU_RT = 4;        // 2 x 2 output tiles per thread: D x 4 <= 4 x 4 x 256 for d <= 54
+constexpr int TH_CU_WG = 1024;     // workgroups per launch (4 per CU): pairs per workgroup = np / 1024
 
 __device__ inline double2 zmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline void zfma(double2 &s, double2 a, double2 b)
+{
+    s.x = fma(a.x, b.x, fma(-a.y, b.y, s.x));
+    s.y = fma(a.x, b.y, fma(a.y, b.x, s.y));
+}
+// P~[r][col]: the imaginary axis' P = Gq (term 0) or Gq^H (term 1), the real
+// axis' P = -i (Gq - Gq^H); Gq row-major d x d
+__device__ inline double2 ptil(const double2 *P, int d, int r, int col, int real, int term)
+{
+    if (real) {
+        const double2 a = P[r * d + col], b = P[col * d + r];
+        return make_double2(a.y + b.y, -(a.x - b.x));
+    }
+    return term == 0 ? P[r * d + col] : zconj(P[col * d + r]);
+}
+// output tile tt -> atom i, channel s', tile row ta, column tb (2 x 2 tiles of
+// the 2 nO x 2 nO block of atom i)
+__device__ inline void tile_of(int tt, int nO, int &i, int &sp, int &ta, int &tb)
+{
+    const int per = 4 * nO * nO;
+    i = tt / per;
+    const int rem = tt - i * per;
+    sp = rem / (nO * nO);
+    const int t2 = rem - sp * nO * nO;
+    ta = t2 / nO;
+    tb = t2 - ta * nO;
+}
 
 struct current_params {
     int nA, nO, d, nhop;
     long np, p0, nk_re;
+    long chunk;                      // pairs per workgroup
     int real;                        // real-axis pairs (node-major: p = ie * nk_re + ik)
     double u0, u1, u2;               // field direction
     const double *R;
@@ -54,12 +83,12 @@ __global__ __launch_bounds__(TH_CU_NT) void current_pair_kernel(current_params c
     double2 *T = Z + d * d;                           // n x n
     double2 *ph = T + n * n;                          // i (u.R) e^{i k.R} per hop
     const int tid = threadIdx.x;
-    const int D = 4 * nA * nO * nO, nout = 4 * D;
-    const long b0 = (long)blockIdx.x * TH_CU_CHUNK;
-    const long b1 = b0 + TH_CU_CHUNK < c.np ? b0 + TH_CU_CHUNK : c.np;
-    double2 acc[TH_CU_RO];
+    const int D = 4 * nA * nO * nO, nout = 4 * D, ntile = nout / 4;
+    const long b0 = (long)blockIdx.x * c.chunk;
+    const long b1 = b0 + c.chunk < c.np ? b0 + c.chunk : c.np;
+    double2 acc[4 * TH_CU_RT];
 #pragma unroll
-    for (int r = 0; r < TH_CU_RO; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (int r = 0; r < 4 * TH_CU_RT; ++r) acc[r] = make_double2(0.0, 0.0);
     for (long pl = b0; pl < b1; ++pl) {
         const long p = c.p0 + pl;
         double k0, k1, k2, w;
@@ -98,58 +127,76 @@ __global__ __launch_bounds__(TH_CU_NT) void current_pair_kernel(current_params c
         __syncthreads();
         const int nterm = c.real ? 1 : 2;
         for (int term = 0; term < nterm; ++term) {
-            // Z[r][cg n + lx] = sum_{jg} P~[r][(2 j + cg) nO + g] T[jg][lx]
-            for (int e = tid; e < d * d; e += TH_CU_NT) {
-                const int r = e / d, c2 = e - (e / d) * d, cg = c2 / n, lx = c2 - cg * n;
-                double2 s = make_double2(0.0, 0.0);
+            // Z[r][cg n + lx] = sum_{jg} P~[r][(2 j + cg) nO + g] T[jg][lx], in 2 x 2 tiles
+            for (int tz = tid; tz < (d / 2) * (d / 2); tz += TH_CU_NT) {
+                const int r0 = 2 * (tz / (d / 2)), cA = 2 * (tz - (tz / (d / 2)) * (d / 2)), cB = cA + 1;
+                const int gA = cA / n, lA = cA - gA * n, gB = cB / n, lB = cB - gB * n;
+                double2 z00 = make_double2(0.0, 0.0), z01 = z00, z10 = z00, z11 = z00;
                 for (int jg = 0; jg < n; ++jg) {
-                    const int j = jg / nO, g = jg - j * nO, col = (2 * j + cg) * nO + g;
-                    double2 pv;
-                    if (c.real) {   // -i (Gq - Gq^H)
-                        const double2 a = P[r * d + col], b = P[col * d + r];
-                        pv = make_double2(a.y + b.y, -(a.x - b.x));
-                    } else {
-                        pv = term == 0 ? P[r * d + col] : zconj(P[col * d + r]);
-                    }
-                    const double2 tv = T[jg * n + lx];
-                    s.x += pv.x * tv.x - pv.y * tv.y;
-                    s.y += pv.x * tv.y + pv.y * tv.x;
+                    const int j = jg / nO, g = jg - j * nO;
+                    const int colA = (2 * j + gA) * nO + g, colB = (2 * j + gB) * nO + g;
+                    const double2 p0A = ptil(P, d, r0, colA, c.real, term), p1A = ptil(P, d, r0 + 1, colA, c.real, term);
+                    const double2 p0B = gB == gA ? p0A : ptil(P, d, r0, colB, c.real, term);
+                    const double2 p1B = gB == gA ? p1A : ptil(P, d, r0 + 1, colB, c.real, term);
+                    const double2 tA = T[jg * n + lA], tB = T[jg * n + lB];
+                    zfma(z00, p0A, tA);
+                    zfma(z01, p0B, tB);
+                    zfma(z10, p1A, tA);
+                    zfma(z11, p1B, tB);
                 }
-                Z[e] = s;
+                Z[r0 * d + cA] = z00;
+                Z[r0 * d + cB] = z01;
+                Z[(r0 + 1) * d + cA] = z10;
+                Z[(r0 + 1) * d + cB] = z11;
             }
             __syncthreads();
             const bool herm = c.real || term == 1;   // Q~ = G^H
+            // per atom i and channel s' = (cg, rx): O[a][b] = sum_lx Z[2 i nO + a][cg n + lx]
+            // Q~[(2 l + rx) nO + x][2 i nO + b], a = (rn, nu), b = (cm, mu), in 2 x 2 tiles
 #pragma unroll
-            for (int rr = 0; rr < TH_CU_RO; ++rr) {
-                const int o = tid + TH_CU_NT * rr;
-                if (o < nout) {
-                    const int X = o % D, sp = o / D;
-                    const int s = X & 3, idx = X >> 2;
-                    const int nu = idx % nO, mu = (idx / nO) % nO, i = idx / (nO * nO);
-                    // s, s' -> (first spin, second spin): (d,u), (u,u), (d,d), (u,d)
-                    const int rn = (s == 0 || s == 2) ? 1 : 0, cm = (s >= 2) ? 1 : 0;
-                    const int cg = (sp == 0 || sp == 2) ? 1 : 0, rx = (sp >= 2) ? 1 : 0;
-                    const double2 *zr = Z + (size_t)((2 * i + rn) * nO + nu) * d + cg * n;
-                    const int qc = (2 * i + cm) * nO + mu;
-                    double2 v = make_double2(0.0, 0.0);
+            for (int rr = 0; rr < TH_CU_RT; ++rr) {
+                const int tt = tid + TH_CU_NT * rr;
+                if (tt < ntile) {
+                    int i, sp, ta, tb;
+                    tile_of(tt, nO, i, sp, ta, tb);
+                    const int cg = 1 - (sp & 1), rx = sp >> 1;
+                    const int a0 = 2 * i * nO + 2 * ta, b0 = 2 * i * nO + 2 * tb;
+                    const double2 *z0 = Z + (size_t)a0 * d + cg * n, *z1 = z0 + d;
+                    double2 o00 = make_double2(0.0, 0.0), o01 = o00, o10 = o00, o11 = o00;
                     for (int lx = 0; lx < n; ++lx) {
                         const int l = lx / nO, x = lx - l * nO, qr = (2 * l + rx) * nO + x;
-                        const double2 qv = herm ? zconj(Q[qc * d + qr]) : Q[qr * d + qc];
-                        const double2 zv = zr[lx];
-                        v.x += zv.x * qv.x - zv.y * qv.y;
-                        v.y += zv.x * qv.y + zv.y * qv.x;
+                        const double2 q0 = herm ? zconj(Q[b0 * d + qr]) : Q[qr * d + b0];
+                        const double2 q1 = herm ? zconj(Q[(b0 + 1) * d + qr]) : Q[qr * d + b0 + 1];
+                        const double2 za = z0[lx], zb = z1[lx];
+                        zfma(o00, za, q0);
+                        zfma(o01, za, q1);
+                        zfma(o10, zb, q0);
+                        zfma(o11, zb, q1);
                     }
-                    acc[rr].x += w * v.x;
-                    acc[rr].y += w * v.y;
+                    acc[4 * rr].x += w * o00.x; acc[4 * rr].y += w * o00.y;
+                    acc[4 * rr + 1].x += w * o01.x; acc[4 * rr + 1].y += w * o01.y;
+                    acc[4 * rr + 2].x += w * o10.x; acc[4 * rr + 2].y += w * o10.y;
+                    acc[4 * rr + 3].x += w * o11.x; acc[4 * rr + 3].y += w * o11.y;
                 }
             }
             __syncthreads();   // Z is rewritten by the next term or pair
         }
     }
+    // tile entry (a, b) -> X = 4 ((i nO + mu) nO + nu) + s, s = 2 cm + 1 - rn; o = X + D s'
 #pragma unroll
-    for (int rr = 0; rr < TH_CU_RO; ++rr) {
-        const int o = tid + TH_CU_NT * rr;
-        if (o < nout) partial[(size_t)blockIdx.x * nout + o] = acc[rr];
+    for (int rr = 0; rr < TH_CU_RT; ++rr) {
+        const int tt = tid + TH_CU_NT * rr;
+        if (tt < ntile) {
+            int i, sp, ta, tb;
+            tile_of(tt, nO, i, sp, ta, tb);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int a = 2 * ta + (e >> 1), b = 2 * tb + (e & 1);
+                const int rn = a / nO, nu = a - rn * nO, cm = b / nO, mu = b - cm * nO;
+                const int X = 4 * ((i * nO + mu) * nO + nu) + 2 * cm + 1 - rn;
+                partial[(size_t)blockIdx.x * nout + X + (size_t)D * sp] = acc[4 * rr + e];
+            }
+        }
     }
 }
 
@@ -216,9 +263,8 @@ extern "C" int th_current_response(th_handle *h, const double *q, double omega, 
     // pairs per batch: the two raw G (d^2 complex each) within the panel budget
     const size_t per = (size_t)2 * d * d * sizeof(double2);
     const long n_re = real_axis ? (long)n_x * h->nk_re : 0;
-    const long bp = std::max<long>(TH_CU_CHUNK, std::min<long>(std::max(h->n_im, n_re),
-                                                               (long)(((size_t)h->batch_mb << 20) / per)));
-    const long nch_max = (bp + TH_CU_CHUNK - 1) / TH_CU_CHUNK;
+    const long bp = std::max<long>(1, std::min<long>(std::max(h->n_im, n_re), (long)(((size_t)h->batch_mb << 20) / per)));
+    const long nch_max = std::min<long>(bp, TH_CU_WG);
     double2 *d_raw = nullptr, *d_part = nullptr, *d_acc = nullptr, *d_pre = nullptr, *d_out = nullptr;
     auto cleanup = [&]() { hipFree(d_raw); hipFree(d_part); hipFree(d_acc); hipFree(d_pre); hipFree(d_out); };
     if (hipMalloc(&d_raw, per * bp) != hipSuccess || hipMalloc(&d_part, sizeof(double2) * nch_max * nout) != hipSuccess ||
@@ -271,7 +317,8 @@ extern "C" int th_current_response(th_handle *h, const double *q, double omega, 
             a.raw = d_rawg;
             if ((rc = th_launch_green(h, a))) break;
             cp.np = np; cp.p0 = p0; cp.real = axis;
-            const long nch = (np + TH_CU_CHUNK - 1) / TH_CU_CHUNK;
+            cp.chunk = (np + TH_CU_WG - 1) / TH_CU_WG;
+            const long nch = (np + cp.chunk - 1) / cp.chunk;
             hipLaunchKernelGGL(current_pair_kernel, dim3((unsigned)nch), dim3(TH_CU_NT), lds, h->stream, cp,
                                (const double2 *)d_rawq, (const double2 *)d_rawg, d_part);
             hipLaunchKernelGGL(current_sum_kernel, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, h->stream,

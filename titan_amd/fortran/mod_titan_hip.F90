@@ -74,6 +74,19 @@ module mod_titan_hip
       real(c_double), value :: Ef, eta
       complex(c_double_complex), intent(out) :: S(*)
     end function
+    ! spin-current response to a field along ue (eintshe, eintshe.F90:2-224);
+    ! prefactor: c_null_ptr for the identity, else c_loc of a D x D array
+    integer(c_int) function th_current_response(h, q, omega, Ef, eta, etap, n_x, x2, p2, ue, prefactor, tchi) &
+        bind(C, name="th_current_response")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: q(3)
+      real(c_double), value :: omega, Ef, eta, etap
+      integer(c_int), value :: n_x
+      real(c_double), intent(in) :: x2(*), p2(*), ue(3)
+      type(c_ptr), value :: prefactor
+      complex(c_double_complex), intent(out) :: tchi(*)
+    end function
     integer(c_int) function th_destroy(h) bind(C, name="th_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: h
