@@ -156,6 +156,16 @@ int th_chi_traces(th_handle *h, const double *chi, const double *lvec, const dou
  * nOrb*(nu + nOrb*(k + 3*i))] (TITAN's Smunuiivec), k = x, y, z. */
 int th_spin_density(th_handle *h, double Ef, double eta, double *S_out);
 
+/* Ground-state occupations and magnetisation per orbital by the same sweep
+ * (replaces expectation_values_greenfunction, mod_expectation.F90:28-149,
+ * without superconductivity): rho = n_up + n_down, mz = n_up - n_down with
+ * n_s = 1/2 + sum_p w_p Re G[(i,s,mu),(i,s,mu)] / pi, mp = mx + i my =
+ * sum_p w_p (G[(i,d,mu),(i,u,mu)] + conj G[(i,u,mu),(i,d,mu)]) / pi over
+ * z = E_F + i(y_p + eta).  rho, mx, my, mz: nOrb x nA doubles, mp: nOrb x nA
+ * complex128, element [mu + nOrb*i] (TITAN's rho(mu,i) etc.). */
+int th_expectation_values(th_handle *h, double Ef, double eta, double *rho, double *mx, double *my, double *mz,
+                          double *mp);
+
 /* Parallel spin-current response to an electric field along ue[3] (replaces
  * eintshe(q,e), eintshe.F90:2-224, with dtdksub, dtdksub.F90:2-28; callers
  * calculate_all.F90:137, calculate_dc_limit.F90:134): over the handle's

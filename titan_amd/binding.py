@@ -20,7 +20,7 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_current_response", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
+           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_expectation_values", "th_current_response", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
            "th_tune", "th_get_tuning")
 
 
@@ -48,6 +48,7 @@ def load():
                                  _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.th_chi_traces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.th_spin_density.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    L.th_expectation_values.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
     L.th_current_response.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                       _dp, _dp, _dp, C.c_void_p, _dp]
     L.th_destroy.argtypes = [C.c_void_p]
@@ -192,6 +193,16 @@ class Handle:
         out = np.zeros(self.nOrb * self.nOrb * 3 * self.nA, dtype=np.complex128)
         _check(load().th_spin_density(self._h, Ef, eta, out.ctypes.data_as(_dp)), "th_spin_density")
         return out.reshape(self.nA, 3, self.nOrb, self.nOrb).transpose(0, 1, 3, 2).copy()
+
+    def expectation_values(self, Ef, eta):
+        """(rho, mx, my, mz, mp) of the imaginary list, each [nA, nOrb] (mp
+        complex): TITAN's expectation_values_greenfunction."""
+        nA, nO = self.nA, self.nOrb
+        rho, mx, my, mz = (np.zeros(nO * nA) for _ in range(4))
+        mp = np.zeros(nO * nA, dtype=np.complex128)
+        _check(load().th_expectation_values(self._h, Ef, eta, _d(rho), _d(mx), _d(my), _d(mz),
+                                            mp.ctypes.data_as(_dp)), "th_expectation_values")
+        return tuple(a.reshape(nA, nO) for a in (rho, mx, my, mz, mp))
 
     def current_response(self, q, omega, Ef, eta, etap, ue, x2=None, p2=None, prefactor=None):
         """TITAN's tchiorbiikl of eintshe: D x 4 complex, the spin-current

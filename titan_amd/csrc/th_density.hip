@@ -56,12 +56,11 @@ __global__ void density_sum_kernel(const double2 *partial, long nchunk, int nout
     acc[o] = s;
 }
 
-}  // namespace
-
-extern "C" int th_spin_density(th_handle *h, double Ef, double eta, double *S_out)
+// acc[((i nO + mu) nO + nu) 4 + blk] = sum_p w_p (G[nu s1, mu s2] + conj(G[mu s2, nu s1]))
+// over the imaginary list at z = E_F + i(y_p + eta), blk = uu, dd, ud, du
+int density_blocks(th_handle *h, double Ef, double eta, std::vector<double2> &acc)
 {
-    if (!h || !S_out || !(eta >= 0.0)) { th_set_error("th_spin_density: bad arguments"); return TH_E_ARG; }
-    if (!h->d_kim || h->n_im <= 0) { th_set_error("th_set_imag_points must be called before th_spin_density"); return TH_E_STATE; }
+    if (!h->d_kim || h->n_im <= 0) { th_set_error("th_set_imag_points must be called before the ground-state sweep"); return TH_E_STATE; }
     const int nA = h->nA, nO = h->nOrb, d = h->d, nout = nA * nO * nO * 4;
     // pairs per batch: the raw G (d^2 complex per pair) within the panel budget
     const size_t per = (size_t)d * d * sizeof(double2);
@@ -99,13 +98,24 @@ extern "C" int th_spin_density(th_handle *h, double Ef, double eta, double *S_ou
                            (const double2 *)d_part, nch, nout, d_acc);
         if (hipGetLastError() != hipSuccess) { rc = TH_E_HIP; th_set_error("th_spin_density: kernel launch"); }
     }
-    std::vector<double2> acc(nout);
+    acc.assign(nout, make_double2(0.0, 0.0));
     if (!rc && hipMemcpyAsync(acc.data(), d_acc, sizeof(double2) * nout, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
         rc = TH_E_HIP;
     if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = TH_E_HIP;
     if (!rc) rc = th_check_status(h);
     cleanup();
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int th_spin_density(th_handle *h, double Ef, double eta, double *S_out)
+{
+    if (!h || !S_out || !(eta >= 0.0)) { th_set_error("th_spin_density: bad arguments"); return TH_E_ARG; }
+    std::vector<double2> acc;
+    const int rc = density_blocks(h, Ef, eta, acc);
     if (rc) return rc;
+    const int nA = h->nA, nO = h->nOrb;
     // / 2 pi, + 1/2 on the diagonal of uu and dd (mod_sumrule.F90:224-239), then
     // S_x = ud + du, S_y = i (ud - du), S_z = uu - dd
     const double inv2pi = 1.0 / (2.0 * std::acos(-1.0));
@@ -127,5 +137,36 @@ extern "C" int th_spin_density(th_handle *h, double Ef, double eta, double *S_ou
                     S_out[2 * o + 1] = sv[k].y;
                 }
             }
+    return TH_OK;
+}
+
+extern "C" int th_expectation_values(th_handle *h, double Ef, double eta, double *rho, double *mx, double *my,
+                                     double *mz, double *mp)
+{
+    if (!h || !rho || !mx || !my || !mz || !mp || !(eta >= 0.0)) {
+        th_set_error("th_expectation_values: bad arguments");
+        return TH_E_ARG;
+    }
+    std::vector<double2> acc;
+    const int rc = density_blocks(h, Ef, eta, acc);
+    if (rc) return rc;
+    // the diagonal mu = nu of the blocks (expectation_values_greenfunction,
+    // mod_expectation.F90:101-142): uu and dd hold 2 w Re G, du holds
+    // w (G[d,u] + conj G[u,d]); / pi, + 1/2 on uu and dd
+    const int nA = h->nA, nO = h->nOrb;
+    const double inv2pi = 1.0 / (2.0 * std::acos(-1.0));
+    for (int i = 0; i < nA; ++i)
+        for (int mu = 0; mu < nO; ++mu) {
+            const int r = ((i * nO + mu) * nO + mu) * 4;
+            const double uu = 0.5 + acc[r].x * inv2pi, dd = 0.5 + acc[r + 1].x * inv2pi;
+            const double2 du = acc[r + 3];
+            const size_t o = (size_t)mu + (size_t)nO * i;
+            rho[o] = uu + dd;
+            mz[o] = uu - dd;
+            mx[o] = 2.0 * du.x * inv2pi;
+            my[o] = 2.0 * du.y * inv2pi;
+            mp[2 * o] = mx[o];
+            mp[2 * o + 1] = my[o];
+        }
     return TH_OK;
 }

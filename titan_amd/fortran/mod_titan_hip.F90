@@ -74,6 +74,16 @@ module mod_titan_hip
       real(c_double), value :: Ef, eta
       complex(c_double_complex), intent(out) :: S(*)
     end function
+    ! occupations and magnetisation per orbital (expectation_values_greenfunction,
+    ! mod_expectation.F90:28-149)
+    integer(c_int) function th_expectation_values(h, Ef, eta, rho, mx, my, mz, mp) &
+        bind(C, name="th_expectation_values")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), value :: Ef, eta
+      real(c_double), intent(out) :: rho(*), mx(*), my(*), mz(*)
+      complex(c_double_complex), intent(out) :: mp(*)
+    end function
     ! spin-current response to a field along ue (eintshe, eintshe.F90:2-224);
     ! prefactor: c_null_ptr for the identity, else c_loc of a D x D array
     integer(c_int) function th_current_response(h, q, omega, Ef, eta, etap, n_x, x2, p2, ue, prefactor, tchi) &
