@@ -6,6 +6,8 @@ oracle on identical inputs; fixture comparisons at the fixture's own print
 precision (10 significant digits)."""
 import ctypes as C
 
+import os
+
 import numpy as np
 import pytest
 
@@ -320,7 +322,11 @@ def test_repeat_calls_are_bitwise_identical(cfg0):
 # the reference's own fixture, computed on the GPU
 # --------------------------------------------------------------------------
 
-def test_gpu_reproduces_fe_bulk_susc_fixture(fe, fe_fixture):
+def test_gpu_reproduces_fe_bulk_susc_fixture(fe, fe_fixture, tmp_path):
+    """examples/Fe_bulk_susc on the GPU, entry by entry against the fixture's
+    numbers and, written as TITAN's chi_1_1 / chihf_1_1 files, against the
+    reference's own result files by its comparator at the example's 1e-5."""
+    from titan_amd import titan_io as IO
     m, _ = fe
     run = fe_fixture["run"]
     y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
@@ -342,6 +348,12 @@ def test_gpu_reproduces_fe_bulk_susc_fixture(fe, fe_fixture):
         bigr = np.abs(rr) > 1e-3
         assert np.max(np.abs(sc[w][bigr] - rr[bigr]) / np.abs(rr[bigr])) < 2e-8
     h.close()
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    for kind, rows in (("chi", sc), ("chihf", s0)):
+        p = IO.write_chi_files(str(tmp_path), kind, oms, 0.0, rows, 1)[0]
+        diffs = IO.compare_files(p, os.path.join(gold, "fe_%s_1_1.dat" % kind), 1e-5)
+        print(kind, "file differences at 1e-5:", diffs[:3], flush=True)
+        assert diffs == []
 
 
 def test_other_orbital_counts_use_generic_kernel():
