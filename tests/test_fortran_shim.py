@@ -57,7 +57,7 @@ def test_fortran_shim_matches_oracle(tmp_path, omega, prec):
     x2, p2 = Ct.real_epoints(m.Ef, omega, 1, 6)
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     F.write_input(fin, m, K, Y, W, kp, wk, [0, 0, 0], m.Ef, 5e-4, 5e-4, omega, x2, p2, prec=prec)
-    chi0, chi, log = F.run(fin, fout, m.D)
+    chi0, chi, log, ex = F.run(fin, fout, m.D, extras=True, nA=m.nA, nOrb=m.nOrb)
     if omega:
         o = s.chi0([0, 0, 0], omega, 5e-4, 5e-4, (K, Y, W), (kp, wk), (x2, p2))
     else:
@@ -66,3 +66,12 @@ def test_fortran_shim_matches_oracle(tmp_path, omega, prec):
     tol = 1e-10 if prec == 0 else 1e-6
     assert np.linalg.norm(chi0 - o) / np.linalg.norm(o) < tol
     assert np.linalg.norm(chi - oc) / np.linalg.norm(oc) < tol
+    if prec == 0:   # eintshe (field along x, HF) and the expectation values through the shim, FP64
+        from oracle import current as OC
+        from oracle import expectation as OE
+        real = ((kp, wk), (x2, p2)) if omega else (None, None)
+        tc = OC.current_response(s, [0, 0, 0], omega, 5e-4, 5e-4, [1.0, 0.0, 0.0], (K, Y, W), *real)
+        assert np.linalg.norm(ex["tchi"] - tc) / np.linalg.norm(tc) < 1e-10
+        want = OE.expectation_values(s, K, Y, W, m.Ef, 5e-4)
+        for name, w in zip(("rho", "mx", "my", "mz", "mp"), want):
+            assert np.abs(ex[name] - w).max() < 1e-11 * max(1.0, np.abs(w).max()), name

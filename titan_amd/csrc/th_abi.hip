@@ -283,6 +283,7 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
             for (double y : h->yim_host) bb = std::max(bb, 1.0 / (y + etap));
             a.presplit = 1;
             a.pexp = th_pow2_scale(bb);
+            h->bcache_pexp = a.pexp;
         }
         a.panel = (char *)h->d_bcache + (size_t)p0 * h->nA2 * 2 * h->Mpad * prec_size(prec);
         int id = tm.begin(0);

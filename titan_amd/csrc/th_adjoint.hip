@@ -470,7 +470,7 @@ __global__ __launch_bounds__(512) void hhat_fused_kernel(assemble_params a, cons
 // B(i,j) = vec G(j,i), so G[(j bs + rr) d + i bs + cc] = B[p][i nA + j][rr bs + cc]
 template <typename P>
 __global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, int bs, int Mpad, th_rowmap rows,
-                                double2 *raw)
+                                float unsplit, double2 *raw)
 {
     const int d = nA * bs;
     const size_t dd = (size_t)d * d;
@@ -481,6 +481,17 @@ __global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, in
     const int r = f / d, c = f - (f / d) * d;
     const int j = r / bs, rr = r - j * bs, i = c / bs, cc = c - i * bs;
     const P *b = panel + ((size_t)th_map_row(rows.pg0 + pl, rows) * nA * nA + (size_t)i * nA + j) * 2 * Mpad;
+    if constexpr (sizeof(P) == sizeof(float)) {
+        if (unsplit != 0.0f) {   // split words: (hi + lo) 2^-e
+            auto dec = [unsplit](P v) {
+                const unsigned wv = __builtin_bit_cast(unsigned, (float)v);
+                return (double)(((float)__builtin_bit_cast(_Float16, (unsigned short)(wv & 0xffffu)) +
+                                 (float)__builtin_bit_cast(_Float16, (unsigned short)(wv >> 16))) * unsplit);
+            };
+            raw[e] = make_double2(dec(b[rr * bs + cc]), dec(b[Mpad + rr * bs + cc]));
+            return;
+        }
+    }
     raw[e] = make_double2((double)b[rr * bs + cc], (double)b[Mpad + rr * bs + cc]);
 }
 
@@ -853,7 +864,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     tm.end(id);
     const size_t accsz = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
     const bool use_bcache = h->d_bcache && h->bcache_prec == precision && h->bcache_key[0] == Ef &&
-                            h->bcache_key[1] == etap && !h->bcache_presplit;
+                            h->bcache_key[1] == etap;
+    // split words in the cache are read back as (hi + lo) 2^-e
+    const int split_e = (use_bcache && h->bcache_presplit) ? h->bcache_pexp : INT_MIN;
     const int ntile = (d + 31) / 32;
     // 3. pair batches of each axis
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
@@ -881,12 +894,13 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 brows.bpc = h->bcache_bp;
                 brows.ntot = h->n_im;
                 const unsigned g = nblk((size_t)np * dd, 256);
+                const float unsplit = split_e == INT_MIN ? 0.0f : ldexpf(1.0f, -split_e);
                 if (precision == TH_PREC_FP64)
                     hipLaunchKernelGGL(raw_from_bpanel<double>, dim3(g), dim3(256), 0, h->stream, (const double *)pB,
-                                       np, h->nA, h->bs, h->Mpad, brows, rawB);
+                                       np, h->nA, h->bs, h->Mpad, brows, 0.0f, rawB);
                 else
                     hipLaunchKernelGGL(raw_from_bpanel<float>, dim3(g), dim3(256), 0, h->stream, (const float *)pB, np,
-                                       h->nA, h->bs, h->Mpad, brows, rawB);
+                                       h->nA, h->bs, h->Mpad, brows, unsplit, rawB);
                 TH_HIP(hipGetLastError());
             } else {
                 a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
@@ -896,7 +910,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             }
             tm.end(id);
             id = tm.begin(1);
-            if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA, brows))) return rc;
+            if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA, brows,
+                                              axis == 0 && use_bcache ? split_e : INT_MIN)))
+                return rc;
             if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB))) return rc;
             h->n_launch[1] += 2;
             tm.end(id);

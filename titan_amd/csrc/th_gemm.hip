@@ -18,6 +18,7 @@
 // register prefetch.  Partial tiles go to a split buffer and are summed in
 // a fixed order (bitwise reproducible).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "th_internal.h"
@@ -532,7 +533,8 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
 template <typename P, bool K3M>
 __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict__ X, const double *__restrict__ That,
                                                             long np, int Mpad, int nA2, int ntile, int trans,
-                                                            double *__restrict__ out, th_rowmap rows)
+                                                            double *__restrict__ out, th_rowmap rows,
+                                                            float unsplit)
 {
     __shared__ __align__(16) P Xs[2][TH_KC][2][TH_TILE];
     __shared__ __align__(16) P Ys[2][TH_KC][2][TH_TILE];
@@ -582,6 +584,15 @@ __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict_
                 int kk, m, comp;
                 xidx(v, kk, m, comp);
                 if (p0 + m < np) x = Xb[(size_t)xrow[u] * pstride + (size_t)comp * Mpad + k0 + kk];
+                if constexpr (sizeof(P) == sizeof(float)) {
+                    // split panels: the word holds fp16 (hi, lo) of x 2^e; unsplit = 2^-e
+                    if (unsplit != 0.0f) {
+                        const unsigned wv = __builtin_bit_cast(unsigned, (float)x);
+                        const float hi = (float)__builtin_bit_cast(_Float16, (unsigned short)(wv & 0xffffu));
+                        const float lo = (float)__builtin_bit_cast(_Float16, (unsigned short)(wv >> 16));
+                        x = (P)((hi + lo) * unsplit);
+                    }
+                }
                 yidx(v, kk, m, comp);
                 const int n = n0 + m, k = k0 + kk;
                 y = (P)(trans ? Tb[(size_t)comp * MM + (size_t)n * Mpad + k] : Tb[(size_t)comp * MM + (size_t)k * Mpad + n]);
@@ -822,8 +833,9 @@ int th_launch_finalize(th_handle *h, double2 *chi0)
 }
 
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
-                            double *out, const th_rowmap &rows)
+                            double *out, const th_rowmap &rows, int split_e)
 {
+    const float unsplit = split_e == INT_MIN ? 0.0f : ldexpf(1.0f, -split_e);
     if (np <= 0) return TH_OK;
     const int ntile = h->Mpad / TH_TILE;
     const long nptile = (np + TH_TILE - 1) / TH_TILE;
@@ -832,17 +844,17 @@ int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const dou
     if (prec == TH_PREC_FP64) {
         if (h->contract_3m)
             hipLaunchKernelGGL((panel_adjoint_kernel<double, true>), grid, blk, 0, h->stream, (const double *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows, unsplit);
         else
             hipLaunchKernelGGL((panel_adjoint_kernel<double, false>), grid, blk, 0, h->stream, (const double *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows, unsplit);
     } else {
         if (h->contract_3m)
             hipLaunchKernelGGL((panel_adjoint_kernel<float, true>), grid, blk, 0, h->stream, (const float *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows, unsplit);
         else
             hipLaunchKernelGGL((panel_adjoint_kernel<float, false>), grid, blk, 0, h->stream, (const float *)panel,
-                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows);
+                               That, np, h->Mpad, h->nA2, ntile, trans, out, rows, unsplit);
     }
     TH_HIP(hipGetLastError());
     return TH_OK;

@@ -1,6 +1,7 @@
 // th_internal.h -- shared definitions of the gfx950 kernels and the C ABI.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -68,6 +69,7 @@ struct th_handle {
     double bcache_key[2] = {0, 0};   // (Ef, etap)
     long bcache_bp = 0;              // stored in summation order per batch of this many pairs (0: pair order)
     int bcache_presplit = 0;         // rows hold fp16 splits (th_green_args::presplit)
+    int bcache_pexp = 0;             // their exponent
     // Dyson work space
     int *d_sup = nullptr;        // support of U
     double2 *d_U = nullptr, *d_W = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_X = nullptr;
@@ -261,8 +263,9 @@ int th_launch_finalize(th_handle *h, double2 *chi0);
 // Adjoint of the contraction for one panel: out[p] = conj(panel[p]) . That^T
 // (trans = 1) or conj(panel[p]) . That (trans = 0) per atom pair, FP64 out
 // [np][nA2][2][Mpad]; That is [nA2][2][Mpad][Mpad] FP64.
+// split_e != INT_MIN: the panel holds split (hi, lo) words of x 2^split_e (FP32 only)
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
-                            double *out, const th_rowmap &rows = th_rowmap());
+                            double *out, const th_rowmap &rows = th_rowmap(), int split_e = INT_MIN);
 
 // ---- Dyson (th_dyson.hip) ----
 int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,

@@ -106,7 +106,8 @@ module mod_titan_hip
     end function
   end interface
 
-  public :: th_setup_from_arrays, th_eintshechi, th_rpa_dyson, th_release, th_error_text
+  public :: th_setup_from_arrays, th_eintshechi, th_rpa_dyson, th_eintshe, th_expectation, th_release, &
+            th_error_text
 
 contains
 
@@ -170,6 +171,29 @@ contains
     real(c_double) :: c
     call check(th_dyson(handle, chiorb_hf, Umatorb, chiorb, c), "th_rpa_dyson")
     if (present(cond)) cond = c
+  end subroutine
+
+  !> tchiorbiikl(dimens,4) for one frequency e: the eintshe(q,e) body for the
+  !> field direction ue (ElectricFieldVector); prefactor as calculate_all sets it
+  !> (omitted: identity, the HF response).
+  subroutine th_eintshe(q, e, Ef, eta, etap, x2, p2, ue, tchiorbiikl, prefactor)
+    real(c_double), intent(in) :: q(3), e, Ef, eta, etap, x2(:), p2(:), ue(3)
+    complex(c_double_complex), intent(out) :: tchiorbiikl(:,:)
+    complex(c_double_complex), intent(in), target, optional :: prefactor(:,:)
+    type(c_ptr) :: pp
+    pp = c_null_ptr
+    if (present(prefactor)) pp = c_loc(prefactor)
+    call check(th_current_response(handle, q, e, Ef, eta, etap, int(size(x2),c_int), x2, p2, ue, pp, tchiorbiikl), &
+               "th_eintshe")
+  end subroutine
+
+  !> rho, mx, my, mz (nOrb,nAtoms) and mp (nOrb,nAtoms) of
+  !> expectation_values_greenfunction over the handle's imaginary list.
+  subroutine th_expectation(Ef, eta, rho, mp, mx, my, mz)
+    real(c_double), intent(in) :: Ef, eta
+    real(c_double), intent(out) :: rho(:,:), mx(:,:), my(:,:), mz(:,:)
+    complex(c_double_complex), intent(out) :: mp(:,:)
+    call check(th_expectation_values(handle, Ef, eta, rho, mx, my, mz, mp), "th_expectation")
   end subroutine
 
   subroutine th_release()
