@@ -72,7 +72,7 @@ int density_blocks(th_handle *h, double Ef, double eta, std::vector<double2> &ac
         hipMalloc(&d_acc, sizeof(double2) * nout) != hipSuccess) {
         cleanup();
         (void)hipGetLastError();
-        th_set_error("th_spin_density: out of device memory");
+        th_set_error("ground-state sweep: out of device memory");
         return TH_E_HIP;
     }
     int rc = th_ensure_panels(h, bp);
@@ -96,7 +96,7 @@ int density_blocks(th_handle *h, double Ef, double eta, std::vector<double2> &ac
                            h->stream, (const double2 *)d_raw, np, nA, nO, h->d_wim, p0, d_part);
         hipLaunchKernelGGL(density_sum_kernel, dim3((unsigned)((nout + 63) / 64)), dim3(64), 0, h->stream,
                            (const double2 *)d_part, nch, nout, d_acc);
-        if (hipGetLastError() != hipSuccess) { rc = TH_E_HIP; th_set_error("th_spin_density: kernel launch"); }
+        if (hipGetLastError() != hipSuccess) { rc = TH_E_HIP; th_set_error("ground-state sweep: kernel launch"); }
     }
     acc.assign(nout, make_double2(0.0, 0.0));
     if (!rc && hipMemcpyAsync(acc.data(), d_acc, sizeof(double2) * nout, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
