@@ -1,0 +1,238 @@
+"""TITAN's input decks and self-consistency files, on the host.
+
+A TITAN run directory holds the `input` card, the `basis` file, one element
+file per atom type and, from an earlier run, results/<SOC>/selfconsistency/
+selfconsistency_*.dat.  This module reads them into the parameter dicts
+`titan_amd.system.TBModel` and `titan_amd.configs.run_params` use, following
+the reference's readers:
+
+  read_input        '-> key = value ! comment' lines (mod_input.F90)
+  read_basis        name, a0, Bravais rows, element names, counts, coordinate
+                    type ('C'/'K' cartesian in a0, otherwise Bravais), one
+                    position per atom (read_basis, mod_polyBasis.F90:23-166)
+  read_element      the two-centre tight-binding element file in TITAN's
+                    tbmode 1 layout (readElementFile, TightBinding.F90:221-653):
+                    its own lattice and positions, sysdim, Fermi level,
+                    occupations, U (one value: Un = Um), the p and d spin-orbit
+                    strengths, the superconducting line, the number of
+                    neighbour stages, four on-site lines (one value, or one per
+                    orbital of the shell when the fourth word is not the line
+                    id) and ten Slater-Koster lines per stage (value in the
+                    fourth word)
+  read_sc_results   5 nOrb numbers per atom (n, mx, my, mz, delta_sc; list-
+                    directed, a record may wrap) then E_F (read_sc_results,
+                    mod_self_consistency.F90:271-435)
+  write_sc_results  the same file, "(5nOrb(es21.11,2x))" rows, the E_F line and
+                    the two comment lines (write_sc_results,
+                    mod_self_consistency.F90:1384-1435)
+
+Only the parts of the decks the chi(omega) path consumes are read; the
+self-consistency loop, DFT (tbmode 2) element files and superconducting runs
+are outside this package.
+"""
+import glob
+import os
+
+ORBITALS = {"s": 1, "px": 2, "py": 3, "pz": 4, "dxy": 5, "dyz": 6, "dzx": 7, "dx2": 8, "dz2": 9,
+            "p": (2, 3, 4), "d": (5, 6, 7, 8, 9), "t2g": (5, 6, 7), "eg": (8, 9)}
+
+
+def fortran_float(tok):
+    """A Fortran real literal (1.d-6, 1e-3, 0.5)."""
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def _floats(line, n=None):
+    """Leading numeric words of a line (a list-directed read stops at the
+    first word that is not a number)."""
+    out = []
+    for tok in line.split():
+        try:
+            out.append(fortran_float(tok))
+        except ValueError:
+            break
+        if n is not None and len(out) == n:
+            break
+    return out
+
+
+def _content(lines):
+    """Lines that are not blank and do not start with '!' (next_line of the
+    reference skips them)."""
+    return [l for l in lines if l.strip() and not l.lstrip().startswith("!")]
+
+
+def read_input(path):
+    """{key: value string} of the '->' lines; keys as written."""
+    keys = {}
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if not line.startswith("->"):
+                continue
+            body = line[2:].split("!")[0]
+            if "=" not in body:
+                continue
+            k, v = body.split("=", 1)
+            keys[k.strip()] = v.strip()
+    return keys
+
+
+def _get(keys, name, default=None):
+    low = {k.lower(): v for k, v in keys.items()}
+    return low.get(name.lower(), default)
+
+
+def _flag(v):
+    return v is not None and v.strip().upper().startswith(("T", ".T"))
+
+
+def read_basis(path):
+    """{"name", "a0", "a" (Bravais rows x a0), "elements", "counts", "coord",
+    "pos" (cartesian, bohr)}."""
+    with open(path) as fh:
+        lines = [l.rstrip("\n") for l in fh]
+    name = lines[0].strip()
+    a0 = _floats(lines[1], 1)[0]
+    a = [[x * a0 for x in _floats(lines[2 + i], 3)] for i in range(3)]
+    elements = []
+    for w in lines[5].split():
+        if w.startswith("!"):
+            break
+        elements.append(w)
+    counts = [int(x) for x in lines[6].split()[:len(elements)]]
+    coord = lines[7].split()[0]
+    pos, k = [], 8
+    for _ in range(sum(counts)):
+        while not lines[k].strip():
+            k += 1
+        p = _floats(lines[k], 3)
+        k += 1
+        if coord[0] in "CcKk":
+            pos.append([x * a0 for x in p])
+        else:
+            pos.append([sum(p[m] * a[m][x] for m in range(3)) for x in range(3)])
+    return {"name": name, "a0": a0, "a": a, "elements": elements, "counts": counts, "coord": coord, "pos": pos}
+
+
+def read_element(path, nstages):
+    """{"onsite" [9] (s, px, py, pz, dxy, dyz, dzx, dx2, dz2), "sk" [nstages][10]
+    (sss pps ppp dds ddp ddd sps sds pds pdp), "lambda" [p, d], "U" [Un, Um],
+    "fermi_level", "sysdim", "occupations", "a0", "bravais", "name"}."""
+    with open(path) as fh:
+        raw = [l.rstrip("\n") for l in fh]
+    name = raw[0].strip()
+    lines = _content(raw[1:])
+    a0 = _floats(lines[0], 1)[0]
+    bravais = [_floats(lines[1 + i], 3) for i in range(3)]
+    counts = [int(x) for x in lines[5].split() if x.lstrip("-").isdigit()]
+    i = 7 + sum(counts)                    # after coordinate type and positions
+    sysdim = int(_floats(lines[i], 1)[0])
+    i += 1
+    if not _floats(lines[i]):              # an orbital selection line precedes the Fermi level
+        i += 1
+    fermi = _floats(lines[i], 1)[0]
+    occ = _floats(lines[i + 1], 3)
+    u = _floats(lines[i + 2], 2)
+    lam = _floats(lines[i + 3], 2)
+    nfile = int(_floats(lines[i + 5], 1)[0])
+    on = lines[i + 6:i + 10]
+
+    def shell(line, line_id, n):
+        w = line.split()
+        if w[3] == str(line_id):
+            return [fortran_float(w[2])] * n
+        return [fortran_float(x) for x in w[2:2 + n]]
+
+    onsite = [fortran_float(on[0].split()[2])] + shell(on[1], 2, 3) + shell(on[2], 3, 3) + shell(on[3], 4, 2)
+    if nstages > nfile:
+        raise ValueError("%s: %d neighbour stages requested, %d in the file" % (path, nstages, nfile))
+    sk = [[fortran_float(lines[i + 10 + 10 * st + p].split()[3]) for p in range(10)] for st in range(nstages)]
+    return {"name": name, "onsite": onsite, "sk": sk, "lambda": lam, "U": [u[0], u[0]] if len(u) == 1 else u,
+            "fermi_level": fermi, "sysdim": sysdim, "occupations": occ, "a0": a0, "bravais": bravais}
+
+
+def read_sc_results(path, norbs):
+    """norbs: orbitals per atom.  Returns {"rho", "mx", "my", "mz", "delta_sc":
+    [per atom lists], "Ef"}."""
+    with open(path) as fh:
+        lines = [l for l in fh]
+    out = {k: [] for k in ("rho", "mx", "my", "mz", "delta_sc")}
+    k = 0
+    for n in norbs:
+        vals = []
+        while len(vals) < 5 * n:
+            if k >= len(lines):
+                raise ValueError("%s: not enough values for %d atoms" % (path, len(norbs)))
+            vals += _floats(lines[k])
+            k += 1
+        for c, key in enumerate(("rho", "mx", "my", "mz", "delta_sc")):
+            out[key].append(vals[c * n:(c + 1) * n])
+    ef = _floats(lines[k], 1) if k < len(lines) else []
+    if not ef:
+        raise ValueError("%s: missing Fermi energy" % path)
+    out["Ef"] = ef[0]
+    return out
+
+
+def write_sc_results(path, rho, mx, my, mz, Ef, delta_sc=None):
+    """The file read_sc_results reads, formatted as the reference writes it."""
+    with open(path, "w") as fh:
+        for i in range(len(rho)):
+            d = delta_sc[i] if delta_sc is not None else [0.0] * len(rho[i])
+            vals = list(rho[i]) + list(mx[i]) + list(my[i]) + list(mz[i]) + list(d)
+            fh.write("  ".join("%21.11E" % float(v) for v in vals) + "\n")
+        fh.write("%21.11E  ! Ef  \n" % float(Ef))
+        fh.write("! n(1:nOrb), mx(1:nOrb), my(1:nOrb), mz(1:nOrb), delta_sc(1:nOrb) per site \n")
+        fh.write("! Ef \n")
+
+
+def orbitals(value):
+    """TITAN orbital indices (1 s, 2-4 p, 5-9 d) of the 'orbitals' input key;
+    all nine when absent."""
+    if value is None:
+        return list(range(1, 10))
+    out = []
+    for w in value.split():
+        o = ORBITALS[w.lower()]
+        out += list(o) if isinstance(o, tuple) else [o]
+    return out
+
+
+def load_deck(directory, sc_file=None):
+    """{"system": TBModel parameters, "run": chi(omega) run parameters} of a
+    TITAN run directory.  The ground state comes from sc_file, or the single
+    results/*SOC/selfconsistency/selfconsistency_*.dat of the directory, or
+    is left at zero with E_F from the input key 'Ef' or the element file."""
+    inp = read_input(os.path.join(directory, "input"))
+    nst = int(_get(inp, "nn_stages", "2"))
+    basis = read_basis(os.path.join(directory, "basis"))
+    types = [read_element(os.path.join(directory, e), nst) for e in basis["elements"]]
+    type_of = [t for t, c in enumerate(basis["counts"]) for _ in range(c)]
+    orbs = orbitals(_get(inp, "orbitals"))
+    nA, nO = len(type_of), len(orbs)
+    soc = _get(inp, "SOC")
+    socscale = fortran_float(_get(inp, "socscale", "1")) if (soc is None or _flag(soc)) else 0.0
+    if sc_file is None:
+        socdir = "TSOC" if (soc is None or _flag(soc)) else "FSOC"
+        found = sorted(glob.glob(os.path.join(directory, "results*", socdir, "selfconsistency",
+                                              "selfconsistency_*.dat")))
+        sc_file = found[0] if len(found) == 1 else None
+    if sc_file is not None:
+        sc = read_sc_results(sc_file, [nO] * nA)
+        ef = sc["Ef"]
+    else:
+        sc = {k: [[0.0] * nO for _ in range(nA)] for k in ("rho", "mx", "my", "mz")}
+        ef = fortran_float(_get(inp, "Ef")) if _get(inp, "Ef") else types[0]["fermi_level"]
+    eta = fortran_float(_get(inp, "eta", "1e-3"))
+    system = {"name": basis["name"], "sysdim": int(_get(inp, "sysdim", types[0]["sysdim"])), "a": basis["a"],
+              "pos": basis["pos"], "type": type_of, "orbs": orbs, "nStages": nst, "relTol": 0.05,
+              "types": [{k: t[k] for k in ("onsite", "sk", "lambda", "U")} for t in types],
+              "rho": sc["rho"], "mx": sc["mx"], "my": sc["my"], "mz": sc["mz"], "Ef": ef, "socscale": socscale}
+    run = {"eta": eta, "etap": fortran_float(_get(inp, "etap", _get(inp, "eta", "1e-3"))),
+           "nkpt": int(_get(inp, "nkpt", "0")), "parts": int(_get(inp, "parts", "2")),
+           "n1gl": int(_get(inp, "n1gl", "64")), "parts3": int(_get(inp, "parts3", "1")),
+           "n3gl": int(_get(inp, "n3gl", "64")), "emin": fortran_float(_get(inp, "emin", "0")),
+           "emax": fortran_float(_get(inp, "emax", "0")), "nEner": int(_get(inp, "nEner", "1")),
+           "q": [0.0, 0.0, 0.0]}
+    return {"system": system, "run": run, "input": inp, "sc_file": sc_file}
