@@ -1245,12 +1245,51 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
     constexpr int VST = KP + 8;                  // LDS row stride in halves (16-byte reads on distinct banks)
     __shared__ __align__(16) _Float16 Vs[2][2][NC * VST];   // [re|im][hi|lo][c][m] = conj(V[c][m])
     __shared__ float2 dv[2][KP];                 // diagonal of the pair, by pair parity
+    __shared__ double pzr[CH], pzi[CH], pzw[CH]; // Re z, Im z and weight of the workgroup's pairs
+    __shared__ int pkb[CH];                      // their eigen-decomposition
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const long s0 = (long)blockIdx.x * CH;
     const long s1 = s0 + CH < g.np ? s0 + CH : g.np;
     const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
     const bool bside = g.mode == TH_G_IMAG_B || g.mode == TH_G_REAL_B;
     const int bs = g.bs, nA = g.nA, Mpad = g.Mpad, Mb = bs * bs;
+    // the pairs' scalars are read once, so that no global load sits between
+    // the per-pair barriers
+    if (tid < CH && s0 + tid < s1) {
+        const long p = g.p0 + s0 + tid;
+        double k0, k1, k2, er, ei, w;
+        pair_point(g, p, k0, k1, k2, er, ei, w);
+        long kb;
+        if (imag) {
+            kb = g.kid_im[p];
+        } else {
+            long ie, ik;
+            real_split(g, p, ie, ik);
+            kb = g.kid_re[ik];
+        }
+        pzr[tid] = er;
+        pzi[tid] = ei;
+        pzw[tid] = w;
+        pkb[tid] = (int)kb;
+    }
+    // diagonal d = (ca Re f + cb Im f) + i cc Im f of pair jj into dv[buf] from
+    // the run's eigenvalue lam of this thread (m = tid)
+    auto diagonal = [&](int jj, int buf, double lam) {
+        if (tid < KP) {
+            const double w = pzw[jj];
+            double ca = 1.0, cb = 0.0, cc = 1.0;
+            if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }
+            if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
+            if (g.mode == TH_G_REAL_B) cc = -1.0;
+            float2 d = make_float2(0.0f, 0.0f);
+            if (tid < DD) {
+                const double ei = pzi[jj], a = pzr[jj] - lam, n = a * a + ei * ei;
+                const double fr = a / n, fi = -ei / n;
+                d = make_float2((float)(ca * fr + cb * fi), (float)(cc * fi));
+            }
+            dv[buf][tid] = d;
+        }
+    };
     int rI[4], rR[4], cI[NB], cR[NB];
     bool rv[4], cv[NB];
 #pragma unroll
@@ -1273,26 +1312,18 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
     // current node weights)
     const int eP = (g.presplit && g.pareal > 0.0) ? th_real_exp(g.p2, g.pnx, g.pareal) : g.pexp;
     long cur = -1;
+    double lam = 0.0;                           // eigenvalue tid of the current run
     float avr[KT][8], avi[KT][8];
+    __syncthreads();                            // the pairs' scalars
     for (long pl = s0; pl < s1; ++pl) {
-        const long p = g.p0 + pl;
-        const int par = (int)(pl & 1);
-        double k0, k1, k2, er, ei, w;
-        pair_point(g, p, k0, k1, k2, er, ei, w);
-        long kb;
-        if (imag) {
-            kb = g.kid_im[p];
-        } else {
-            long ie, ik;
-            real_split(g, p, ie, ik);
-            kb = g.kid_re[ik];
-        }
+        const int jj = (int)(pl - s0), par = (int)(pl & 1);
+        const long kb = pkb[jj];
+        const double ei = pzi[jj], w = pzw[jj];
         // diagonal coefficients: d = (ca Re f + cb Im f) + i cc Im f
         double ca = 1.0, cb = 0.0, cc = 1.0;
         if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }
         if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
         if (g.mode == TH_G_REAL_B) cc = -1.0;
-        const double* Lk = g.eigL + (size_t)kb * DD;
         if (kb != cur) {                        // same kb in every thread
             __syncthreads();                    // the previous run's reads of Vs are done
             const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
@@ -1320,18 +1351,15 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
                     avr[t][i] = (float)x.x;
                     avi[t][i] = (float)x.y;
                 }
+            lam = tid < DD ? g.eigL[(size_t)kb * DD + tid] : 0.0;
             cur = kb;
-        }
-        for (int m = tid; m < KP; m += NT) {
-            float2 d = make_float2(0.0f, 0.0f);
-            if (m < DD) {
-                const double a = er - Lk[m], n = a * a + ei * ei;
-                const double fr = a / n, fi = -ei / n;
-                d = make_float2((float)(ca * fr + cb * fi), (float)(cc * fi));
-            }
-            dv[par][m] = d;
+            diagonal(jj, par, lam);
         }
         __syncthreads();                        // the pair's diagonal (and a new run's V) in LDS
+        // the next pair's diagonal, when it shares this run, into the other
+        // buffer (every wave has passed this pair's barrier, so the reads of
+        // that buffer for the previous pair are done)
+        if (pl + 1 < s1 && pkb[jj + 1] == cur) diagonal(jj + 1, par ^ 1, lam);
         // power-of-two scale of V diag(d): |d_m| <= max(|ca|, |cb|, |cc|) / Im z
         const double bound = fmax(fmax(fabs(ca), fabs(cb)), fabs(cc)) / fabs(ei);
         // bound = m 2^ex with m in [0.5, 1) (exponent bits; an infinite or zero
