@@ -1238,8 +1238,9 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_chunk_ke
 // accumulators).  The diagonal d_m is computed once per pair (one eigenvalue
 // per thread, double-buffered in LDS by pair parity, one barrier per pair).
 typedef _Float16 ghv8 __attribute__((ext_vector_type(8)));
+constexpr unsigned TH_G_OOB = 0x10000000u;     // byte offset past any panel row (dropped stores)
 template <int DD, int CH>
-__global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kernel(green_params g)
+__global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_per_eu(3))) void green_spectral_h2_kernel(green_params g)
 {
     constexpr int NB = (DD + 15) / 16, NT = 64 * NB, KT = (DD + 31) / 32, KP = 32 * KT, NC = 16 * NB;
     constexpr int VST = KP + 8;                  // LDS row stride in halves (16-byte reads on distinct banks)
@@ -1308,6 +1309,18 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
     }
     const int arow = 16 * wave + (lane & 15);   // row of V in this lane's A operand
     const int kofs = 8 * (lane >> 4);            // first of the lane's 8 eigen-indices per K step
+    // byte offsets of the lane's outputs in a pair's panel row, a row part plus
+    // a column part; an entry outside the matrix gets an offset far past the
+    // row (TH_G_OOB >= the row's bytes, checked by the launcher), which the
+    // buffer store drops
+    unsigned rof[4], cof[NB];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        rof[q] = rv[q] ? 4u * (unsigned)((bside ? rI[q] : rI[q] * nA) * 2 * Mpad + rR[q]) : TH_G_OOB;
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        cof[j] = cv[j] ? 4u * (unsigned)((bside ? cI[j] * nA : cI[j]) * 2 * Mpad + cR[j]) : TH_G_OOB;
+    const int rowbytes = 4 * g.nA2 * 2 * Mpad, imoff = 4 * Mpad;
     // split panels: exponent of the panel (the real-axis A side from the
     // current node weights)
     const int eP = (g.presplit && g.pareal > 0.0) ? th_real_exp(g.p2, g.pnx, g.pareal) : g.pexp;
@@ -1409,46 +1422,35 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_h2_kerne
             }
         const float back = ldexpf(1.0f, -es);
         const int npad = Mpad - Mb;
-        if (g.presplit) {
-            // x 2^eP = hi + lo as the contraction would split the FP32 value
-            // x = acc 2^-es (the power-of-two scalings are exact, x 2^eP - hi
-            // is exact in FP32); hi and lo share one 4-byte word
-            unsigned *out = reinterpret_cast<unsigned *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
+        void *row = reinterpret_cast<unsigned *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
+        const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(row, (short)0, rowbytes, 0x00020000);
 #pragma unroll
-            for (int j = 0; j < NB; ++j)
+        for (int j = 0; j < NB; ++j)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (rv[q] && cv[j]) {
-                        const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
-                        const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
-                        const float xr = ldexpf(acp[j][q] - acn[j][q], eP - es), xi = ldexpf(aci[j][q], eP - es);
-                        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
-                        const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
-                        out[o] = (unsigned)__builtin_bit_cast(unsigned short, hr) |
-                                 ((unsigned)__builtin_bit_cast(unsigned short, lr) << 16);
-                        out[o + Mpad] = (unsigned)__builtin_bit_cast(unsigned short, hi_) |
-                                        ((unsigned)__builtin_bit_cast(unsigned short, li) << 16);
-                    }
-            for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
-                const int part = e / npad;
-                out[(size_t)part * Mpad + Mb + (e - part * npad)] = 0u;
+            for (int q = 0; q < 4; ++q) {
+                unsigned wr, wi;
+                if (g.presplit) {
+                    // x 2^eP = hi + lo as the contraction would split the FP32
+                    // value x = acc 2^-es (the power-of-two scalings are exact,
+                    // x 2^eP - hi is exact in FP32); hi and lo share one word
+                    const float xr = ldexpf(acp[j][q] - acn[j][q], eP - es), xi = ldexpf(aci[j][q], eP - es);
+                    const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+                    const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+                    wr = (unsigned)__builtin_bit_cast(unsigned short, hr) |
+                         ((unsigned)__builtin_bit_cast(unsigned short, lr) << 16);
+                    wi = (unsigned)__builtin_bit_cast(unsigned short, hi_) |
+                         ((unsigned)__builtin_bit_cast(unsigned short, li) << 16);
+                } else {
+                    wr = __builtin_bit_cast(unsigned, (acp[j][q] - acn[j][q]) * back);
+                    wi = __builtin_bit_cast(unsigned, aci[j][q] * back);
+                }
+                const unsigned o = rof[q] + cof[j];
+                __builtin_amdgcn_raw_buffer_store_b32(wr, ro, o, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(wi, ro, o, imoff, 0);
             }
-        } else {
-            float *out = reinterpret_cast<float *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
-#pragma unroll
-            for (int j = 0; j < NB; ++j)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (rv[q] && cv[j]) {
-                        const int blk = bside ? cI[j] * nA + rI[q] : rI[q] * nA + cI[j];
-                        const size_t o = (size_t)blk * 2 * Mpad + rR[q] + cR[j];
-                        out[o] = (acp[j][q] - acn[j][q]) * back;
-                        out[o + Mpad] = aci[j][q] * back;
-                    }
-            for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
-                const int part = e / npad;
-                out[(size_t)part * Mpad + Mb + (e - part * npad)] = 0.0f;
-            }
+        for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+            const int part = e / npad;
+            __builtin_amdgcn_raw_buffer_store_b32(0u, ro, 4u * (unsigned)(part * Mpad + Mb + (e - part * npad)), 0, 0);
         }
     }
 }
@@ -1615,10 +1617,15 @@ int launch_green_spectral(th_handle *h, const th_green_args &a, const green_para
     if (!a.raw && h->spectral_chunk > 0) {   // runs of pairs per k-point; G is not kept
         constexpr int CH = TH_RUN, NT = 64 * ((DD + 15) / 16);
         const unsigned nblk = (unsigned)((a.np + CH - 1) / CH);
-        if (sizeof(P) == sizeof(float) && h->green_f16)
+        if (sizeof(P) == sizeof(float) && h->green_f16) {
+            if ((size_t)4 * g.nA2 * 2 * g.Mpad >= TH_G_OOB) {
+                th_set_error("half-precision spectral kernel: panel row too large");
+                return TH_E_ARG;
+            }
             hipLaunchKernelGGL((green_spectral_h2_kernel<DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
-        else
+        } else {
             hipLaunchKernelGGL((green_spectral_chunk_kernel<P, DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
+        }
     } else {
         hipLaunchKernelGGL((green_spectral_kernel<P, DD>), dim3((unsigned)a.np), dim3(256), 0, gstream(h, a), g);
     }
