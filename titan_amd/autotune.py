@@ -48,7 +48,7 @@ def _knobs(h, base, setting):
     yield "contract_f16", [1, 0]
     yield "contract_3m", [0, 1]
     yield "splits", sorted({max(1, s0 // 2), s0, 2 * s0})
-    yield "batch_mb", [256, 512, 1024]
+    yield "batch_mb", [256, 512, 1024, 2048]
 
 
 def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, repeat=2, log=None):
@@ -67,9 +67,14 @@ def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, r
     def timed(st):
         h.tune(**st)
         best = np.inf
-        for _ in range(repeat + 1):     # the first call warms caches and buffers
-            h.chi0(q, omega, Ef, eta, etap, x2, p2, precision=precision)
-            best = min(best, h.last_timing()["total_ms"])
+        try:
+            for _ in range(repeat + 1):     # the first call warms caches and buffers
+                h.chi0(q, omega, Ef, eta, etap, x2, p2, precision=precision)
+                best = min(best, h.last_timing()["total_ms"])
+        except B.TitanHipError as err:      # a shape this system cannot take (e.g. a batch too large)
+            if log:
+                log("autotune %s rejected: %s" % (st, err))
+            best = np.inf
         table.append((dict(st), best))
         if log:
             log("autotune %s %.3f ms" % (st, best))
@@ -86,5 +91,7 @@ def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, r
                 best_ms = ms
                 setting.update(trial)
     h.tune(**setting)
+    if not np.isfinite(best_ms):
+        raise B.TitanHipError("autotune: no setting ran on this system")
     return {"best": setting, "best_ms": best_ms, "default": base, "table": table,
             "seconds": time.perf_counter() - t0}
