@@ -48,7 +48,7 @@ def _knobs(h, base, setting):
     yield "contract_f16", [1, 0]
     yield "contract_3m", [0, 1]
     yield "splits", sorted({max(1, s0 // 2), s0, 2 * s0})
-    yield "batch_mb", [256, 512, 1024, 2048]
+    yield "batch_mb", [256, 512, 1024]
 
 
 def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, repeat=2, log=None):
