@@ -232,7 +232,9 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *   "overlap"        1 = th_chi0 pipelines the batches on two streams: the
  *                    eliminations of a batch run while the MFMA contraction
  *                    of the previous one runs (default; it applies when the
- *                    spectral path is not taken); 0 = one stream
+ *                    spectral path is not taken); 2 = the same, also for the
+ *                    spectral Green functions; 0 = one stream.  Results are
+ *                    bitwise the same for every value
  *   "contract_f16"   TH_PREC_MIXED: 1 = the contraction runs on the fp16 matrix
  *                    cores, each FP32 panel value split into two fp16 parts
  *                    (hi + lo, 22 bits) under one power-of-two scale per

@@ -686,3 +686,31 @@ def test_split_panels_are_bitwise_the_fp32_panel_path(q, omega):
     for a, b in zip(out[1], out[0]):
         print("q", q, "omega", omega, "max |split - fp32 panels| %.3e" % np.abs(a - b).max(), flush=True)
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [B.PREC_MIXED, B.PREC_FP64])
+def test_spectral_two_stream_pipeline_is_bitwise(precision):
+    """overlap = 2 runs the spectral Green functions of a batch on the second
+    stream while the previous batch is contracted (two panel slots, events
+    between the streams): chi0 equals the one-stream result bit for bit, over
+    several small batches (ragged last ones) on both axes, q != 0, and again
+    with the cached imaginary B panels."""
+    spec, run = CF.system(2), CF.run_params(2)
+    m = TBModel(spec)
+    kp, wk = Ct.uniform_kmesh(m.b, 6, 6, 6)
+    y, wy = Ct.imag_epoints(1, 12)
+    x2, p2 = Ct.real_epoints(m.Ef, 0.00078125, 1, 8)
+    q = [0.1, 0.0, 0.05]
+    out = {}
+    for ov in (0, 2):
+        h = m.handle()
+        h.tune(batch_mb=16, overlap=ov)
+        h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+        h.set_real_kmesh(kp, wk)
+        out[ov] = [h.chi0(q, 0.00078125, m.Ef, run["eta"], run["etap"], x2, p2, precision=precision)
+                   for _ in range(2)]
+        h.close()
+    for a, b in zip(out[2], out[0]):
+        print("precision", precision, "max |two streams - one| %.3e" % np.abs(a - b).max(), flush=True)
+        assert np.array_equal(a, b)

@@ -7,8 +7,9 @@ of the library's tuning knobs (th_tune):
                  elimination of every z - H (0), where built for d
   green_variant  elimination kernel shape (0 default tiles, 2 one wave per
                  matrix, 3 two waves per matrix, where built for d)
-  overlap        eliminations of one batch concurrent with the contraction
-                 of the previous one on a second stream
+  overlap        Green functions of one batch concurrent with the contraction
+                 of the previous one on a second stream (1: eliminations
+                 only, 2: also the spectral kernels)
   contract_f16   mixed precision: contraction on fp16 MFMA with two-term
                  splits of the FP32 panels (1) or on FP32 MFMA (0)
   contract_3m    three instead of four real MFMA products per complex one
@@ -45,6 +46,8 @@ def _knobs(h, base, setting):
     if not setting["spectral"]:
         yield "green_variant", list(VARIANTS.get(h.d, (0,)))
         yield "overlap", [1, 0]
+    else:
+        yield "overlap", [1, 2]
     yield "contract_f16", [1, 0]
     yield "contract_3m", [0, 1]
     yield "splits", sorted({max(1, s0 // 2), s0, 2 * s0})

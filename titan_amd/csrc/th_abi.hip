@@ -386,10 +386,11 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
     // for its inversions; the inversions into slot s wait for the contraction
     // that last read it.  The fork and join events make the two streams one
     // dependency graph, so the step can be captured as a hipGraph.
-    // (the spectral Green functions run on the matrix cores too, so the
-    // overlap only pays with the elimination kernels)
+    // overlap 1 pipelines the elimination kernels only; 2 also the spectral
+    // Green functions (their fp16 variant is issue-bound on the vector ALUs,
+    // the contraction on the matrix cores and LDS)
     const bool spectral_on = h->spectral && h->eig_valid[0];
-    const bool ov = h->overlap != 0 && !spectral_on;
+    const bool ov = h->overlap == 2 || (h->overlap == 1 && !spectral_on);
     hipStream_t gs = ov ? h->stream2 : h->stream;
     void *pA[2] = {h->d_pA, ov ? h->d_pA2 : h->d_pA}, *pB[2] = {h->d_pB, ov ? h->d_pB2 : h->d_pB};
     if (ov) {
@@ -941,7 +942,7 @@ int th_tune(th_handle *h, const char *key, long value)
         if (value < 0 || value > 1) { th_set_error("th_tune: pair_order is 0 or 1"); return TH_E_ARG; }
         h->pair_order = (int)value;   // applies from the next th_set_imag_points
     } else if (k == "overlap") {
-        if (value < 0 || value > 1) { th_set_error("th_tune: overlap is 0 or 1"); return TH_E_ARG; }
+        if (value < 0 || value > 2) { th_set_error("th_tune: overlap is 0, 1 or 2"); return TH_E_ARG; }
         h->overlap = (int)value;
     } else if (k == "contract_3m") {
         if (value < 0 || value > 1) { th_set_error("th_tune: contract_3m is 0 or 1"); return TH_E_ARG; }
