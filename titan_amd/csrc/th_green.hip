@@ -1424,30 +1424,52 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
         const int npad = Mpad - Mb;
         void *row = reinterpret_cast<unsigned *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
         const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(row, (short)0, rowbytes, 0x00020000);
+        auto put = [&](int j, int q, unsigned wr, unsigned wi) {
+            const unsigned o = rof[q] + cof[j];
+            __builtin_amdgcn_raw_buffer_store_b32(wr, ro, o, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(wi, ro, o, imoff, 0);
+        };
+        auto pack = [](_Float16 h, _Float16 l) {   // hi in the low half
+            typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+            const h2v v = {h, l};
+            return __builtin_bit_cast(unsigned, v);
+        };
+        if (g.presplit && eP - es >= -126 && eP - es <= 127) {
+            // x 2^eP = hi + lo as the contraction would split the FP32 value
+            // x = acc 2^-es; y 2^(eP-es) (y the accumulator, 2^(eP-es) a normal
+            // float) and y 2^(eP-es) - hi are exact in FP32, so one fused
+            // multiply-add rounded once to fp16 gives each part; hi and lo share
+            // one word
+            const float fs = ldexpf(1.0f, eP - es);
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                unsigned wr, wi;
-                if (g.presplit) {
-                    // x 2^eP = hi + lo as the contraction would split the FP32
-                    // value x = acc 2^-es (the power-of-two scalings are exact,
-                    // x 2^eP - hi is exact in FP32); hi and lo share one word
+                for (int q = 0; q < 4; ++q) {
+                    const float yr = acp[j][q] - acn[j][q], yi = aci[j][q];
+                    const _Float16 hr = (_Float16)fmaf(yr, fs, 0.0f), hi_ = (_Float16)fmaf(yi, fs, 0.0f);
+                    const _Float16 lr = (_Float16)fmaf(yr, fs, -(float)hr), li = (_Float16)fmaf(yi, fs, -(float)hi_);
+                    put(j, q, pack(hr, lr), pack(hi_, li));
+                }
+        } else if (g.presplit) {
+            // the same split with the scaling by ldexpf (2^(eP-es) outside the
+            // normal floats)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
                     const float xr = ldexpf(acp[j][q] - acn[j][q], eP - es), xi = ldexpf(aci[j][q], eP - es);
                     const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
                     const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
-                    wr = (unsigned)__builtin_bit_cast(unsigned short, hr) |
-                         ((unsigned)__builtin_bit_cast(unsigned short, lr) << 16);
-                    wi = (unsigned)__builtin_bit_cast(unsigned short, hi_) |
-                         ((unsigned)__builtin_bit_cast(unsigned short, li) << 16);
-                } else {
-                    wr = __builtin_bit_cast(unsigned, (acp[j][q] - acn[j][q]) * back);
-                    wi = __builtin_bit_cast(unsigned, aci[j][q] * back);
+                    put(j, q, pack(hr, lr), pack(hi_, li));
                 }
-                const unsigned o = rof[q] + cof[j];
-                __builtin_amdgcn_raw_buffer_store_b32(wr, ro, o, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b32(wi, ro, o, imoff, 0);
-            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    put(j, q, __builtin_bit_cast(unsigned, (acp[j][q] - acn[j][q]) * back),
+                        __builtin_bit_cast(unsigned, aci[j][q] * back));
+        }
         for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
             const int part = e / npad;
             __builtin_amdgcn_raw_buffer_store_b32(0u, ro, 4u * (unsigned)(part * Mpad + Mb + (e - part * npad)), 0, 0);
