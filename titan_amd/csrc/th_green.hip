@@ -1386,13 +1386,15 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const float2 d = dv[par][32 * t + kofs + i];
-                const float xr = (avr[t][i] * d.x - avi[t][i] * d.y) * sc;
-                const float xi = (avr[t][i] * d.y + avi[t][i] * d.x) * sc;
-                const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+                // x sc (a power of two) and x sc - hi are exact in FP32: one
+                // fused multiply-add rounded once to fp16 gives each part
+                const float xr = avr[t][i] * d.x - avi[t][i] * d.y;
+                const float xi = avr[t][i] * d.y + avi[t][i] * d.x;
+                const _Float16 hr = (_Float16)fmaf(xr, sc, 0.0f), hi_ = (_Float16)fmaf(xi, sc, 0.0f);
                 arh[t][i] = hr;
-                arl[t][i] = (_Float16)(xr - (float)hr);
+                arl[t][i] = (_Float16)fmaf(xr, sc, -(float)hr);
                 aih[t][i] = hi_;
-                ail[t][i] = (_Float16)(xi - (float)hi_);
+                ail[t][i] = (_Float16)fmaf(xi, sc, -(float)hi_);
             }
         sfv4 acp[NB], acn[NB], aci[NB];
 #pragma unroll
