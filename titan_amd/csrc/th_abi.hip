@@ -241,6 +241,11 @@ long batch_pairs(th_handle *h, long total)
     const size_t per = (size_t)h->nA2 * 2 * h->Mpad * sizeof(double);
     const size_t budget = (size_t)h->batch_mb << 20;
     long b = (long)std::max<size_t>(TH_KC, budget / per);
+    // a budget of at least one full wave of the fp16 spectral Green kernel's
+    // workgroups is rounded to the nearest whole number of waves, so that no
+    // launch ends with a part-empty wave
+    const long qw = th_green_wave_pairs(h);
+    if (qw > 0 && b >= qw) b = ((b + qw / 2) / qw) * qw;
     return std::min(b, std::max<long>(total, 1));
 }
 

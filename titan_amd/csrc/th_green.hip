@@ -1732,6 +1732,27 @@ static const th_eig_set *side_eig(const th_handle *h, bool qside, const double *
     return nullptr;
 }
 
+long th_green_wave_pairs(th_handle *h)
+{
+    if (h->wave_pairs >= 0) return h->wave_pairs;
+    long q = 0;
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
+        hipError_t e = hipErrorInvalidValue;
+        if (h->d == 18)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, green_spectral_h2_kernel<18, TH_RUN>, 64 * 2, 0);
+        else if (h->d == 36)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, green_spectral_h2_kernel<36, TH_RUN>, 64 * 3, 0);
+        else if (h->d == 54)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, green_spectral_h2_kernel<54, TH_RUN>, 64 * 4, 0);
+        if (e == hipSuccess && occ > 0) q = (long)TH_RUN * ncu * occ;
+    }
+    (void)hipGetLastError();
+    h->wave_pairs = q;
+    return q;
+}
+
 bool th_green_presplit_ok(const th_handle *h, const double *q)
 {
     return h->presplit && h->kernel_variant == 0 && h->spectral_chunk > 0 && h->green_f16 &&

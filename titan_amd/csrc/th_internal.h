@@ -88,6 +88,7 @@ struct th_handle {
     int contract_f16 = 1;        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
     int presplit = 1;            // mixed spectral: Green function kernels write the fp16 splits themselves
     int last_presplit = 0;       // the last chi0 did
+    long wave_pairs = -1;        // pairs of one full wave of fp16 spectral workgroups (0: none; -1: not yet known)
     int green_f16 = 1;           // mixed: spectral products on fp16 MFMA with two-term splits
     int pair_order = 1;          // imaginary list sorted by k-point, real axis k-major (runs per k-point)
     long n_kuniq = 0;            // unique k-points of the imaginary list and the real mesh
@@ -206,6 +207,11 @@ int th_launch_green(th_handle *h, const th_green_args &a);
 // The Green function kernels of both sides of chi0(q) can write split
 // half-precision panels (th_green_args::presplit)
 bool th_green_presplit_ok(const th_handle *h, const double *q);
+// Pairs that fill every resident workgroup of the fp16 spectral Green kernel
+// once (TH_RUN pairs per workgroup x compute units x workgroups per unit), 0
+// when the handle's d has no such kernel; batches of whole multiples leave no
+// part-empty last wave
+long th_green_wave_pairs(th_handle *h);
 int th_launch_eig(th_handle *h, int which, const double *q);
 
 // Row of global pair b in panels stored per batch of bpc pairs with each
