@@ -258,7 +258,10 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    read from device memory; per-stage times are then not
  *                    split (only t[4] is set).  0 = plain launches (default)
  *   "splits"         K-splits of the contraction (workgroups per tile)
- *   "batch_mb"       panel memory per batch in MB (pairs per launch)
+ *   "batch_mb"       panel memory per batch in MB of FP64-sized panels (pairs
+ *                    per launch); a budget of at least one full wave of the
+ *                    fp16 spectral kernel's workgroups (d = 18, 36, 54) is
+ *                    rounded to the nearest whole number of waves
  * th_get_tuning reads the current values. */
 int th_tune(th_handle *h, const char *key, long value);
 int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_mb);
