@@ -1245,7 +1245,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
     constexpr int NB = (DD + 15) / 16, NT = 64 * NB, KT = (DD + 31) / 32, KP = 32 * KT, NC = 16 * NB;
     constexpr int VST = KP + 8;                  // LDS row stride in halves (16-byte reads on distinct banks)
     __shared__ __align__(16) _Float16 Vs[2][2][NC * VST];   // [re|im][hi|lo][c][m] = conj(V[c][m])
-    __shared__ float2 dv[2][KP];                 // diagonal of the pair, by pair parity
+    __shared__ float2 dv[CH][KP];                // diagonals of the workgroup's pairs
     __shared__ double pzr[CH], pzi[CH], pzw[CH]; // Re z, Im z and weight of the workgroup's pairs
     __shared__ int pkb[CH];                      // their eigen-decomposition
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1273,23 +1273,17 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
         pzw[tid] = w;
         pkb[tid] = (int)kb;
     }
-    // diagonal d = (ca Re f + cb Im f) + i cc Im f of pair jj into dv[buf] from
-    // the run's eigenvalue lam of this thread (m = tid)
-    auto diagonal = [&](int jj, int buf, double lam) {
-        if (tid < KP) {
-            const double w = pzw[jj];
-            double ca = 1.0, cb = 0.0, cc = 1.0;
-            if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }
-            if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
-            if (g.mode == TH_G_REAL_B) cc = -1.0;
-            float2 d = make_float2(0.0f, 0.0f);
-            if (tid < DD) {
-                const double ei = pzi[jj], a = pzr[jj] - lam, n = a * a + ei * ei;
-                const double fr = a / n, fi = -ei / n;
-                d = make_float2((float)(ca * fr + cb * fi), (float)(cc * fi));
-            }
-            dv[buf][tid] = d;
-        }
+    // entry m of the diagonal d = (ca Re f + cb Im f) + i cc Im f of pair jj,
+    // f = 1 / (z - lam) with lam the m-th eigenvalue of its k-point
+    auto diagonal = [&](int jj, double lam) {
+        const double w = pzw[jj];
+        double ca = 1.0, cb = 0.0, cc = 1.0;
+        if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }
+        if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
+        if (g.mode == TH_G_REAL_B) cc = -1.0;
+        const double ei = pzi[jj], a = pzr[jj] - lam, n = a * a + ei * ei;
+        const double fr = a / n, fi = -ei / n;
+        return make_float2((float)(ca * fr + cb * fi), (float)(cc * fi));
     };
     int rI[4], rR[4], cI[NB], cR[NB];
     bool rv[4], cv[NB];
@@ -1325,11 +1319,10 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
     // current node weights)
     const int eP = (g.presplit && g.pareal > 0.0) ? th_real_exp(g.p2, g.pnx, g.pareal) : g.pexp;
     long cur = -1;
-    double lam = 0.0;                           // eigenvalue tid of the current run
     float avr[KT][8], avi[KT][8];
     __syncthreads();                            // the pairs' scalars
     for (long pl = s0; pl < s1; ++pl) {
-        const int jj = (int)(pl - s0), par = (int)(pl & 1);
+        const int jj = (int)(pl - s0);
         const long kb = pkb[jj];
         const double ei = pzi[jj], w = pzw[jj];
         // diagonal coefficients: d = (ca Re f + cb Im f) + i cc Im f
@@ -1338,7 +1331,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
         if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
         if (g.mode == TH_G_REAL_B) cc = -1.0;
         if (kb != cur) {                        // same kb in every thread
-            __syncthreads();                    // the previous run's reads of Vs are done
+            __syncthreads();                    // the previous run's reads of Vs and dv are done
             const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
             for (int e = tid; e < NC * KP; e += NT) {
                 const int c = e / KP, m = e - (e / KP) * KP;
@@ -1364,15 +1357,17 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
                     avr[t][i] = (float)x.x;
                     avi[t][i] = (float)x.y;
                 }
-            lam = tid < DD ? g.eigL[(size_t)kb * DD + tid] : 0.0;
+            // the diagonals of every later pair of the workgroup on this k-point,
+            // so that the run needs no barrier per pair
+            const double *Lk = g.eigL + (size_t)kb * DD;
+            for (int e = tid; e < CH * KP; e += NT) {
+                const int j = e / KP, m = e - (e / KP) * KP;
+                if (j >= jj && s0 + j < s1 && pkb[j] == kb)
+                    dv[j][m] = m < DD ? diagonal(j, Lk[m]) : make_float2(0.0f, 0.0f);
+            }
             cur = kb;
-            diagonal(jj, par, lam);
+            __syncthreads();                    // the run's V and diagonals in LDS
         }
-        __syncthreads();                        // the pair's diagonal (and a new run's V) in LDS
-        // the next pair's diagonal, when it shares this run, into the other
-        // buffer (every wave has passed this pair's barrier, so the reads of
-        // that buffer for the previous pair are done)
-        if (pl + 1 < s1 && pkb[jj + 1] == cur) diagonal(jj + 1, par ^ 1, lam);
         // power-of-two scale of V diag(d): |d_m| <= max(|ca|, |cb|, |cc|) / Im z
         const double bound = fmax(fmax(fabs(ca), fabs(cb)), fabs(cc)) / fabs(ei);
         // bound = m 2^ex with m in [0.5, 1) (exponent bits; an infinite or zero
@@ -1385,7 +1380,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
         for (int t = 0; t < KT; ++t)
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const float2 d = dv[par][32 * t + kofs + i];
+                const float2 d = dv[jj][32 * t + kofs + i];
                 // x sc (a power of two) and x sc - hi are exact in FP32: one
                 // fused multiply-add rounded once to fp16 gives each part
                 const float xr = avr[t][i] * d.x - avi[t][i] * d.y;
