@@ -87,3 +87,26 @@ def test_sc_records_may_wrap(tmp_path):
     assert np.allclose(sc["mz"], v[:, 3]) and np.allclose(sc["delta_sc"], v[:, 4]) and sc["Ef"] == 0.25
     with pytest.raises(ValueError):
         D.read_sc_results(str(p), [3, 3, 3])
+
+
+def test_deck_without_sc_file(tmp_path):
+    """No self-consistency file: zero moments, E_F from the input key 'Ef'
+    (nanoribbon) or from the element file (Fe); SOC = F switches the
+    spin-orbit scale off as readElementFile does."""
+    import shutil
+    fe = tmp_path / "fe"
+    fe.mkdir()
+    for f in ("input", "basis", "Fe"):
+        shutil.copy(os.path.join(DECKS, "Fe_bulk_susc", f), fe / f)
+    d = D.load_deck(str(fe))
+    assert d["sc_file"] is None and d["system"]["Ef"] == 0.7415
+    assert d["system"]["mz"] == [[0.0] * 9]
+    text = (fe / "input").read_text().replace("-> SOC = T", "-> SOC = F")
+    (fe / "input").write_text(text)
+    assert D.load_deck(str(fe))["system"]["socscale"] == 0.0
+    rib = tmp_path / "rib"
+    rib.mkdir()
+    for f in ("input", "basis", "A2", "A3"):
+        shutil.copy(os.path.join(DECKS, "nanoribbon", f), rib / f)
+    r = D.load_deck(str(rib))
+    assert r["system"]["Ef"] == -0.25 and len(r["system"]["rho"]) == 10
