@@ -30,7 +30,7 @@ class _SysIn(C.Structure):
         ("onsite", _dp), ("sk", _dp), ("lam", _dp), ("U", _dp),
         ("rho", _dp), ("mx", _dp), ("my", _dp), ("mz", _dp), ("rho0", _dp),
         ("rhod0", _dp), ("mzd0", _dp), ("mpd0_re", _dp), ("mpd0_im", _dp),
-        ("socscale", C.c_double),
+        ("socscale", C.c_double), ("simplemix", C.c_int), ("maxel", C.c_int), ("nel", _ip), ("estage", _dp),
     ]
 
 
@@ -119,6 +119,41 @@ def lmat():
     return L.reshape(3, 9, 9).transpose(0, 2, 1).copy()   # [m][row][col]
 
 
+def element_shells(el, nstages, reltol):
+    """Shell distances of the atoms of an element file's own element
+    (readElementFile, TightBinding.F90:577-650): for each such atom, the
+    first nstages distinct distances to the other atoms of the file's
+    lattice, a new shell wherever a distance differs from the current one by
+    at least reltol times the first.  Computed here by brute force over a
+    cell range twice the one the reference uses (the first nstages shells
+    lie inside both)."""
+    a = np.asarray(el["a"], dtype=float).reshape(3, 3)
+    pos = np.asarray(el["pos"], dtype=float).reshape(-1, 3)
+    sd = int(el["sysdim"])
+    n = 2 * nstages
+    out = []
+    for l, p in enumerate(pos):
+        if not el["lelement"][l]:
+            continue
+        ds = []
+        for c in np.ndindex(*([2 * n + 1] * sd)):
+            R = sum((ci - n) * a[x] for x, ci in enumerate(c)) if sd else np.zeros(3)
+            for q in pos:
+                d = np.linalg.norm(R + q - p)
+                if d > 1e-9:
+                    ds.append(d)
+        ds = np.sort(ds)
+        shells = [ds[0]]
+        for d in ds[1:]:
+            if len(shells) == nstages:
+                break
+            if abs(d - shells[-1]) >= shells[0] * reltol:
+                shells.append(d)
+        shells += [1e12] * (nstages - len(shells))
+        out.append(shells)
+    return out
+
+
 class OracleSystem:
     """Built tight-binding system held by the oracle library."""
 
@@ -164,6 +199,17 @@ class OracleSystem:
         si.mpd0_re = _d(arr(mpd0.real))
         si.mpd0_im = _d(arr(mpd0.imag))
         si.socscale = float(spec.get("socscale", 1.0))
+        si.simplemix = 1 if spec.get("simplemix") else 0
+        rows = [element_shells(t["element"], nS, si.relTol) if t.get("element") else [] for t in types]
+        if any(len(r) for r in rows):
+            maxel = max(len(r) for r in rows)
+            est = np.full((len(types), maxel, nS), 1e12)
+            for ti, r in enumerate(rows):
+                for l, d in enumerate(r):
+                    est[ti, l, :] = d
+            si.maxel = maxel
+            si.nel = _i(arr([len(r) for r in rows], np.int32))
+            si.estage = _d(arr(est))
         self._in = si
         self._s = _Sys()
         rc = lib().or_build_system(C.byref(si), C.byref(self._s))

@@ -257,6 +257,14 @@ typedef struct {
     const double *rho0;    /* nA*nOrb reference occupations (0 when Un=0) */
     const double *rhod0, *mzd0, *mpd0_re, *mpd0_im; /* nA reference d sums */
     double socscale;
+    int simplemix;         /* 1: arithmetic mean of the two types' integrals */
+    /* Andersen scaling (get_SK_parameter, TightBinding.F90:144-162): nel[t]
+     * element-file atoms of type t's own element, each with its nStages shell
+     * distances estage[((t*maxel) + l)*nStages + k]; nel NULL or nel[t] = 0:
+     * no scaling of type t */
+    int maxel;
+    const int *nel;
+    const double *estage;
 } or_sys_in;
 
 /* Hop table: H(k)[row block jb, col block ib] += exp(i k.R) t (both spins).
@@ -373,10 +381,28 @@ int or_build_system(const or_sys_in *in, or_sys *out)
             for (int x = 0; x < 3; ++x) dc[x] = (posl[3 * s + x] - in->pos[3 * i + x]) / dd;
             const double *t1 = in->sk + ((size_t)in->type[i] * nS + stage) * 10;
             const double *t2 = in->sk + ((size_t)in->type[j] * nS + stage) * 10;
+            /* distance exponents l + l' + 1 of the ten integrals */
+            static const double expon[10] = {1, 3, 3, 5, 5, 5, 2, 3, 4, 4};
+            double sc[2] = {1.0, 1.0};
+            for (int side = 0; side < 2; ++side) {
+                const int ty = in->type[side ? j : i];
+                if (!in->nel || in->nel[ty] <= 0) continue;
+                double best = 0.0;   /* element atom whose shell ratio is closest to 1 */
+                for (int l = 0; l < in->nel[ty]; ++l) {
+                    double r = in->estage[((size_t)ty * in->maxel + l) * nS + stage] / dd;
+                    if (fabs(r - 1.0) < fabs(best - 1.0)) best = r;
+                }
+                sc[side] = best;
+            }
             double mix[10], bb[81];
             for (int p = 0; p < 10; ++p) {
-                double g = sqrt(fabs(t1[p]) * fabs(t2[p]));
-                mix[p] = (t1[p] + t2[p] >= 0.0) ? g : -g;  /* Fortran sign(a,b) */
+                const double a1 = t1[p] * pow(sc[0], expon[p]), a2 = t2[p] * pow(sc[1], expon[p]);
+                if (in->simplemix) {
+                    mix[p] = 0.5 * (a1 + a2);
+                } else {
+                    double g = sqrt(fabs(a1) * fabs(a2));
+                    mix[p] = (a1 + a2 >= 0.0) ? g : -g;  /* Fortran sign(a,b) */
+                }
             }
             or_intd(mix, dc, bb);
             for (int x = 0; x < 3; ++x) out->R[3 * nh + x] = cellv[3 * s + x];

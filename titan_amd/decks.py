@@ -115,17 +115,39 @@ def read_basis(path):
     return {"name": name, "a0": a0, "a": a, "elements": elements, "counts": counts, "coord": coord, "pos": pos}
 
 
-def read_element(path, nstages):
+def read_element(path, nstages, elname=None):
     """{"onsite" [9] (s, px, py, pz, dxy, dyz, dzx, dx2, dz2), "sk" [nstages][10]
     (sss pps ppp dds ddp ddd sps sds pds pdp), "lambda" [p, d], "U" [Un, Um],
-    "fermi_level", "sysdim", "occupations", "a0", "bravais", "name"}."""
+    "fermi_level", "sysdim", "occupations", "a0", "bravais", "name",
+    "element": {"a", "pos", "sysdim", "lelement"} (the file's own lattice; an
+    atom is of this element when its type name is elname, the file's base
+    name by default)}.  Two-centre lines missing at the end of the file read
+    as 0."""
+    if elname is None:
+        elname = os.path.basename(path)
     with open(path) as fh:
         raw = [l.rstrip("\n") for l in fh]
     name = raw[0].strip()
     lines = _content(raw[1:])
     a0 = _floats(lines[0], 1)[0]
     bravais = [_floats(lines[1 + i], 3) for i in range(3)]
+    tnames = []
+    for w in lines[4].split():
+        if w.startswith("!"):
+            break
+        tnames.append(w)
     counts = [int(x) for x in lines[5].split() if x.lstrip("-").isdigit()]
+    coord = lines[6].split()[0]
+    avec = [[x * a0 for x in row] for row in bravais]
+    epos, lel = [], []
+    for t, c in enumerate(counts):
+        for _ in range(c):
+            p = _floats(lines[7 + len(epos)], 3)
+            if coord[0] in "CcKk":
+                epos.append([x * a0 for x in p])
+            else:
+                epos.append([sum(p[m] * avec[m][x] for m in range(3)) for x in range(3)])
+            lel.append(t < len(tnames) and tnames[t] == elname)
     i = 7 + sum(counts)                    # after coordinate type and positions
     sysdim = int(_floats(lines[i], 1)[0])
     i += 1
@@ -147,9 +169,13 @@ def read_element(path, nstages):
     onsite = [fortran_float(on[0].split()[2])] + shell(on[1], 2, 3) + shell(on[2], 3, 3) + shell(on[3], 4, 2)
     if nstages > nfile:
         raise ValueError("%s: %d neighbour stages requested, %d in the file" % (path, nstages, nfile))
-    sk = [[fortran_float(lines[i + 10 + 10 * st + p].split()[3]) for p in range(10)] for st in range(nstages)]
+    def sk_value(k):
+        return fortran_float(lines[k].split()[3]) if k < len(lines) else 0.0
+
+    sk = [[sk_value(i + 10 + 10 * st + p) for p in range(10)] for st in range(nstages)]
     return {"name": name, "onsite": onsite, "sk": sk, "lambda": lam, "U": [u[0], u[0]] if len(u) == 1 else u,
-            "fermi_level": fermi, "sysdim": sysdim, "occupations": occ, "a0": a0, "bravais": bravais}
+            "fermi_level": fermi, "sysdim": sysdim, "occupations": occ, "a0": a0, "bravais": bravais,
+            "element": {"a": avec, "pos": epos, "sysdim": sysdim, "lelement": lel}}
 
 
 def read_sc_results(path, norbs):
@@ -199,20 +225,81 @@ def orbitals(value):
     return out
 
 
+def read_initial_occupations(path, nOrb):
+    """TITAN's initial (paramagnetic) occupation file of an element with
+    Un != 0 (read_init_expecs, mod_initial_expectation.F90:473-534):
+    {"rho0" [nOrb], "rhod0", "mpd0" (complex), "mzd0", "U" [Un, Um]}."""
+    with open(path) as fh:
+        vals = [v for line in fh for v in _floats(line)]
+    if len(vals) < nOrb + 6:
+        raise ValueError("%s: expected %d values" % (path, nOrb + 6))
+    r = vals[nOrb:nOrb + 4]
+    return {"rho0": vals[:nOrb], "rhod0": r[0], "mpd0": complex(r[1], r[2]), "mzd0": r[3],
+            "U": vals[nOrb + 4:nOrb + 6]}
+
+
+def mesh_from_input(nkpt_words, sysdim):
+    """(points per dimension, total) of the 'nkpt' key (mod_io.F90:99-121):
+    one value per dimension, or one total rounded up to a cube/square."""
+    vals = [int(fortran_float(w)) for w in nkpt_words]
+    if len(vals) == sysdim:
+        dims = tuple(vals) + (1,) * (3 - sysdim)
+    elif len(vals) == 1:
+        from titan_amd.contour import mesh_dims
+        dims = mesh_dims(vals[0], sysdim)
+    else:
+        raise ValueError("'nkpt' has %d values (expected 1 or sysdim = %d)" % (len(vals), sysdim))
+    total = 1
+    for n in dims[:sysdim]:
+        total *= n
+    return dims, total
+
+
 def load_deck(directory, sc_file=None):
     """{"system": TBModel parameters, "run": chi(omega) run parameters} of a
-    TITAN run directory.  The ground state comes from sc_file, or the single
-    results/*SOC/selfconsistency/selfconsistency_*.dat of the directory, or
-    is left at zero with E_F from the input key 'Ef' or the element file."""
+    TITAN run directory, set up as the reference does before its chi(omega)
+    loop:
+      - element files with their own lattices: integrals scaled from the
+        element's neighbour distances to the system's (get_SK_parameter,
+        TightBinding.F90:144-162, in titan_amd.system);
+      - on-site energies shifted to the Fermi level of type 'fermi_layer'
+        (TightBinding.F90:109-115);
+      - ground state from sc_file, or the single results/*SOC/selfconsistency/
+        selfconsistency_*.dat of the directory (read_previous_results), and
+        the input key 'Ef' overriding its E_F (mod_self_consistency.F90:76);
+        types with Un != 0 also need their initial occupations
+        (results*/FSOC/selfconsistency/initialrho_<type>_*.dat);
+      - 'relTol', 'nkpt' (one value or one per dimension) and 'minimumBZmesh'
+        (default 1000, at most the mesh) as mod_io.F90:91-141 reads them.
+    Decks this package cannot model raise ValueError: DFT element files
+    (tbmode 2), superconductivity, an external field, missing initial
+    occupations for Un != 0, a type absent from its element file."""
     inp = read_input(os.path.join(directory, "input"))
+    if int(_get(inp, "tbmode", "1")) != 1:
+        raise ValueError("tbmode 2 (DFT Hamiltonians) is not supported")
+    if _flag(_get(inp, "superCond")):
+        raise ValueError("superconducting runs (superCond = T) are not supported")
+    if _flag(_get(inp, "FIELD")):
+        raise ValueError("runs with an external field (FIELD = T) are not supported")
+    options = (_get(inp, "Options", "") or "").split()
     nst = int(_get(inp, "nn_stages", "2"))
+    reltol = fortran_float(_get(inp, "relTol", "0.05"))
     basis = read_basis(os.path.join(directory, "basis"))
-    types = [read_element(os.path.join(directory, e), nst) for e in basis["elements"]]
+    types = [read_element(os.path.join(directory, e), nst, e) for e in basis["elements"]]
+    for e, t in zip(basis["elements"], types):
+        if not any(t["element"]["lelement"]):
+            raise ValueError("element %s not found in its element file" % e)
     type_of = [t for t, c in enumerate(basis["counts"]) for _ in range(c)]
     orbs = orbitals(_get(inp, "orbitals"))
     nA, nO = len(type_of), len(orbs)
+    sysdim = int(_get(inp, "sysdim", "3"))
     soc = _get(inp, "SOC")
     socscale = fortran_float(_get(inp, "socscale", "1")) if (soc is None or _flag(soc)) else 0.0
+    # common Fermi level of the element files (get_SK_parameter)
+    fl = int(_get(inp, "fermi_layer", "1"))
+    ef = types[fl - 1]["fermi_level"]
+    for t in types:
+        t["onsite"] = [v - t["fermi_level"] + ef for v in t["onsite"]]
     if sc_file is None:
         socdir = "TSOC" if (soc is None or _flag(soc)) else "FSOC"
         found = sorted(glob.glob(os.path.join(directory, "results*", socdir, "selfconsistency",
@@ -223,16 +310,48 @@ def load_deck(directory, sc_file=None):
         ef = sc["Ef"]
     else:
         sc = {k: [[0.0] * nO for _ in range(nA)] for k in ("rho", "mx", "my", "mz")}
-        ef = fortran_float(_get(inp, "Ef")) if _get(inp, "Ef") else types[0]["fermi_level"]
+    if _get(inp, "Ef") is not None:
+        ef = fortran_float(_get(inp, "Ef"))
+    rho0 = [[0.0] * nO for _ in range(nA)]
+    rhod0, mzd0, mpd0 = [0.0] * nA, [0.0] * nA, [0.0] * nA
+    for ti, (e, t) in enumerate(zip(basis["elements"], types)):
+        if abs(t["U"][0]) < 1e-8:
+            continue
+        found = sorted(glob.glob(os.path.join(directory, "results*", "FSOC", "selfconsistency",
+                                              "initialrho_%s_*.dat" % e)))
+        if len(found) != 1:
+            raise ValueError("type %s has Un != 0: its initial occupation file "
+                             "(results/FSOC/selfconsistency/initialrho_%s_*.dat) is needed" % (e, e))
+        ini = read_initial_occupations(found[0], nO)
+        for i in range(nA):
+            if type_of[i] == ti:
+                rho0[i], rhod0[i], mzd0[i], mpd0[i] = ini["rho0"], ini["rhod0"], ini["mzd0"], ini["mpd0"]
     eta = fortran_float(_get(inp, "eta", "1e-3"))
-    system = {"name": basis["name"], "sysdim": int(_get(inp, "sysdim", types[0]["sysdim"])), "a": basis["a"],
-              "pos": basis["pos"], "type": type_of, "orbs": orbs, "nStages": nst, "relTol": 0.05,
-              "types": [{k: t[k] for k in ("onsite", "sk", "lambda", "U")} for t in types],
-              "rho": sc["rho"], "mx": sc["mx"], "my": sc["my"], "mz": sc["mz"], "Ef": ef, "socscale": socscale}
+    dims, total = mesh_from_input((_get(inp, "nkpt", "0") or "0").split(), sysdim)
+    minimum = int(fortran_float(_get(inp, "minimumBZmesh", "1000")))
+    system = {"name": basis["name"], "sysdim": sysdim, "a": basis["a"],
+              "pos": basis["pos"], "type": type_of, "orbs": orbs, "nStages": nst, "relTol": reltol,
+              "types": [{k: t[k] for k in ("onsite", "sk", "lambda", "U", "element")} for t in types],
+              "rho": sc["rho"], "mx": sc["mx"], "my": sc["my"], "mz": sc["mz"], "Ef": ef, "socscale": socscale,
+              "rho0": rho0, "rhod0": rhod0, "mzd0": mzd0, "mpd0": mpd0, "simplemix": "simplemix" in options}
     run = {"eta": eta, "etap": fortran_float(_get(inp, "etap", _get(inp, "eta", "1e-3"))),
-           "nkpt": int(_get(inp, "nkpt", "0")), "parts": int(_get(inp, "parts", "2")),
+           "nkpt": total, "nk": dims, "minimumBZmesh": min(minimum, total),
+           "parts": int(_get(inp, "parts", "2")),
            "n1gl": int(_get(inp, "n1gl", "64")), "parts3": int(_get(inp, "parts3", "1")),
            "n3gl": int(_get(inp, "n3gl", "64")), "emin": fortran_float(_get(inp, "emin", "0")),
            "emax": fortran_float(_get(inp, "emax", "0")), "nEner": int(_get(inp, "nEner", "1")),
            "q": [0.0, 0.0, 0.0]}
     return {"system": system, "run": run, "input": inp, "sc_file": sc_file}
+
+
+def work_lists(model, run):
+    """The imaginary-axis (E, k) list and the real-axis k-mesh of a deck's run,
+    as TITAN builds them: per imaginary energy the adaptive mesh of
+    generateAdaptiveMeshes (adaptiveMesh.F90:23-74, 199-218) with
+    'minimumBZmesh', and the 'nkpt' mesh on the real axis (realBZ).
+    Returns ((k, y, w), (k_real, w_real))."""
+    from titan_amd import contour as Ct
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    imag = Ct.imag_work_list(model, y, wy, run["nkpt"], run.get("minimumBZmesh"))
+    kp, wk = Ct.uniform_kmesh(model.b, *run["nk"])
+    return imag, (kp, wk)

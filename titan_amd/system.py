@@ -18,11 +18,18 @@ element, basis and self-consistency files; energies in Ry):
   nStages, relTol, types[{onsite[9], sk[nStages][10], lambda[2], U[2]}],
   rho/mx/my/mz [nA][nOrb], optional rho0, rhod0, mzd0, mpd0, socscale, Ef.
 SK integral order: sss, pps, ppp, dds, ddp, ddd, sps, sds, pds, pdp.
+A type may carry "element": the lattice of its element file (a0-scaled
+Bravais rows "a", positions "pos", "sysdim", and "lelement", which of its
+atoms are this element); its integrals are then scaled from the element
+file's neighbour distances to the system's (Andersen's law, see
+scale_factor).
 """
 import numpy as np
 
 R3 = np.sqrt(3.0)
 SK_NAMES = ("sss", "pps", "ppp", "dds", "ddp", "ddd", "sps", "sds", "pds", "pdp")
+# distance exponents l + l' + 1 of the ten integrals (TightBinding.F90:105)
+SK_EXPON = np.array([1.0, 3.0, 3.0, 5.0, 5.0, 5.0, 2.0, 3.0, 4.0, 4.0])
 _ANG = np.array([0, 1, 1, 1, 2, 2, 2, 2, 2])
 
 
@@ -94,6 +101,49 @@ def l_matrices():
     return np.array([0.5 * (Lp + Lm), -0.5j * (Lp - Lm), Lz])
 
 
+def shell_distances(a, pos, sysdim, nstages, reltol):
+    """Neighbour-shell distances seen from each basis atom: [nstages][nA],
+    1e12 where fewer shells exist (Lattice.F90:40-232 for the system,
+    readElementFile, TightBinding.F90:577-650, for an element file): the
+    (2 nstages + 1)^sysdim cells around the origin, distances sorted, a new
+    shell wherever a distance differs from the current shell's by at least
+    reltol times the first shell's."""
+    a = np.asarray(a, dtype=float).reshape(3, 3)
+    pos = np.asarray(pos, dtype=float).reshape(-1, 3)
+    rng = np.arange(-nstages, nstages + 1)
+    grids = [rng if k < sysdim else np.zeros(1, dtype=int) for k in range(3)]
+    cells = np.array([(i, j, k) for k in grids[2] for j in grids[1] for i in grids[0]], dtype=float)
+    sites = (cells @ a)[:, None, :] + pos[None, :, :]
+    sites = sites.reshape(-1, 3)
+    dist = np.linalg.norm(sites[:, None, :] - pos[None, :, :], axis=2)
+    out = np.full((nstages, len(pos)), 1e12)
+    for k in range(len(pos)):
+        dk = np.sort(dist[:, k][dist[:, k] > 1e-9])
+        out[0, k] = dk[0]
+        l = 0
+        for v in dk[1:]:
+            if abs(v - out[l, k]) < out[0, k] * reltol:
+                continue
+            l += 1
+            if l >= nstages:
+                break
+            out[l, k] = v
+    return out
+
+
+def scale_factor(stage_rows, stage, dist):
+    """Andersen's scaling of a type's integrals to a neighbour at distance
+    dist in shell 'stage' (get_SK_parameter, TightBinding.F90:144-162): of
+    the element file's atoms of this element, the ratio (its shell distance)
+    / dist closest to 1 (0 when none is closer to 1 than 0 is)."""
+    best = 0.0
+    for row in stage_rows:
+        t = row[stage] / dist
+        if abs(t - 1.0) < abs(best - 1.0):
+            best = t
+    return best
+
+
 def reciprocal(a, sysdim):
     a = np.asarray(a, dtype=float)
     b = np.zeros((3, 3))
@@ -146,18 +196,7 @@ class TBModel:
         sites_j = np.tile(np.arange(nA), len(cellv))
         sites_pos = sites_R + self.pos[sites_j]
         dist = np.linalg.norm(sites_pos[:, None, :] - self.pos[None, :, :], axis=2)   # [site, k]
-        stages = np.full((nS, nA), 1e12)
-        for k in range(nA):
-            dk = np.sort(dist[:, k][dist[:, k] > 1e-9])
-            stages[0, k] = dk[0]
-            l = 0
-            for v in dk[1:]:
-                if abs(v - stages[l, k]) < stages[0, k] * self.relTol:
-                    continue
-                l += 1
-                if l >= nS:
-                    break
-                stages[l, k] = v
+        stages = shell_distances(self.a, self.pos, self.sysdim, nS, self.relTol)
         hops = []
         for s in range(len(sites_j)):
             j = sites_j[s]
@@ -168,7 +207,7 @@ class TBModel:
                 st = np.nonzero(np.abs(dd - stages[:, i]) < stages[0, i] * self.relTol)[0]
                 if len(st) == 0:
                     continue
-                hops.append((sites_R[s], j, i, int(st[0]), (sites_pos[s] - self.pos[i]) / dd))
+                hops.append((sites_R[s], j, i, int(st[0]), (sites_pos[s] - self.pos[i]) / dd, dd))
         self._hops = hops
         self.stages = stages
 
@@ -184,11 +223,27 @@ class TBModel:
         self.stage_of = np.zeros(n, dtype=np.int32)
         self.dircos = np.zeros((n, 3))
         self.t = np.zeros((n, nO, nO))
-        for h, (R, j, i, st, dc) in enumerate(self._hops):
+        # shell distances of each type's element file, for the scaling law
+        erows = []
+        for ty in types:
+            el = ty.get("element")
+            if el is None:
+                erows.append(None)
+                continue
+            sd = shell_distances(el["a"], el["pos"], int(el["sysdim"]), self.nStages, self.relTol)
+            erows.append([sd[:, l] for l in range(sd.shape[1]) if el["lelement"][l]])
+        for h, (R, j, i, st, dc, dd) in enumerate(self._hops):
             t1 = np.asarray(types[self.type[i]]["sk"][st], dtype=float)
             t2 = np.asarray(types[self.type[j]]["sk"][st], dtype=float)
-            g = np.sqrt(np.abs(t1) * np.abs(t2))
-            mix = np.where(t1 + t2 >= 0, g, -g)
+            if erows[self.type[i]] is not None:
+                t1 = t1 * scale_factor(erows[self.type[i]], st, dd) ** SK_EXPON
+            if erows[self.type[j]] is not None:
+                t2 = t2 * scale_factor(erows[self.type[j]], st, dd) ** SK_EXPON
+            if spec.get("simplemix"):
+                mix = 0.5 * (t1 + t2)
+            else:
+                g = np.sqrt(np.abs(t1) * np.abs(t2))
+                mix = np.where(t1 + t2 >= 0, g, -g)
             b = sk_table(mix, dc)
             self.R[h], self.jb[h], self.ib[h], self.stage_of[h], self.dircos[h] = R, j, i, st, dc
             self.t[h] = b[np.ix_(o, o)]          # t[nu (orbital of j), mu (orbital of i)]
