@@ -47,9 +47,13 @@ typedef struct th_handle th_handle;
 
 /* Precision modes of th_chi0 / th_chi_sweep. */
 #define TH_PREC_FP64   0     /* G, chi0 contraction and sums in FP64 */
-#define TH_PREC_MIXED  1     /* G inverted in FP64 and rounded once to FP32 GEMM panels,
-                                FP32 MFMA contraction with FP32 in-tile accumulation,
-                                FP64 contour sums across tiles and splits, FP64 Dyson */
+#define TH_PREC_MIXED  1     /* eigen-decomposition (or elimination) of H in FP64; the
+                                Green function products and the chi0 contraction on the
+                                fp16 matrix cores with two-term fp16 splits of FP32
+                                values (FP32 accumulation, FP32 MFMA where H is not
+                                Hermitian); FP64 contour sums across tiles and splits;
+                                the block chi0[S,S] on the support S of U recomputed in
+                                FP64 (see th_set_interaction); FP64 Dyson */
 
 /* Creates a handle on the current HIP device and uploads the tight-binding
  * model: the local Hamiltonian h0 (d x d complex, d = 2*nOrb*nA) built by
@@ -82,6 +86,19 @@ int th_set_real_kmesh(th_handle *h, long nk, const double *k, const double *w);
  * D = 4*nOrb*nOrb*nA (chiorb_hf, source/mod_susceptibilities.F90:8). */
 int th_chi0(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
             int n_x, const double *x2, const double *p2, int precision, double *chi0_out);
+
+/* Declares the interaction matrix U (D x D complex, column-major, Umatorb of
+ * build_identity_and_U_matrix, source/mod_susceptibilities.F90:106-159) that
+ * the Dyson step will use; NULL clears it.  No counterpart in the reference:
+ * the Dyson step chi = chi0 - chi0[:,S] U_S (I + chi0[S,S] U_S)^-1 chi0[S,:]
+ * multiplies an error of the block chi0[S,S] on the support S of U by the
+ * condition number of (I + chi0[S,S] U_S), which is large near the
+ * Goldstone/FMR pole.  With U declared, TH_PREC_MIXED th_chi0 recomputes that
+ * block in FP64 (where H is Hermitian and d is 18, 36 or 54: from the FP64
+ * eigenpairs of H(k) with the energy sum taken per k-point first), so the
+ * mixed chi keeps the accuracy of the mixed chi0 at any distance from the
+ * pole.  th_chi_sweep in TH_PREC_MIXED declares its own U. */
+int th_set_interaction(th_handle *h, const double *U);
 
 /* RPA Dyson step chi = (I + chi0 U)^-1 chi0 in FP64 on the GPU
  * (source/calculate_chi.F90:84-89; U = Umatorb of
@@ -258,6 +275,18 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    read from device memory; per-stage times are then not
  *                    split (only t[4] is set).  0 = plain launches (default)
  *   "splits"         K-splits of the contraction (workgroups per tile)
+ *   "cache_mb"       cap (MB) of this handle's omega-independent caches, the
+ *                    imaginary-axis panel cache and the eigenpair sets
+ *                    (0 = none: the panel cache may take 70 % and the
+ *                    eigenpairs 30 % of free HBM); a cache over the cap is
+ *                    not kept (panels recomputed per frequency, Green
+ *                    functions by elimination), so many handles can share HBM
+ *   "drop_caches"    1: frees the omega-independent imaginary-axis panel
+ *                    cache and forgets the eigenpairs, so that the next call
+ *                    computes them again (timing from a cold start)
+ *   "refine"         TH_PREC_MIXED with a declared U (th_set_interaction):
+ *                    1 = chi0[S,S] on the support of U recomputed in FP64
+ *                    (default); 0 = left as the mixed contraction gives it
  *   "batch_mb"       panel memory per batch in MB of FP64-sized panels (pairs
  *                    per launch); a budget of at least one full wave of the
  *                    fp16 spectral kernel's workgroups (d = 18, 36, 54) is
@@ -276,11 +305,13 @@ int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_m
  * thx_contract: uploads panels A, B of that layout, runs the contraction and
  *   returns the accumulator [nA*nA][2][Mpad][Mpad] (re block, im block).
  * thx_last_split: 1 when the last chi0 had the Green function kernels write
- *   split half-precision panels, else 0. */
+ *   split half-precision panels, else 0.
+ * thx_last_refined: 1 when the last chi0 recomputed chi0[S,S] in FP64. */
 int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, double omega, double Ef,
                     double eta, double etap, int prec, int n_x, const double *x2, const double *p2, double *out);
 int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out);
 int thx_last_split(th_handle *h);
+int thx_last_refined(th_handle *h);
 
 #ifdef __cplusplus
 }

@@ -114,3 +114,28 @@ def test_deck_without_sc_file(tmp_path):
         shutil.copy(os.path.join(DECKS, "nanoribbon", f), rib / f)
     r = D.load_deck(str(rib))
     assert r["system"]["Ef"] == -0.25 and len(r["system"]["rho"]) == 10
+
+
+def test_adaptive_mesh_sizes():
+    """Per-energy meshes of the imaginary axis (adaptiveMesh.F90:199-218):
+    N(y) = max(ceil(N / (y/y1)^sqrt(d)), minimumBZmesh) rounded up to a cube,
+    y1 the smallest y; weights of each energy sum to its contour weight; the
+    deck's meshes (nkpt = minimumBZmesh = 1000) are all 10^3."""
+    import math
+    from titan_amd import contour as Ct
+    from titan_amd.system import TBModel
+    d = D.load_deck(os.path.join(DECKS, "Fe_bulk_susc"))
+    m = TBModel(d["system"])
+    run = dict(d["run"])
+    (K, Y, W), (kp, wk) = D.work_lists(m, run)
+    y, wy = Ct.imag_epoints(run["parts"], run["n1gl"])
+    assert len(Y) == 1000 * len(y) and len(wk) == 1000
+    run.update({"nkpt": 4096, "nk": (16, 16, 16), "minimumBZmesh": 64})
+    (K, Y, W), _ = D.work_lists(m, run)
+    for e in range(len(y)):
+        sel = Y == y[e]
+        n = max(math.ceil(4096 / (y[e] / y[0]) ** math.sqrt(3)), 64)
+        edge = math.ceil(n ** (1 / 3) - 1e-9)
+        assert sel.sum() == edge ** 3
+        assert W[sel].sum() == pytest.approx(wy[e], rel=1e-12)
+    assert (Y == y[0]).sum() == 4096 and (Y == y[-1]).sum() == 64

@@ -500,6 +500,30 @@ def test_graph_sweep_equals_plain_sweep(cfg0, prec):
     h.close()
 
 
+@pytest.mark.parametrize("prec", [B.PREC_FP64, B.PREC_MIXED])
+def test_graph_sweep_fresh_handle_larger_real_batch(cfg0, prec):
+    """A graph sweep on a fresh handle whose first frequency is 0 and whose
+    real-axis batch (nodes x real k-points) is larger than the imaginary one:
+    the omega = 0 step is captured before any real-axis step needs bigger
+    panels, so its replay at the third frequency must still see live
+    buffers.  Compared bit for bit with a plain sweep on another handle."""
+    m, _ = cfg0
+    (K, Y, W), (kp, wk) = _work(m, 3, n1gl=4)
+    oms = [0.0, 3e-3, 0.0, 5e-3]
+    args = ([0.0, 0.0, 0.0], oms, m.Ef, 5e-4, 5e-4, 2, 12, m.Umat)   # 24 real nodes > 4 imaginary energies
+    out = []
+    for graph in (1, 0):
+        h = m.handle()
+        h.tune(graph=graph)
+        h.set_imag_points(K, Y, W)
+        h.set_real_kmesh(kp, wk)
+        print("fresh handle sweep, graph", graph, flush=True)
+        out.append(h.sweep(*args, precision=prec, full=True))
+        h.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_block_elimination_d288_matches_oracle():
     """d = 288 (16-site bcc supercell, the largest cell of the uneven batch):
     G by 2 x 2 block elimination with register-resident 144 x 144

@@ -6,8 +6,9 @@ sweeps are issued from a pool of host threads (the ctypes calls release the
 GIL), so the kernels of cells of different sizes run concurrently on the
 GPU and small cells fill the gaps of large ones.  Tile shapes are tuned once
 per distinct matrix size d on a sample of the first cell of that size
-(titan_amd.autotune) and applied to every cell of that size.  The library
-keeps every cell's buffers bounded (batch_mb per handle) so that 40 cells
+(titan_amd.autotune) and applied to every cell of that size.  Every cell's
+buffers are bounded: panels by batch_mb per handle, the omega-independent
+caches (panel cache, eigenpairs) by cache_mb per handle, so that 40 cells
 fit next to each other in HBM.
 """
 import concurrent.futures as cf
@@ -22,14 +23,14 @@ from titan_amd.system import TBModel
 
 
 class Cell:
-    def __init__(self, spec, run, kmesh=None, batch_mb=256):
+    def __init__(self, spec, run, kmesh=None, batch_mb=256, cache_mb=4096):
         self.spec, self.run = spec, run
         self.model = m = TBModel(spec)
         self.y, self.wy = Ct.imag_epoints(run["parts"], run["n1gl"])
         nk = tuple(kmesh if kmesh is not None else run["nk"])
         self.kp, self.wk = Ct.uniform_kmesh(m.b, *nk)
         self.h = m.handle()
-        self.h.tune(batch_mb=batch_mb)
+        self.h.tune(batch_mb=batch_mb, cache_mb=cache_mb)
         self.set_mesh(self.kp, self.wk)
 
     def set_mesh(self, kp, wk):
@@ -54,8 +55,8 @@ class Cell:
 class UnevenBatch:
     """cells = [(spec, run)]; kmesh overrides every cell's k-mesh (tests)."""
 
-    def __init__(self, cells, kmesh=None, batch_mb=256):
-        self.cells = [Cell(s, r, kmesh, batch_mb) for s, r in cells]
+    def __init__(self, cells, kmesh=None, batch_mb=256, cache_mb=4096):
+        self.cells = [Cell(s, r, kmesh, batch_mb, cache_mb) for s, r in cells]
 
     def autotune(self, precision=B.PREC_MIXED, sample=8, log=None):
         """One tuning per distinct d, on every sample-th k-point of the first

@@ -20,7 +20,7 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_expectation_values", "th_current_response", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
+           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_expectation_values", "th_current_response", "th_set_interaction", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
            "th_tune", "th_get_tuning")
 
 
@@ -51,6 +51,7 @@ def load():
     L.th_expectation_values.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
     L.th_current_response.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                       _dp, _dp, _dp, C.c_void_p, _dp]
+    L.th_set_interaction.argtypes = [C.c_void_p, C.c_void_p]
     L.th_destroy.argtypes = [C.c_void_p]
     L.th_last_error.restype = C.c_char_p
     L.th_limits.argtypes = [_ip, _ip, _ip]
@@ -142,6 +143,13 @@ class Handle:
                             out.ctypes.data_as(_dp))
         _check(rc, "th_chi0")
         return out
+
+    def set_interaction(self, U):
+        """Declares U (D x D) so that TH_PREC_MIXED chi0 recomputes its block on
+        the support of U in FP64 (None clears it)."""
+        u = _c128(U) if U is not None else None
+        _check(load().th_set_interaction(self._h, u.ctypes.data_as(C.c_void_p) if u is not None else None),
+               "th_set_interaction")
 
     def dyson(self, chi0, U):
         a = _c128(chi0)

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libtitan_hip.so")
-SOURCES = ["th_green.hip", "th_gemm.hip", "th_dyson.hip", "th_adjoint.hip", "th_traces.hip", "th_density.hip", "th_current.hip", "th_abi.hip"]
+SOURCES = ["th_green.hip", "th_gemm.hip", "th_dyson.hip", "th_adjoint.hip", "th_traces.hip", "th_density.hip", "th_current.hip", "th_refine.hip", "th_abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

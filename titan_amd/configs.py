@@ -9,11 +9,30 @@ with it.  The seed is stored in the dict.
         16 omega in [0, 0.01] Ry, eta = 5e-4 Ry
   cfg1  fcc(001) slab, 8 sites (144 x 144 H), layer-dependent on-site
         shifts, 128^2 k, 64 + 32 energies, 32 omega
-  cfg2  bcc supercell, 3 sites (54 x 54 H), 48^3 k, 64 + 32 energies,
-        64 omega clustered near the transverse (FMR) pole
+  cfg2  bcc supercell, 3 sites (54 x 54 H) of the cfg0 element with U_d =
+        3 eV, 48^3 k, 64 + 32 energies, 64 omega clustered at the
+        transverse (FMR) pole of its self-consistent ground state
+        (omega_grid)
   cfg3  cfg0 system at 24^3 k, 32 omega (adjoint workload)
+
+A seeded moment is not self-consistent, so the RPA susceptibility of a
+seeded state has no Goldstone/FMR pole (cfg0 keeps its seeded m = 2.2 muB
+at U_d = 1 eV, as specified).  With U_d = 1 eV the seeded element's
+self-consistent state is nonmagnetic (checked on the GPU at 48^3 k-points:
+m = 0, Dyson condition numbers below 10 from -0.3 to 0.5 Ry), so cfg2, whose
+frequencies must sit at the pole, takes U_d = 3 eV, where the element is a
+self-consistent ferromagnet (3.85 muB of d moment per atom at cfg2's mesh).  sc_states.json (next
+to this module) holds the self-consistent d moments and Fermi level that
+titan_amd.selfconsistency computes on the GPU for cfg2's own k-mesh and
+contour (python -m titan_amd.selfconsistency --config 2), and the pole it
+then finds; system(cfg) applies them where present.
 """
+import json
+import os
+
 import numpy as np
+
+_SC_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sc_states.json")
 
 RY2EV = 13.6
 
@@ -168,22 +187,74 @@ def uneven_batch(ncells=40, seed=4, sizes=None):
     return cells
 
 
-def system(cfg):
+def seeded_system(cfg):
+    """The seeded system of a configuration (moments as drawn, E_F at the
+    filling on an 8^3 mesh)."""
     if cfg in (0, 3):
         return bcc_1site()
     if cfg == 1:
         return fcc001_slab()
     if cfg == 2:
-        return bcc_supercell(3)
+        return bcc_supercell(3, seed=0, U_ev=3.0)
     raise ValueError("unknown configuration %r" % cfg)
 
 
+def filling(cfg):
+    """Electrons per cell of a configuration (8 per site)."""
+    return 8.0 * {0: 1, 1: 8, 2: 3, 3: 1}[cfg]
+
+
+def sc_state(cfg):
+    """Self-consistent state of a configuration from sc_states.json, or None."""
+    key = str(cfg)
+    if not os.path.exists(_SC_PATH):
+        return None
+    with open(_SC_PATH) as fh:
+        return json.load(fh).get(key)
+
+
+def system(cfg):
+    """The configuration's system: seeded parameters with the self-consistent
+    d moments and Fermi level of sc_states.json where present."""
+    spec = seeded_system(cfg)
+    st = sc_state(cfg)
+    if st is None:
+        return spec
+    dorb = [k for k, o in enumerate(spec["orbs"]) if o >= 5]
+    mz = np.asarray(spec["mz"], dtype=float).copy()
+    for i, m in enumerate(st["mz_d"]):
+        mz[i, :] = 0.0
+        mz[i, dorb] = m / len(dorb)
+    spec["mz"] = mz.tolist()
+    spec["Ef"] = float(st["Ef"])
+    spec["name"] += " (self-consistent)"
+    return spec
+
+
+def clustered_omegas(center, half_width, n=64, span=16.0, omax=0.01):
+    """n frequencies in (0, omax] clustered at a resonance: the quantiles
+    (k + 1/2)/n of a Lorentzian of the given centre and half width, cut to
+    [center - span*half_width, center + span*half_width] and to (0, omax]."""
+    lo = max(center - span * half_width, 1e-3 * half_width)
+    hi = min(center + span * half_width, omax)
+    a, b = np.arctan((lo - center) / half_width), np.arctan((hi - center) / half_width)
+    u = (np.arange(n) + 0.5) / n
+    return center + half_width * np.tan(a + u * (b - a))
+
+
 def omega_grid(cfg, spec=None):
-    """Frequencies of a configuration (Ry).  cfg2 uses 64 nonzero
-    frequencies up to 0.01 Ry."""
+    """Frequencies of a configuration (Ry).  cfg2: 64 frequencies clustered
+    at the transverse resonance of its self-consistent state
+    (clustered_omegas at the pole and half width of sc_states.json: the
+    maximum of |Im chi^{+-}| found by titan_amd.selfconsistency.locate_pole
+    with FP64 sweeps); without a stored state, 64 uniform frequencies up to
+    0.01 Ry."""
     r = run_params(cfg)
     if r["omega"] is not None:
         return np.asarray(r["omega"], dtype=float)
+    st = sc_state(cfg)
+    if st is not None and "omega_pole" in st:
+        return clustered_omegas(st["omega_pole"], st["half_width"])
     return np.linspace(0.01 / 64, 0.01, 64)
 
 

@@ -198,6 +198,25 @@ static int rebuild_kunion(th_handle *h)
         TH_HIP(hipMalloc(&h->d_kid_re, sizeof(int) * kid_re.size()));
         TH_HIP(hipMemcpy(h->d_kid_re, kid_re.data(), sizeof(int) * kid_re.size(), hipMemcpyHostToDevice));
     }
+    // imaginary pairs (stored order) and real-mesh points of each unique k-point
+    hipFree(h->d_kim_ptr); hipFree(h->d_kim_idx); hipFree(h->d_kre_ptr); hipFree(h->d_kre_idx);
+    h->d_kim_ptr = h->d_kim_idx = h->d_kre_ptr = h->d_kre_idx = nullptr;
+    if (h->n_kuniq > 0) {
+        auto csr = [&](const std::vector<int> &kid, int **dptr, int **didx) -> int {
+            std::vector<int> ptr(h->n_kuniq + 1, 0), idx(std::max<size_t>(kid.size(), 1), 0);
+            for (int u : kid) ++ptr[u + 1];
+            for (long u = 0; u < h->n_kuniq; ++u) ptr[u + 1] += ptr[u];
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (size_t p = 0; p < kid.size(); ++p) idx[fill[kid[p]]++] = (int)p;
+            TH_HIP(hipMalloc(dptr, sizeof(int) * ptr.size()));
+            TH_HIP(hipMalloc(didx, sizeof(int) * idx.size()));
+            TH_HIP(hipMemcpy(*dptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+            TH_HIP(hipMemcpy(*didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+            return TH_OK;
+        };
+        if (int rc = csr(kid_im, &h->d_kim_ptr, &h->d_kim_idx)) return rc;
+        if (int rc = csr(kid_re, &h->d_kre_ptr, &h->d_kre_idx)) return rc;
+    }
     return TH_OK;
 }
 
@@ -210,6 +229,8 @@ int th_ensure_eig(th_handle *h, const double *q)
     if (!h->spectral || !h->hermitian || (h->d != 18 && h->d != 36 && h->d != 54) || h->n_kuniq <= 0) return TH_OK;
     const bool qz = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
     const size_t bytes = (size_t)h->n_kuniq * h->d * (h->d * sizeof(double2) + sizeof(double));
+    // the handle's cap on its caches (both eigenpair sets count against it)
+    if (h->cache_mb > 0 && (qz ? 1 : 2) * bytes > ((size_t)h->cache_mb << 20)) return TH_OK;
     for (int which = 0; which < (qz ? 1 : 2); ++which) {
         th_eig_set &es = h->eig[which];
         const double qq[3] = {which ? q[0] : 0.0, which ? q[1] : 0.0, which ? q[2] : 0.0};
@@ -267,8 +288,15 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
     const size_t need = (size_t)h->n_im * h->nA2 * 2 * h->Mpad * prec_size(prec);
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return TH_OK;
-    // the cache may take most of HBM (288 GB on MI355X); otherwise recompute per batch
+    // the cache may take most of HBM (288 GB on MI355X); otherwise recompute per
+    // batch.  With a per-handle cap, the eigenpairs already held count against it.
     if (need > (size_t)(0.7 * (double)freeb)) return TH_OK;
+    if (h->cache_mb > 0) {
+        size_t held = 0;
+        for (const th_eig_set &es : h->eig)
+            if (es.V) held += (size_t)h->n_kuniq * h->d * (h->d * sizeof(double2) + sizeof(double));
+        if (need + held > ((size_t)h->cache_mb << 20)) return TH_OK;
+    }
     if (hipMalloc(&h->d_bcache, need) != hipSuccess) {
         h->d_bcache = nullptr;
         (void)hipGetLastError();
@@ -309,8 +337,9 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
 // device, the omega-independent B cache, panel buffers.  Issues no kernel of
 // the contraction itself.
 int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double etap, int n_x, const double *x2,
-                 const double *p2, int prec, stage_timer &tm, bool *real_axis, bool *have_b)
+                 const double *p2, int prec, stage_timer &tm, bool *real_axis, bool *have_b, bool *refine_on)
 {
+    *refine_on = false;
     if (!h->d_kim || h->n_im <= 0) {
         th_set_error("th_set_imag_points must be called before th_chi0");
         return TH_E_STATE;
@@ -337,6 +366,7 @@ int chi0_prepare(th_handle *h, const double *q, double omega, double Ef, double 
     const bool presplit = prec == TH_PREC_MIXED && h->contract_f16 && h->hermitian && etap > 0.0 &&
                           std::isfinite(etap) && th_green_presplit_ok(h, q);
     if ((rc = ensure_bcache(h, Ef, etap, prec, presplit, tm, have_b))) return rc;
+    if ((rc = th_refine_prepare(h, q, prec, refine_on))) return rc;
     long need = batch_pairs(h, h->n_im);
     if (*real_axis) need = std::max(need, batch_pairs(h, (long)n_x * h->nk_re));
     if ((rc = th_ensure_part(h, th_contract_splits(h, need, prec)))) return rc;
@@ -379,7 +409,7 @@ bool contract_scales(th_handle *h, double eta, double etap, int n_x, th_contract
 // handle's stream, no host synchronisation, no allocation: capturable).
 // omega_p, when set, points to omega in device memory.
 int chi0_launch(th_handle *h, const double *q, double omega, const double *omega_p, double Ef, double eta,
-                double etap, int n_x, bool real_axis, bool have_b, int prec, stage_timer *tm)
+                double etap, int n_x, bool real_axis, bool have_b, int prec, stage_timer *tm, bool refine_on)
 {
     const size_t nacc = (size_t)2 * h->nA2 * 2 * h->Mpad * h->Mpad;
     int rc = th_launch_zero(h, h->d_acc, nacc);
@@ -477,6 +507,9 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
     if (ov && nb > 0) TH_HIP(hipStreamWaitEvent(h->stream, h->ev_green[(nb - 1) & 1], 0));
     int id = begin(2, h->stream);
     rc = th_launch_finalize(h, h->d_chi0);
+    // the block chi0[S,S] that the Dyson step amplifies near a pole, in FP64
+    if (!rc && refine_on) rc = th_refine_launch(h, q, omega, omega_p, Ef, eta, etap, n_x, real_axis, h->d_chi0);
+    h->last_refined = refine_on ? 1 : 0;
     end(id);
     return rc;
 }
@@ -485,10 +518,10 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
 int chi0_device(th_handle *h, const double *q, double omega, double Ef, double eta, double etap, int n_x,
                 const double *x2, const double *p2, int prec, stage_timer &tm)
 {
-    bool real_axis = false, have_b = false;
-    int rc = chi0_prepare(h, q, omega, Ef, etap, n_x, x2, p2, prec, tm, &real_axis, &have_b);
+    bool real_axis = false, have_b = false, refine_on = false;
+    int rc = chi0_prepare(h, q, omega, Ef, etap, n_x, x2, p2, prec, tm, &real_axis, &have_b, &refine_on);
     if (rc) return rc;
-    return chi0_launch(h, q, omega, nullptr, Ef, eta, etap, n_x, real_axis, have_b, prec, &tm);
+    return chi0_launch(h, q, omega, nullptr, Ef, eta, etap, n_x, real_axis, have_b, prec, &tm, refine_on);
 }
 
 bool bad(const void *p) { return p == nullptr; }
@@ -765,6 +798,12 @@ int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out,
     return rc;
 }
 
+int th_set_interaction(th_handle *h, const double *U)
+{
+    if (!h) { th_set_error("th_set_interaction: bad arguments"); return TH_E_ARG; }
+    return th_refine_set_support(h, reinterpret_cast<const double2 *>(U));
+}
+
 namespace {
 
 // One omega-step of the sweep captured as a hipGraph: chi0 launches with
@@ -772,12 +811,12 @@ namespace {
 // host work (nodes, caches, buffers, U support) is done before capture, so
 // the graph only references buffers that stay in place for the whole sweep.
 int capture_step(th_handle *h, const double *q, double Ef, double eta, double etap, int n_x, bool real_axis,
-                 bool have_b, int prec, int r, double2 *d_s, hipGraphExec_t *out)
+                 bool have_b, bool refine_on, int prec, int r, double2 *d_s, hipGraphExec_t *out)
 {
     const int n4 = 4 * h->nA;
     hipGraph_t graph = nullptr;
     TH_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = chi0_launch(h, q, 0.0, h->d_omega, Ef, eta, etap, n_x, real_axis, have_b, prec, nullptr);
+    int rc = chi0_launch(h, q, 0.0, h->d_omega, Ef, eta, etap, n_x, real_axis, have_b, prec, nullptr, refine_on);
     if (!rc) rc = th_dyson_launch(h, r, h->d_chi0, h->d_chi, h->d_norms);
     if (!rc) rc = th_launch_trace(h, h->d_chi0, d_s);
     if (!rc) rc = th_launch_trace(h, h->d_chi, d_s + n4 * n4);
@@ -819,6 +858,14 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
         if (!h->d_norms) TH_HIP(hipMalloc(&h->d_norms, 2 * sizeof(double)));
     }
     hipGraphExec_t ge[2] = {nullptr, nullptr};
+    // device buffers a captured step reads or writes; a replay is valid only
+    // while every one of them is still where it was at capture
+    auto buffers = [h]() {
+        return std::vector<const void *>{h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_bcache, h->d_x2,
+                                         h->d_p2, h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L,
+                                         h->d_ref_int, h->d_ref_part};
+    };
+    std::vector<const void *> ge_buf[2];
     stage_timer tm(h);
     int idall = tm.begin(4);
     for (auto &c : h->n_launch) c = 0;
@@ -826,7 +873,20 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
     // a sweep diagonalises H(k) afresh (once, at its first frequency)
     h->eig_valid[0] = h->eig_valid[1] = 0;
     int rc = th_reset_status(h);
+    // the sweep's U is the one the mixed chi0 refines its support block for
+    if (!rc && precision == TH_PREC_MIXED) rc = th_refine_set_support(h, reinterpret_cast<const double2 *>(U));
     if (!rc && h->use_graph) rc = th_dyson_prepare(h, reinterpret_cast<const double2 *>(U), &r);
+    if (!rc && h->use_graph) {
+        // panels and partial sums sized for the largest batch of any frequency
+        // of the list before the first capture, so that no later frequency
+        // reallocates a buffer a captured step refers to
+        bool any_real = false;
+        for (double e : om) any_real = any_real || std::fabs(e) >= 1e-15;
+        long need = batch_pairs(h, h->n_im);
+        if (any_real && h->nk_re > 0) need = std::max(need, batch_pairs(h, (long)n_x * h->nk_re));
+        rc = th_ensure_part(h, th_contract_splits(h, need, precision));
+        if (!rc) rc = th_ensure_panels(h, need);
+    }
     for (int w = 0; w < n_omega && !rc; ++w) {
         const double e = om[w];
         if (std::fabs(e) >= 1e-15)
@@ -837,8 +897,9 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
             }
         double cond = 0.0;
         if (h->use_graph) {
-            bool real_axis = false, have_b = false;
-            rc = chi0_prepare(h, q, e, Ef, etap, n_x, x2.data(), p2.data(), precision, tm, &real_axis, &have_b);
+            bool real_axis = false, have_b = false, refine_on = false;
+            rc = chi0_prepare(h, q, e, Ef, etap, n_x, x2.data(), p2.data(), precision, tm, &real_axis, &have_b,
+                              &refine_on);
             if (rc) break;
             if (hipMemcpyAsync(h->d_omega, &om[w], sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
                 th_set_error("th_chi_sweep: omega upload failed");
@@ -846,9 +907,16 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
                 break;
             }
             const int kind = real_axis ? 1 : 0;
-            if (!ge[kind] && (rc = capture_step(h, q, Ef, eta, etap, n_x, real_axis, have_b, precision, r, d_s,
-                                                &ge[kind])))
-                break;
+            if (ge[kind] && ge_buf[kind] != buffers()) {   // a buffer moved since the capture
+                hipGraphExecDestroy(ge[kind]);
+                ge[kind] = nullptr;
+            }
+            if (!ge[kind]) {
+                if ((rc = capture_step(h, q, Ef, eta, etap, n_x, real_axis, have_b, refine_on, precision, r, d_s,
+                                       &ge[kind])))
+                    break;
+                ge_buf[kind] = buffers();
+            }
             if (hipGraphLaunch(ge[kind], h->stream) != hipSuccess) {
                 th_set_error("th_chi_sweep: hipGraphLaunch failed");
                 rc = TH_E_HIP;
@@ -955,6 +1023,21 @@ int th_tune(th_handle *h, const char *key, long value)
     } else if (k == "graph") {
         if (value < 0 || value > 1) { th_set_error("th_tune: graph is 0 or 1"); return TH_E_ARG; }
         h->use_graph = (int)value;
+    } else if (k == "drop_caches") {
+        // frees the omega-independent panel cache and forgets the eigenpairs:
+        // the next call starts as cold as on a fresh handle
+        if (value != 1) { th_set_error("th_tune: drop_caches takes 1"); return TH_E_ARG; }
+        TH_HIP(hipStreamSynchronize(h->stream));
+        hipFree(h->d_bcache);
+        h->d_bcache = nullptr;
+        h->bcache_prec = -1;
+        h->eig_valid[0] = h->eig_valid[1] = 0;
+    } else if (k == "cache_mb") {
+        if (value < 0) { th_set_error("th_tune: cache_mb >= 0"); return TH_E_ARG; }
+        h->cache_mb = value;
+    } else if (k == "refine") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: refine is 0 or 1"); return TH_E_ARG; }
+        h->refine = (int)value;
     } else if (k == "batch_mb") {
         if (value < 16 || value > 65536) { th_set_error("th_tune: batch_mb in 16..65536"); return TH_E_ARG; }
         h->batch_mb = (int)value;
@@ -981,7 +1064,8 @@ int th_destroy(th_handle *h)
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
                     h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk, h->d_kuniq, h->d_kid_im, h->d_kid_re,
-                    h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L};
+                    h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L, h->d_kim_ptr, h->d_kim_idx,
+                    h->d_kre_ptr, h->d_kre_idx, h->d_ref_int, h->d_ref_part};
     for (void *b : bufs)
         if (b) hipFree(b);
     {
@@ -1040,6 +1124,7 @@ int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, d
 }
 
 int thx_last_split(th_handle *h) { return h ? h->last_presplit : 0; }
+int thx_last_refined(th_handle *h) { return h ? h->last_refined : 0; }
 
 int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out)
 {

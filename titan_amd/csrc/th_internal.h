@@ -107,12 +107,26 @@ struct th_handle {
     double *d_omega = nullptr;   // omega of the current replay
     double *d_norms = nullptr;   // Dyson norms of the current replay
     int batch_mb = 512;          // panel budget per batch (MB)
+    long cache_mb = 0;           // cap of the omega-independent caches (B panels, eigenpairs), MB; 0: none
     // adjoint work space and the caller's hop order (hops are stored sorted)
     std::vector<int> hop_order;  // stored hop n came from caller's entry hop_order[n]
     void *d_adj = nullptr;
     size_t adj_bytes = 0;
     void *d_blk = nullptr;       // work space of the block elimination (d = 288)
     size_t blk_bytes = 0;
+    // imaginary pairs and real-mesh points of each unique k-point (CSR, for
+    // the per-k-point FP64 block of chi0 on the support of U)
+    int *d_kim_ptr = nullptr, *d_kim_idx = nullptr, *d_kre_ptr = nullptr, *d_kre_idx = nullptr;
+    // FP64 chi0[S,S] of the mixed path (th_refine.hip)
+    int refine = 1;              // recompute chi0 on the support of U in FP64 (mixed precision)
+    int last_refined = 0;        // the last chi0 did
+    std::vector<int> ref_S;      // support of the declared U (chi indices, ascending)
+    std::vector<int> ref_built_S;   // support the device tables below were built for
+    long ref_built_nk = -1;
+    int ref_ready = 0, ref_nR = 0, ref_ntile = 0, ref_nxc = 1, ref_kpc = 1, ref_nchunk = 0;
+    size_t ref_smem = 0, ref_part_bytes = 0;
+    int *d_ref_int = nullptr;    // R, sx, sxp, tiles, S sorted
+    void *d_ref_part = nullptr;  // per-chunk partial blocks
 };
 
 // error plumbing
@@ -272,6 +286,16 @@ int th_launch_finalize(th_handle *h, double2 *chi0);
 // split_e != INT_MIN: the panel holds split (hi, lo) words of x 2^split_e (FP32 only)
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
                             double *out, const th_rowmap &rows = th_rowmap(), int split_e = INT_MIN);
+
+// ---- FP64 chi0 on the support of U (th_refine.hip) ----
+constexpr int TH_REFINE_MAX_S = 1024;   // larger supports are not refined
+// Support of U (D x D host, column-major; NULL clears it)
+int th_refine_set_support(th_handle *h, const double2 *U);
+// Host preparation (tables, partial buffer); *on = the block will be refined
+int th_refine_prepare(th_handle *h, const double *q, int prec, bool *on);
+// Launches only (capturable): chi0[S,S] overwritten by the FP64 block
+int th_refine_launch(th_handle *h, const double *q, double omega, const double *omega_p, double Ef, double eta,
+                     double etap, int n_x, bool real_axis, double2 *chi0);
 
 // ---- Dyson (th_dyson.hip) ----
 int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,
