@@ -213,8 +213,12 @@ def main():
     run = CF.run_params(args.config)
     om_all = np.asarray(run["omega"] if run["omega"] is not None else CF.omega_grid(args.config, spec), dtype=float)
     need = args.warmup + args.steps
-    om_all = np.resize(om_all, max(need, len(om_all)))
-    om_warm, om_time = om_all[:args.warmup], om_all[args.warmup:args.warmup + args.steps]
+    # frequencies spread evenly over the configuration's grid (for cfg2 most of
+    # them sit at the FMR pole, where the Dyson step is ill-conditioned)
+    pick = np.round(np.linspace(0, len(om_all) - 1, need)).astype(int) if need <= len(om_all) else \
+        np.resize(np.arange(len(om_all)), need)
+    om_sel = om_all[pick]
+    om_warm, om_time = om_sel[:args.warmup], om_sel[args.warmup:]
     log("config", args.config, spec["name"], "nk", run["nk"], "omega timed", om_time)
     t0 = time.perf_counter()
     wl = ChiWorkload(spec, run)

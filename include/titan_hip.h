@@ -281,9 +281,10 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    eigenpairs 30 % of free HBM); a cache over the cap is
  *                    not kept (panels recomputed per frequency, Green
  *                    functions by elimination), so many handles can share HBM
- *   "drop_caches"    1: frees the omega-independent imaginary-axis panel
- *                    cache and forgets the eigenpairs, so that the next call
- *                    computes them again (timing from a cold start)
+ *   "drop_caches"    1: forgets the omega-independent imaginary-axis panel
+ *                    cache and the eigenpairs, so that the next call computes
+ *                    them again into the allocations it keeps (timing from a
+ *                    cold start without the one-time allocations)
  *   "refine"         TH_PREC_MIXED with a declared U (th_set_interaction):
  *                    1 = chi0[S,S] on the support of U recomputed in FP64
  *                    (default); 0 = left as the mixed contraction gives it

@@ -270,6 +270,9 @@ long batch_pairs(th_handle *h, long total)
     return std::min(b, std::max<long>(total, 1));
 }
 
+int fill_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit, long geo, long bp, stage_timer &tm,
+                bool *have);
+
 // imaginary-axis B panels G(k, Ef + i(y+etap)) do not depend on omega:
 // computed once per (Ef, etap, precision) and kept while they fit
 int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit, stage_timer &tm, bool *have)
@@ -282,10 +285,14 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
         *have = true;
         return TH_OK;
     }
-    hipFree(h->d_bcache);
-    h->d_bcache = nullptr;
     h->bcache_prec = -1;
     const size_t need = (size_t)h->n_im * h->nA2 * 2 * h->Mpad * prec_size(prec);
+    if (h->d_bcache && h->bcache_bytes != need) {
+        hipFree(h->d_bcache);
+        h->d_bcache = nullptr;
+        h->bcache_bytes = 0;
+    }
+    if (h->d_bcache) return fill_bcache(h, Ef, etap, prec, presplit, geo, bp, tm, have);   // refill in place
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return TH_OK;
     // the cache may take most of HBM (288 GB on MI355X); otherwise recompute per
@@ -302,6 +309,17 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
         (void)hipGetLastError();
         return TH_OK;
     }
+    h->bcache_bytes = need;
+    return fill_bcache(h, Ef, etap, prec, presplit, geo, bp, tm, have);
+}
+
+// the cache's panels computed into its allocation
+int fill_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit, long geo, long bp, stage_timer &tm,
+                bool *have)
+{
+    double bb = 0.0;   // |G| <= 1/(y + eta') over the whole list
+    if (presplit)
+        for (double y : h->yim_host) bb = std::max(bb, 1.0 / (y + etap));
     for (long p0 = 0; p0 < h->n_im; p0 += bp) {
         th_green_args a{};
         a.mode = TH_G_IMAG_B;
@@ -311,9 +329,7 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
         a.etap = etap;
         a.prec = prec;
         a.sum_order = 1;
-        if (presplit) {   // |G| <= 1/(y + eta')
-            double bb = 0.0;
-            for (double y : h->yim_host) bb = std::max(bb, 1.0 / (y + etap));
+        if (presplit) {
             a.presplit = 1;
             a.pexp = th_pow2_scale(bb);
             h->bcache_pexp = a.pexp;
@@ -676,6 +692,7 @@ int th_set_imag_points(th_handle *h, long n, const double *k, const double *y, c
     hipFree(h->d_kim); hipFree(h->d_yim); hipFree(h->d_wim); hipFree(h->d_bcache);
     h->d_kim = h->d_yim = h->d_wim = nullptr;
     h->d_bcache = nullptr;
+    h->bcache_bytes = 0;
     h->bcache_prec = -1;
     // unique k-points of the list
     std::unordered_map<kkey, int, kkey_hash> ids;
@@ -1024,12 +1041,10 @@ int th_tune(th_handle *h, const char *key, long value)
         if (value < 0 || value > 1) { th_set_error("th_tune: graph is 0 or 1"); return TH_E_ARG; }
         h->use_graph = (int)value;
     } else if (k == "drop_caches") {
-        // frees the omega-independent panel cache and forgets the eigenpairs:
-        // the next call starts as cold as on a fresh handle
+        // forgets the omega-independent panel cache and the eigenpairs: the
+        // next call computes them again (into the allocations it keeps)
         if (value != 1) { th_set_error("th_tune: drop_caches takes 1"); return TH_E_ARG; }
         TH_HIP(hipStreamSynchronize(h->stream));
-        hipFree(h->d_bcache);
-        h->d_bcache = nullptr;
         h->bcache_prec = -1;
         h->eig_valid[0] = h->eig_valid[1] = 0;
     } else if (k == "cache_mb") {

@@ -65,6 +65,7 @@ struct th_handle {
     double2 *d_chi = nullptr;    // D*D
     // omega-independent imaginary-axis B panels (G(k, E_F + i(y+etap)))
     void *d_bcache = nullptr;
+    size_t bcache_bytes = 0;     // size of the d_bcache allocation
     int bcache_prec = -1;
     double bcache_key[2] = {0, 0};   // (Ef, etap)
     long bcache_bp = 0;              // stored in summation order per batch of this many pairs (0: pair order)
