@@ -109,6 +109,20 @@ int th_set_interaction(th_handle *h, const double *U);
  * the near-pole ill-conditioning. */
 int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out, double *cond_est);
 
+/* th_dyson without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  chi0_dev
+ * and chi_dev are device pointers to D x D complex (column-major, as
+ * th_dyson's host arrays); U is the host D x D interaction matrix.  Work
+ * space (the support of U, U_S, the reduced matrix and the two rank-|S|
+ * factors) lives in a fixed device array of 64 MiB per device, set up by
+ * th_create: 2 |S| D + 2 |S|^2 complex values must fit (TH_E_UNSUPPORTED
+ * otherwise).  Every call on one device shares that array, so calls that
+ * overlap on one device (from different handles or threads) race; one call
+ * at a time per device.  Same kernels and results as th_dyson; synchronises
+ * the handle's stream before it returns. */
+int th_dyson_noalloc(th_handle *h, const double *chi0_dev, const double *U, double *chi_dev, double *cond_est);
+
 /* Full frequency sweep on the device (calculate_chi's omega loop,
  * source/calculate_chi.F90:66-107): for each omega[w] computes chi0 and chi
  * and their orbital traces

@@ -20,7 +20,7 @@ _lp = C.POINTER(C.c_long)
 _lib = None
 
 EXPORTS = ("th_create", "th_set_imag_points", "th_set_real_kmesh", "th_chi0", "th_dyson",
-           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_expectation_values", "th_current_response", "th_set_interaction", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
+           "th_chi_sweep", "th_chi_adjoint", "th_chi_traces", "th_spin_density", "th_expectation_values", "th_current_response", "th_set_interaction", "th_dyson_noalloc", "th_destroy", "th_last_error", "th_limits", "th_last_timing",
            "th_tune", "th_get_tuning")
 
 
@@ -52,6 +52,7 @@ def load():
     L.th_current_response.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                       _dp, _dp, _dp, C.c_void_p, _dp]
     L.th_set_interaction.argtypes = [C.c_void_p, C.c_void_p]
+    L.th_dyson_noalloc.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p, _dp]
     L.th_destroy.argtypes = [C.c_void_p]
     L.th_last_error.restype = C.c_char_p
     L.th_limits.argtypes = [_ip, _ip, _ip]
@@ -159,6 +160,15 @@ class Handle:
         _check(load().th_dyson(self._h, a.ctypes.data_as(_dp), u.ctypes.data_as(_dp),
                                out.ctypes.data_as(_dp), C.byref(cond)), "th_dyson")
         return out, cond.value
+
+    def dyson_noalloc(self, chi0_dev_ptr, U, chi_dev_ptr):
+        """th_dyson_noalloc on device buffers (integer device addresses of
+        D x D complex128, column-major); returns the condition number."""
+        u = _c128(U)
+        cond = C.c_double()
+        _check(load().th_dyson_noalloc(self._h, C.c_void_p(chi0_dev_ptr), u.ctypes.data_as(_dp),
+                                       C.c_void_p(chi_dev_ptr), C.byref(cond)), "th_dyson_noalloc")
+        return cond.value
 
     def sweep(self, q, omegas, Ef, eta, etap, parts3, n3gl, U, precision=PREC_FP64, full=False):
         q = np.ascontiguousarray(q, dtype=np.float64)

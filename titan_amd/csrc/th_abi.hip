@@ -583,6 +583,11 @@ int th_create(int nA, int nOrb, int nhop, const double *R, const int *jb, const 
     }
     th_handle *h = new th_handle();
     TH_HIP(hipGetDevice(&h->device));
+    // the allocation-free Dyson entry point's scratch on this device
+    if (int rc = th_noalloc_init(h->device)) {
+        delete h;
+        return rc;
+    }
     hipDeviceProp_t prop;
     TH_HIP(hipGetDeviceProperties(&prop, h->device));
     if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
@@ -813,6 +818,16 @@ int th_dyson(th_handle *h, const double *chi0, const double *U, double *chi_out,
     rc = th_check_status(h);
     tm.collect();
     return rc;
+}
+
+int th_dyson_noalloc(th_handle *h, const double *chi0_dev, const double *U, double *chi_dev, double *cond_est)
+{
+    if (!h || bad(chi0_dev) || bad(U) || bad(chi_dev)) {
+        th_set_error("th_dyson_noalloc: bad arguments");
+        return TH_E_ARG;
+    }
+    return th_dyson_noalloc_impl(h, reinterpret_cast<const double2 *>(chi0_dev), reinterpret_cast<const double2 *>(U),
+                                 reinterpret_cast<double2 *>(chi_dev), cond_est);
 }
 
 int th_set_interaction(th_handle *h, const double *U)

@@ -11,9 +11,19 @@
 // All of the near-pole ill-conditioning of (I + chi0 U) sits in W; its
 // 1-norm condition number is returned.
 #include <algorithm>
+#include <climits>
+#include <mutex>
 #include <vector>
 
 #include "th_internal.h"
+
+// Device scratch of th_dyson_noalloc: one array per device (each GPU the
+// library runs on holds its own copy until the process ends).  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device, never through the array's name.
+static __device__ __align__(256) unsigned char scratch[TH_NOALLOC_SCRATCH];
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;   // [device] -> address of scratch on that device
 
 namespace {
 
@@ -303,5 +313,108 @@ int th_launch_trace(th_handle *h, const double2 *chi, double2 *schi)
     hipLaunchKernelGGL(trace_kernel, dim3(nblk((size_t)n4 * n4, 64)), dim3(64), 0, h->stream, chi, h->nA, h->nOrb,
                        h->D, schi);
     TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+int th_noalloc_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return TH_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        th_set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return TH_E_HIP;
+    }
+    g_scratch_ptr[device] = p;
+    return TH_OK;
+}
+
+namespace {
+
+size_t up256(size_t n) { return (n + 255) / 256 * 256; }
+
+}  // namespace
+
+int th_dyson_noalloc_impl(th_handle *h, const double2 *chi0, const double2 *U, double2 *chi, double *cond)
+{
+    unsigned char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (h->device < (int)g_scratch_ptr.size()) base = static_cast<unsigned char *>(g_scratch_ptr[h->device]);
+    }
+    if (!base) {
+        th_set_error("th_dyson_noalloc: the device scratch was not set up by th_create on this device");
+        return TH_E_STATE;
+    }
+    const int D = h->D;
+    std::vector<char> on(D, 0);
+    for (int c = 0; c < D; ++c)
+        for (int a = 0; a < D; ++a) {
+            const double2 u = U[a + (size_t)D * c];
+            if (u.x != 0.0 || u.y != 0.0) { on[a] = 1; on[c] = 1; }
+        }
+    std::vector<int> S;
+    for (int a = 0; a < D; ++a)
+        if (on[a]) S.push_back(a);
+    const int r = (int)S.size();
+    // scratch layout: norms[2] + status | perm[r] | S[r] | U_S | W | V | Z
+    const size_t o_perm = 256, o_S = o_perm + up256(sizeof(int) * (size_t)std::max(r, 1));
+    const size_t o_US = o_S + up256(sizeof(int) * (size_t)std::max(r, 1));
+    const size_t o_W = o_US + up256(sizeof(double2) * (size_t)r * r);
+    const size_t o_V = o_W + up256(sizeof(double2) * (size_t)r * r);
+    const size_t o_Z = o_V + up256(sizeof(double2) * (size_t)D * r);
+    const size_t need = o_Z + up256(sizeof(double2) * (size_t)r * D);
+    if (need > TH_NOALLOC_SCRATCH) {
+        th_set_error("th_dyson_noalloc: support of U (" + std::to_string(r) + " indices) with D = " +
+                     std::to_string(D) + " needs " + std::to_string(need) + " bytes of scratch, the limit is " +
+                     std::to_string((size_t)TH_NOALLOC_SCRATCH));
+        return TH_E_UNSUPPORTED;
+    }
+    double *norms = reinterpret_cast<double *>(base);
+    int *status = reinterpret_cast<int *>(base + 2 * sizeof(double));
+    int *perm = reinterpret_cast<int *>(base + o_perm);
+    int *dS = reinterpret_cast<int *>(base + o_S);
+    double2 *dUS = reinterpret_cast<double2 *>(base + o_US), *W = reinterpret_cast<double2 *>(base + o_W);
+    double2 *V = reinterpret_cast<double2 *>(base + o_V), *Z = reinterpret_cast<double2 *>(base + o_Z);
+    // the scratch keeps the previous call's contents: the status word starts
+    // at 'no zero pivot'
+    const int none = INT_MAX;
+    TH_HIP(hipMemcpyAsync(status, &none, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (r == 0) {
+        TH_HIP(hipMemcpyAsync(chi, chi0, sizeof(double2) * (size_t)D * D, hipMemcpyDeviceToDevice, h->stream));
+        hipLaunchKernelGGL(unit_norms, dim3(1), dim3(64), 0, h->stream, norms);
+        TH_HIP(hipGetLastError());
+    } else {
+        std::vector<double2> US((size_t)r * r);
+        for (int b = 0; b < r; ++b)
+            for (int a = 0; a < r; ++a) US[a + (size_t)r * b] = U[S[a] + (size_t)D * S[b]];
+        TH_HIP(hipMemcpyAsync(dS, S.data(), sizeof(int) * r, hipMemcpyHostToDevice, h->stream));
+        TH_HIP(hipMemcpyAsync(dUS, US.data(), sizeof(double2) * (size_t)r * r, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(form_w, dim3(nblk((size_t)r * r, 256)), dim3(256), 0, h->stream, chi0, D, dS, dUS, r, W);
+        TH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gj_global, dim3(1), dim3(1024), sizeof(double2) * 2 * r, h->stream, W, r, perm, norms,
+                           status);
+        TH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(form_z, dim3(nblk((size_t)r * D, 256)), dim3(256), 0, h->stream, W, r, chi0, D, dS, Z);
+        TH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(form_v, dim3(nblk((size_t)r * D, 256)), dim3(256), 0, h->stream, chi0, D, dS, dUS, r, V);
+        TH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(rank_update, dim3(nblk((size_t)D * D, 256)), dim3(256), 0, h->stream, chi0, V, Z, D, r,
+                           chi);
+        TH_HIP(hipGetLastError());
+    }
+    double nv[2];
+    int st = INT_MAX;
+    TH_HIP(hipMemcpyAsync(nv, norms, sizeof(nv), hipMemcpyDeviceToHost, h->stream));
+    TH_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    // also keeps the host copies of S and U_S alive until their uploads are done
+    TH_HIP(hipStreamSynchronize(h->stream));
+    if (cond) *cond = nv[0] * nv[1];
+    if (st != INT_MAX) {
+        th_set_error("th_dyson_noalloc: exactly singular pivot of (I + chi0 U), step " + std::to_string(st));
+        return st;
+    }
     return TH_OK;
 }

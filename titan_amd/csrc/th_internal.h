@@ -299,6 +299,11 @@ int th_refine_launch(th_handle *h, const double *q, double omega, const double *
                      double etap, int n_x, bool real_axis, double2 *chi0);
 
 // ---- Dyson (th_dyson.hip) ----
+// device scratch of the allocation-free Dyson entry point, per device
+constexpr size_t TH_NOALLOC_SCRATCH = (size_t)64 << 20;
+int th_noalloc_init(int device);   // once per device, that device current
+int th_dyson_noalloc_impl(th_handle *h, const double2 *chi0_dev, const double2 *U_host, double2 *chi_dev,
+                          double *cond);
 int th_dyson_device(th_handle *h, const double2 *chi0, const double2 *U_host_colmajor,
                     double2 *chi, double *cond);
 // Graph-friendly split of the Dyson step: set-up from U (host work, uploads,
