@@ -301,7 +301,11 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    cold start without the one-time allocations)
  *   "refine"         TH_PREC_MIXED with a declared U (th_set_interaction):
  *                    1 = chi0[S,S] on the support of U recomputed in FP64
- *                    (default); 0 = left as the mixed contraction gives it
+ *                    (default); 0 = left as the mixed contraction gives it;
+ *                    2 = as 1, and TH_PREC_FP64 too takes chi0[S,S] from the
+ *                    eigenpair form (a second FP64 evaluation of the block
+ *                    that the Dyson step amplifies, to measure how far two
+ *                    FP64 summations of chi disagree at a pole)
  *   "batch_mb"       panel memory per batch in MB of FP64-sized panels (pairs
  *                    per launch); a budget of at least one full wave of the
  *                    fp16 spectral kernel's workgroups (d = 18, 36, 54) is

@@ -45,12 +45,17 @@ def _support(U):
     return np.nonzero(np.any(U != 0, axis=0) | np.any(U != 0, axis=1))[0]
 
 
-@pytest.mark.parametrize("nA,q", [(1, (0.0, 0.0, 0.0)), (2, (0.1, 0.0, 0.05)), (3, (0.0, 0.0, 0.0))])
-def test_refined_block_matches_oracle(nA, q):
+@pytest.mark.parametrize("nA,q,un", [(1, (0.0, 0.0, 0.0), 0.0), (2, (0.1, 0.0, 0.05), 0.0), (3, (0.0, 0.0, 0.0), 0.0),
+                                     (1, (0.05, 0.0, 0.0), 0.04), (2, (0.0, 0.0, 0.0), 0.04)])
+def test_refined_block_matches_oracle(nA, q, un):
     """Mixed th_chi0 with U declared, d = 18 / 36 / 54, imaginary and real
     axis, q = 0 and q != 0, eta != eta': chi0[S,S] is the FP64 oracle's to
-    1e-12, every other entry within 1e-6."""
+    1e-12, every other entry within 1e-6.  Un = 0 gives supports of 10 nA
+    indices (the matrix-core form), Un != 0 supports of 60 nA (the
+    vector-ALU form)."""
     spec = CF.bcc_supercell(nA, seed=3)
+    for ty in spec["types"]:
+        ty["U"] = [un, ty["U"][1]]
     m, s = TBModel(spec), O.OracleSystem(spec)
     y, wy = Ct.imag_epoints(1, 12)
     kp, wk = Ct.uniform_kmesh(m.b, 3, 3, 3)
@@ -63,7 +68,7 @@ def test_refined_block_matches_oracle(nA, q):
     h.set_imag_points(K, Y, W)
     h.set_real_kmesh(kp, wk)
     h.set_interaction(m.Umat)
-    print("refined block: nA", nA, "q", q, flush=True)
+    print("refined block: nA", nA, "q", q, "support", len(_support(m.Umat)), flush=True)
     g = h.chi0(list(q), om, m.Ef, eta, etap, x2, p2, precision=B.PREC_MIXED)
     assert _refined().thx_last_refined(h._h) == 1
     h.close()

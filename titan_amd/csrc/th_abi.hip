@@ -906,7 +906,8 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
     h->eig_valid[0] = h->eig_valid[1] = 0;
     int rc = th_reset_status(h);
     // the sweep's U is the one the mixed chi0 refines its support block for
-    if (!rc && precision == TH_PREC_MIXED) rc = th_refine_set_support(h, reinterpret_cast<const double2 *>(U));
+    if (!rc && (precision == TH_PREC_MIXED || h->refine == 2))
+        rc = th_refine_set_support(h, reinterpret_cast<const double2 *>(U));
     if (!rc && h->use_graph) rc = th_dyson_prepare(h, reinterpret_cast<const double2 *>(U), &r);
     if (!rc && h->use_graph) {
         // panels and partial sums sized for the largest batch of any frequency
@@ -1066,7 +1067,7 @@ int th_tune(th_handle *h, const char *key, long value)
         if (value < 0) { th_set_error("th_tune: cache_mb >= 0"); return TH_E_ARG; }
         h->cache_mb = value;
     } else if (k == "refine") {
-        if (value < 0 || value > 1) { th_set_error("th_tune: refine is 0 or 1"); return TH_E_ARG; }
+        if (value < 0 || value > 2) { th_set_error("th_tune: refine is 0, 1 or 2"); return TH_E_ARG; }
         h->refine = (int)value;
     } else if (k == "batch_mb") {
         if (value < 16 || value > 65536) { th_set_error("th_tune: batch_mb in 16..65536"); return TH_E_ARG; }

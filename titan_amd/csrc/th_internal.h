@@ -125,8 +125,9 @@ struct th_handle {
     std::vector<int> ref_built_S;   // support the device tables below were built for
     long ref_built_nk = -1;
     int ref_ready = 0, ref_nR = 0, ref_ntile = 0, ref_nxc = 1, ref_kpc = 1, ref_nchunk = 0;
+    int ref_mfma = 0;            // the support fits the matrix-core form (at most 32 indices)
     size_t ref_smem = 0, ref_part_bytes = 0;
-    int *d_ref_int = nullptr;    // R, sx, sxp, tiles, S sorted
+    int *d_ref_int = nullptr;    // R, sx, sxp, sy, syp, tiles, S sorted
     void *d_ref_part = nullptr;  // per-chunk partial blocks
 };
 

@@ -12,8 +12,9 @@
 // (imaginary axis) and :146-164 (real axis) is
 //   chi0[X,Y] += c_mn Vq[x,m] conj(V[x',n]) conj(Vq[y,m]) V[y',n]   (summed over m, n)
 // where X = (sigma, i, mu, nu) selects the G_q row x = (i, a_sigma, nu) and the
-// G column x' = (i, b_sigma, mu), with (a, b) = (dn, up), (up, up), (dn, dn),
-// (up, dn) for sigma = +, up, dn, - (Y likewise), and
+// G column x' = (i, b_sigma, mu), and Y = (sigma', j, gamma, xi) the G_q column
+// y = (j, a_sigma', gamma) and the G row y' = (j, b_sigma', xi), with (a, b) =
+// (dn, up), (up, up), (dn, dn), (up, dn) for sigma = +, up, dn, -, and
 //   imaginary axis  c_mn = w (fq_m f_n + conj(fq_m f_n)) = 2 w Re(fq_m f_n)
 //   real axis       c_mn = -i w (fq_m - conj fq_m) conj(f_n) = 2 w Im(fq_m) conj(f_n).
 // The energy sum is taken first, K_mn(k) = sum_E c_mn(E) for each k-point, and
@@ -40,6 +41,7 @@ struct refine_params {
     int nR, nS, nx_chunk;
     const int *R;              // [nR] Hamiltonian rows
     const int *sx, *sxp;       // [nS] index in R of x_X and of x'_X (S sorted by x_X)
+    const int *sy, *syp;       // [nS] index in R of y_Y and of y'_Y
     const int *tiles;          // [ntile][4]: S rows s0..s1, R range r0..r1 of their x
     long nk;
     int kpc;                   // k-points per chunk
@@ -69,12 +71,105 @@ __device__ inline double2 crecip(double re, double im)
     return make_double2(re / n, -im / n);
 }
 
+// Loads the eigenvector rows of R and the eigenvalues of k-point k into LDS and
+// sums K_mn over the k-point's energies into sK (row pitch PITCH); sW is the
+// scratch of one energy chunk.  Ends with K stored, without a barrier.
+template <int D, int PITCH>
+__device__ void load_and_k(const refine_params &p, long k, double omega, double2 *sVq, double2 *sV, double2 *sK,
+                           double *sLq, double *sL, double2 *sW)
+{
+    constexpr int NKE = (D * D + RF_NT - 1) / RF_NT;
+    const int nR = p.nR, tid = threadIdx.x;
+    // eigenvector rows of R and the eigenvalues
+    const double2 *Vqk = p.Vq + (size_t)k * D * D, *Vk = p.V + (size_t)k * D * D;
+    for (int e = tid; e < nR * D; e += RF_NT) {
+        const int r = e / D, m = e - r * D;
+        sVq[e] = Vqk[(size_t)p.R[r] * D + m];
+        sV[e] = Vk[(size_t)p.R[r] * D + m];
+    }
+    for (int m = tid; m < D; m += RF_NT) {
+        sLq[m] = p.Lq[(size_t)k * D + m];
+        sL[m] = p.L[(size_t)k * D + m];
+    }
+    // K_mn = sum over the energies of this k-point
+    double kr[NKE], ki[NKE];
+#pragma unroll
+    for (int j = 0; j < NKE; ++j) kr[j] = ki[j] = 0.0;
+    const int i0 = p.im_ptr[k], i1 = p.im_ptr[k + 1];
+    const int q0 = p.real_axis ? p.re_ptr[k] : 0, q1 = p.real_axis ? p.re_ptr[k + 1] : 0;
+    const long nim = i1 - i0, ntot = nim + (long)(q1 - q0) * p.nx;
+    double2 *sA = sW, *sB = sW + RF_EC * D;
+    for (long c0 = 0, ne = 0; c0 < ntot; c0 += ne) {
+        // chunks never mix the two axes: imaginary energies first
+        const bool imag = c0 < nim;
+        ne = imag ? min((long)RF_EC, nim - c0) : min((long)RF_EC, ntot - c0);
+        __syncthreads();   // previous chunk consumed (first pass: V, L stored)
+        for (int v = tid; v < 2 * ne * D; v += RF_NT) {
+            const int side = v / (ne * D), e = (v - side * ne * D) / D, m = v - side * ne * D - e * D;
+            double2 val;
+            if (imag) {
+                const int pr = p.im_idx[i0 + c0 + e];
+                const double y = p.yim[pr], w = p.wim[pr];
+                if (side == 0) {
+                    const double2 f = crecip(p.Ef + omega - sLq[m], y + p.eta);
+                    val = make_double2(2.0 * w * f.x, 2.0 * w * f.y);
+                } else {
+                    val = crecip(p.Ef - sL[m], y + p.etap);
+                }
+            } else {
+                const long g = c0 - nim + e;
+                const int rm = p.re_idx[q0 + (int)(g / p.nx)], node = (int)(g % p.nx);
+                const double w = p.p2[node] * p.wre[rm], x = p.x2[node];
+                if (side == 0) {
+                    val = make_double2(2.0 * w * crecip(x + omega - sLq[m], p.eta).y, 0.0);
+                } else {
+                    const double2 f = crecip(x - sL[m], p.etap);
+                    val = make_double2(f.x, -f.y);
+                }
+            }
+            (side ? sB : sA)[e * D + m] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NKE; ++j) {
+            const int e = tid + RF_NT * j;
+            if (e < D * D) {
+                const int m = e / D, n = e - m * D;
+                double a = kr[j], b = ki[j];
+                if (imag) {
+                    for (int t = 0; t < ne; ++t) {
+                        const double2 x = sA[t * D + m], y = sB[t * D + n];
+                        a = fma(x.x, y.x, fma(-x.y, y.y, a));
+                    }
+                } else {
+                    for (int t = 0; t < ne; ++t) {
+                        const double x = sA[t * D + m].x;
+                        const double2 y = sB[t * D + n];
+                        a = fma(x, y.x, a);
+                        b = fma(x, y.y, b);
+                    }
+                }
+                kr[j] = a;
+                ki[j] = b;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NKE; ++j) {
+        const int e = tid + RF_NT * j;
+        if (e < D * D) {
+            const int m = e / D, n = e - m * D;
+            sK[m * PITCH + n] = make_double2(kr[j], ki[j]);
+        }
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(RF_NT) void refine_kernel(refine_params p)
 {
     constexpr int DP = (D + 3) / 4 * 4;     // padded row of K and C
     constexpr int NQ = DP / 4;              // column quads of a C row
-    constexpr int NKE = (D * D + RF_NT - 1) / RF_NT;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nR = p.nR, nS = p.nS, tid = threadIdx.x;
     double2 *sVq = reinterpret_cast<double2 *>(smem);      // [nR][D]
@@ -98,89 +193,7 @@ __global__ __launch_bounds__(RF_NT) void refine_kernel(refine_params p)
     }
     const long k0 = (long)blockIdx.x * p.kpc, k1 = min(p.nk, k0 + p.kpc);
     for (long k = k0; k < k1; ++k) {
-        // eigenvector rows of R and the eigenvalues
-        const double2 *Vqk = p.Vq + (size_t)k * D * D, *Vk = p.V + (size_t)k * D * D;
-        for (int e = tid; e < nR * D; e += RF_NT) {
-            const int r = e / D, m = e - r * D;
-            sVq[e] = Vqk[(size_t)p.R[r] * D + m];
-            sV[e] = Vk[(size_t)p.R[r] * D + m];
-        }
-        for (int m = tid; m < D; m += RF_NT) {
-            sLq[m] = p.Lq[(size_t)k * D + m];
-            sL[m] = p.L[(size_t)k * D + m];
-        }
-        // K_mn = sum over the energies of this k-point
-        double kr[NKE], ki[NKE];
-#pragma unroll
-        for (int j = 0; j < NKE; ++j) kr[j] = ki[j] = 0.0;
-        const int i0 = p.im_ptr[k], i1 = p.im_ptr[k + 1];
-        const int q0 = p.real_axis ? p.re_ptr[k] : 0, q1 = p.real_axis ? p.re_ptr[k + 1] : 0;
-        const long nim = i1 - i0, ntot = nim + (long)(q1 - q0) * p.nx;
-        double2 *sA = sW, *sB = sW + RF_EC * D;
-        for (long c0 = 0, ne = 0; c0 < ntot; c0 += ne) {
-            // chunks never mix the two axes: imaginary energies first
-            const bool imag = c0 < nim;
-            ne = imag ? min((long)RF_EC, nim - c0) : min((long)RF_EC, ntot - c0);
-            __syncthreads();   // previous chunk consumed (first pass: V, L stored)
-            for (int v = tid; v < 2 * ne * D; v += RF_NT) {
-                const int side = v / (ne * D), e = (v - side * ne * D) / D, m = v - side * ne * D - e * D;
-                double2 val;
-                if (imag) {
-                    const int pr = p.im_idx[i0 + c0 + e];
-                    const double y = p.yim[pr], w = p.wim[pr];
-                    if (side == 0) {
-                        const double2 f = crecip(p.Ef + omega - sLq[m], y + p.eta);
-                        val = make_double2(2.0 * w * f.x, 2.0 * w * f.y);
-                    } else {
-                        val = crecip(p.Ef - sL[m], y + p.etap);
-                    }
-                } else {
-                    const long g = c0 - nim + e;
-                    const int rm = p.re_idx[q0 + (int)(g / p.nx)], node = (int)(g % p.nx);
-                    const double w = p.p2[node] * p.wre[rm], x = p.x2[node];
-                    if (side == 0) {
-                        val = make_double2(2.0 * w * crecip(x + omega - sLq[m], p.eta).y, 0.0);
-                    } else {
-                        const double2 f = crecip(x - sL[m], p.etap);
-                        val = make_double2(f.x, -f.y);
-                    }
-                }
-                (side ? sB : sA)[e * D + m] = val;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < NKE; ++j) {
-                const int e = tid + RF_NT * j;
-                if (e < D * D) {
-                    const int m = e / D, n = e - m * D;
-                    double a = kr[j], b = ki[j];
-                    if (imag) {
-                        for (int t = 0; t < ne; ++t) {
-                            const double2 x = sA[t * D + m], y = sB[t * D + n];
-                            a = fma(x.x, y.x, fma(-x.y, y.y, a));
-                        }
-                    } else {
-                        for (int t = 0; t < ne; ++t) {
-                            const double x = sA[t * D + m].x;
-                            const double2 y = sB[t * D + n];
-                            a = fma(x, y.x, a);
-                            b = fma(x, y.y, b);
-                        }
-                    }
-                    kr[j] = a;
-                    ki[j] = b;
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < NKE; ++j) {
-            const int e = tid + RF_NT * j;
-            if (e < D * D) {
-                const int m = e / D, n = e - m * D;
-                sK[m * DP + n] = make_double2(kr[j], ki[j]);
-            }
-        }
+        load_and_k<D, DP>(p, k, omega, sVq, sV, sK, sLq, sL, sW);
         if constexpr (DP > D)
             for (int e = tid; e < D * (DP - D); e += RF_NT)
                 sK[(e / (DP - D)) * DP + D + e % (DP - D)] = make_double2(0.0, 0.0);
@@ -225,8 +238,8 @@ __global__ __launch_bounds__(RF_NT) void refine_kernel(refine_params p)
                 const int xr = p.sx[X];
                 if (xr < xa || xr >= xa + nxc) continue;
                 const int Y = ey[j];
-                const double2 *cr = sW + ((size_t)(xr - xa) * nR + p.sx[Y]) * DP;
-                const double2 *vxp = sV + p.sxp[X] * D, *vyp = sV + p.sxp[Y] * D;
+                const double2 *cr = sW + ((size_t)(xr - xa) * nR + p.sy[Y]) * DP;
+                const double2 *vxp = sV + p.sxp[X] * D, *vyp = sV + p.syp[Y] * D;
                 double2 s = acc[j];
 #pragma unroll 2
                 for (int n = 0; n < D; ++n) s = cfma(cr[n], cmulc(vyp[n], vxp[n]), s);
@@ -239,6 +252,106 @@ __global__ __launch_bounds__(RF_NT) void refine_kernel(refine_params p)
 #pragma unroll
     for (int j = 0; j < RF_ACC; ++j)
         if (ex[j] >= 0) out[ex[j] + (size_t)nS * ey[j]] = acc[j];
+}
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
+// Supports of at most 32 indices on the FP64 matrix cores: per k-point the block
+// is one GEMM over the d^2 eigen-index pairs (m, n),
+//   chi0_SS(k) = A B,   A[X, (m,n)] = Vq[x_X, m] conj(V[x'_X, n]) K_mn,
+//                       B[(m,n), Y] = conj(Vq[y_Y, m]) V[y'_Y, n],
+// 4 pairs per v_mfma_f64_16x16x4 step, the operands formed in registers from
+// the eigenvector rows in LDS and the complex product
+// as 4 real MFMAs.  Each of the 8 waves takes every 8th step and keeps its own
+// 32 x 32 accumulators over the workgroup's k-points; the waves' sums are
+// added in a fixed order through LDS at the end.  Lane l supplies A[l & 15][l >> 4]
+// and B[l >> 4][l & 15] and holds C[(l >> 4) + 4 q][l & 15] in register q.
+template <int D>
+__global__ __launch_bounds__(RF_NT) void refine_mfma_kernel(refine_params p)
+{
+    constexpr int NSTEP = D * D / 4;
+    static_assert(D * D % 4 == 0, "pairs (m, n) must fill whole MFMA steps");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nR = p.nR, nS = p.nS, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double2 *sVq = reinterpret_cast<double2 *>(smem);       // [nR][D]
+    double2 *sV = sVq + nR * D;                             // [nR][D]
+    double2 *sK = sV + nR * D;                              // [D][D]
+    double *sLq = reinterpret_cast<double *>(sK + D * D);   // [D]
+    double *sL = sLq + D;                                   // [D]
+    double2 *sW = reinterpret_cast<double2 *>(sL + D);      // energy chunk; at the end the 32 x 32 sums
+    const double omega = p.omega_p ? *p.omega_p : p.omega;
+    const int nt = (nS + 15) >> 4;                          // 16-wide tiles per dimension (1 or 2)
+    // this lane's rows of A (X = 16 t + (lane & 15): x, x') and columns of B
+    // (Y = 16 t + (lane & 15): y, y') as row offsets in sVq / sV
+    int px[2], ppx[2], py[2], ppy[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int X = 16 * t + (lane & 15);
+        px[t] = X < nS ? p.sx[X] * D : -1;
+        ppx[t] = X < nS ? p.sxp[X] * D : -1;
+        py[t] = X < nS ? p.sy[X] * D : -1;
+        ppy[t] = X < nS ? p.syp[X] * D : -1;
+    }
+    dv4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { cr[a][b][q] = 0.0; ci[a][b][q] = 0.0; }
+    const long k0 = (long)blockIdx.x * p.kpc, k1 = min(p.nk, k0 + p.kpc);
+    for (long k = k0; k < k1; ++k) {
+        load_and_k<D, D>(p, k, omega, sVq, sV, sK, sLq, sL, sW);
+        __syncthreads();   // K stored
+        for (int st = wave; st < NSTEP; st += RF_NT / 64) {
+            const int idx = 4 * st + (lane >> 4), m = idx / D, n = idx - m * D;
+            const double2 kk = sK[idx];
+            double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+                if (t < nt && px[t] >= 0) {
+                    a = cmul(cmulc(sVq[px[t] + m], sV[ppx[t] + n]), kk);
+                    b = cmulc(sV[ppy[t] + n], sVq[py[t] + m]);
+                }
+                ar[t] = a.x; ai[t] = a.y; br[t] = b.x; bi[t] = b.y;
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    if (a < nt && b < nt) {
+                        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+                        cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
+                        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+                        ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+                    }
+                }
+        }
+        __syncthreads();   // this k-point's rows and K consumed
+    }
+    // waves' sums added one wave at a time in wave order: sum[X][Y] in LDS
+    double2 *sum = sW;   // [32][32]
+    for (int w = 0; w < RF_NT / 64; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int X = 16 * a + (lane >> 4) + 4 * q, Y = 16 * b + (lane & 15);
+                        const double2 v = make_double2(cr[a][b][q], ci[a][b][q]);
+                        sum[X * 32 + Y] = w ? make_double2(sum[X * 32 + Y].x + v.x, sum[X * 32 + Y].y + v.y) : v;
+                    }
+        }
+        __syncthreads();
+    }
+    double2 *out = p.part + (size_t)blockIdx.x * nS * nS;
+    for (int e = tid; e < nS * nS; e += RF_NT) {
+        const int X = e % nS, Y = e / nS;
+        out[e] = sum[X * 32 + Y];
+    }
 }
 
 // chi0[S[a], S[b]] = (1/2pi) sum_c part[c][a][b], chunks in order
@@ -293,14 +406,20 @@ int refine_build(th_handle *h)
     const int d = h->d, nO = h->nOrb;
     // sigma = +, up, dn, -: G_q row spin a, G column spin b
     const int sa[4] = {1, 0, 1, 0}, sb[4] = {0, 0, 1, 1};
-    std::vector<int> x(nS), xp(nS);
+    // as a row X: x = (i, a, nu), x' = (i, b, mu); as a column Y: y = (i, a, mu),
+    // y' = (i, b, nu)
+    std::vector<int> x(nS), xp(nS), y(nS), yp(nS);
     for (int s = 0; s < nS; ++s) {
         const int X = h->ref_S[s], sg = X % 4, rest = X / 4, nu = rest % nO, mu = (rest / nO) % nO, i = rest / (nO * nO);
         x[s] = hrow(h, i, sa[sg], nu);
         xp[s] = hrow(h, i, sb[sg], mu);
+        y[s] = hrow(h, i, sa[sg], mu);
+        yp[s] = hrow(h, i, sb[sg], nu);
     }
     std::vector<int> R(x);
     R.insert(R.end(), xp.begin(), xp.end());
+    R.insert(R.end(), y.begin(), y.end());
+    R.insert(R.end(), yp.begin(), yp.end());
     std::sort(R.begin(), R.end());
     R.erase(std::unique(R.begin(), R.end()), R.end());
     const int nR = (int)R.size();
@@ -310,11 +429,13 @@ int refine_build(th_handle *h)
     std::vector<int> ord(nS);
     for (int s = 0; s < nS; ++s) ord[s] = s;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ridx[x[a]] < ridx[x[b]]; });
-    std::vector<int> Ss(nS), sx(nS), sxp(nS);
+    std::vector<int> Ss(nS), sx(nS), sxp(nS), sy(nS), syp(nS);
     for (int s = 0; s < nS; ++s) {
         Ss[s] = h->ref_S[ord[s]];
         sx[s] = ridx[x[ord[s]]];
         sxp[s] = ridx[xp[ord[s]]];
+        sy[s] = ridx[y[ord[s]]];
+        syp[s] = ridx[yp[ord[s]]];
     }
     // tiles of S rows: at most RF_NT * RF_ACC accumulator entries each
     const int rows = std::max(1, std::min(nS, RF_NT * RF_ACC / nS));
@@ -323,7 +444,21 @@ int refine_build(th_handle *h)
         const int s1 = std::min(nS, s0 + rows);
         tiles.insert(tiles.end(), {s0, s1, sx[s0], sx[s1 - 1] + 1});
     }
-    const int ntile = (int)tiles.size() / 4;
+    int ntile = (int)tiles.size() / 4;
+    // supports of at most 32 indices: one GEMM per k-point on the FP64 matrix cores
+    h->ref_mfma = nS <= 32 ? 1 : 0;
+    if (h->ref_mfma) {
+        const long fixed = (2L * nR * d + (long)d * d) * 16 + 2L * d * 8;
+        h->ref_smem = (size_t)(fixed + std::max(2L * RF_EC * d * 16, 32L * 32 * 16));
+        if (h->ref_smem > RF_LDS) h->ref_mfma = 0;
+    }
+    if (h->ref_mfma) {
+        ntile = 1;
+        h->ref_nxc = 1;
+        h->ref_nR = nR;
+        h->ref_ntile = 1;
+        tiles.assign({0, nS, 0, nR});
+    }
     // LDS: chunk of x values of C that fits next to V, K and the eigenvalues
     int dp = 0, fixed = 0;
     switch (d) {
@@ -332,18 +467,20 @@ int refine_build(th_handle *h)
     case 54: dp = refine_dp<54>(); fixed = refine_smem_fixed<54>(nR); break;
     default: return TH_E_UNSUPPORTED;
     }
-    const long avail = (long)RF_LDS - fixed;
-    const long per_x = (long)nR * dp * 16, ebuf = 2L * RF_EC * d * 16;
-    if (avail < std::max(per_x, ebuf)) return TH_E_UNSUPPORTED;
-    const int nxc = (int)std::max(1L, std::min((long)nR, avail / per_x));
-    h->ref_smem = (size_t)fixed + (size_t)std::max(ebuf, (long)nxc * per_x);
-    h->ref_nxc = nxc;
-    h->ref_nR = nR;
-    h->ref_ntile = ntile;
+    if (!h->ref_mfma) {
+        const long avail = (long)RF_LDS - fixed;
+        const long per_x = (long)nR * dp * 16, ebuf = 2L * RF_EC * d * 16;
+        if (avail < std::max(per_x, ebuf)) return TH_E_UNSUPPORTED;
+        const int nxc = (int)std::max(1L, std::min((long)nR, avail / per_x));
+        h->ref_smem = (size_t)fixed + (size_t)std::max(ebuf, (long)nxc * per_x);
+        h->ref_nxc = nxc;
+        h->ref_nR = nR;
+        h->ref_ntile = ntile;
+    }
     // k-point chunks: enough workgroups for the chip, partials within 256 MB
     const long nk = h->n_kuniq;
     const size_t blk = (size_t)nS * nS * sizeof(double2);
-    long nchunk = std::max(1L, std::min(nk, std::max(1L, 4096L / ntile)));
+    long nchunk = std::max(1L, std::min(nk, h->ref_mfma ? 1024L : std::max(1L, 4096L / ntile)));
     nchunk = std::max(1L, std::min(nchunk, (long)((256u << 20) / blk)));
     h->ref_kpc = (int)((nk + nchunk - 1) / nchunk);
     h->ref_nchunk = (int)((nk + h->ref_kpc - 1) / h->ref_kpc);
@@ -352,6 +489,8 @@ int refine_build(th_handle *h)
     ints.insert(ints.end(), R.begin(), R.end());
     ints.insert(ints.end(), sx.begin(), sx.end());
     ints.insert(ints.end(), sxp.begin(), sxp.end());
+    ints.insert(ints.end(), sy.begin(), sy.end());
+    ints.insert(ints.end(), syp.begin(), syp.end());
     ints.insert(ints.end(), tiles.begin(), tiles.end());
     ints.insert(ints.end(), Ss.begin(), Ss.end());
     hipFree(h->d_ref_int);
@@ -379,7 +518,8 @@ int refine_build(th_handle *h)
 int th_refine_prepare(th_handle *h, const double *q, int prec, bool *on)
 {
     *on = false;
-    if (prec != TH_PREC_MIXED || !h->refine || h->ref_S.empty() || (int)h->ref_S.size() > TH_REFINE_MAX_S) return TH_OK;
+    const bool want = (prec == TH_PREC_MIXED && h->refine) || (prec == TH_PREC_FP64 && h->refine == 2);
+    if (!want || h->ref_S.empty() || (int)h->ref_S.size() > TH_REFINE_MAX_S) return TH_OK;
     if (h->d != 18 && h->d != 36 && h->d != 54) return TH_OK;
     const bool qz = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
     if (!h->spectral || !h->eig_valid[0] || (!qz && !h->eig_valid[1]) || !h->d_kim_ptr) return TH_OK;
@@ -404,7 +544,9 @@ int th_refine_launch(th_handle *h, const double *q, double omega, const double *
     p.R = h->d_ref_int;
     p.sx = p.R + nR;
     p.sxp = p.sx + nS;
-    p.tiles = p.sxp + nS;
+    p.sy = p.sxp + nS;
+    p.syp = p.sy + nS;
+    p.tiles = p.syp + nS;
     const int *Ssorted = p.tiles + 4 * h->ref_ntile;
     p.nk = h->n_kuniq;
     p.kpc = h->ref_kpc;
@@ -420,17 +562,21 @@ int th_refine_launch(th_handle *h, const double *q, double omega, const double *
     p.part = reinterpret_cast<double2 *>(h->d_ref_part);
     const dim3 grid((unsigned)h->ref_nchunk, (unsigned)h->ref_ntile);
     const void *fn = nullptr;
+    const bool mf = h->ref_mfma != 0;
     switch (h->d) {
-    case 18: fn = reinterpret_cast<const void *>(refine_kernel<18>); break;
-    case 36: fn = reinterpret_cast<const void *>(refine_kernel<36>); break;
-    case 54: fn = reinterpret_cast<const void *>(refine_kernel<54>); break;
+    case 18: fn = mf ? reinterpret_cast<const void *>(refine_mfma_kernel<18>) : reinterpret_cast<const void *>(refine_kernel<18>); break;
+    case 36: fn = mf ? reinterpret_cast<const void *>(refine_mfma_kernel<36>) : reinterpret_cast<const void *>(refine_kernel<36>); break;
+    case 54: fn = mf ? reinterpret_cast<const void *>(refine_mfma_kernel<54>) : reinterpret_cast<const void *>(refine_kernel<54>); break;
     default: th_set_error("refinement built for d = 18, 36, 54"); return TH_E_UNSUPPORTED;
     }
     TH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ref_smem));
-    switch (h->d) {
-    case 18: hipLaunchKernelGGL(refine_kernel<18>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
-    case 36: hipLaunchKernelGGL(refine_kernel<36>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
-    default: hipLaunchKernelGGL(refine_kernel<54>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
+    switch (h->d * 2 + (mf ? 1 : 0)) {
+    case 36: hipLaunchKernelGGL(refine_kernel<18>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
+    case 37: hipLaunchKernelGGL(refine_mfma_kernel<18>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
+    case 72: hipLaunchKernelGGL(refine_kernel<36>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
+    case 73: hipLaunchKernelGGL(refine_mfma_kernel<36>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
+    case 108: hipLaunchKernelGGL(refine_kernel<54>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
+    default: hipLaunchKernelGGL(refine_mfma_kernel<54>, grid, dim3(RF_NT), h->ref_smem, h->stream, p); break;
     }
     TH_HIP(hipGetLastError());
     hipLaunchKernelGGL(refine_reduce_kernel, dim3((unsigned)((nS * nS + 255) / 256)), dim3(256), 0, h->stream,
