@@ -256,9 +256,12 @@ def main():
         for name, prec in (("mixed", B.PREC_MIXED), ("fp64", B.PREC_FP64)):
             if name == "fp64" and args.no_fp64:
                 continue
-            r = AT.tune(wl.h, run["q"], om_t, m.Ef, run["eta"], run["etap"], x2, p2, precision=prec, repeat=1)
+            r = AT.tune(wl.h, run["q"], om_t, m.Ef, run["eta"], run["etap"], x2, p2, precision=prec,
+                        repeat=2 if prec == B.PREC_MIXED else 1)
             per[name] = {"best": dict(r["best"]), "seconds": round(r["seconds"], 1),
                          "default_ms": r["table"][0][1], "best_ms": r["best_ms"]}
+            for st, ms in r["table"]:
+                log("autotune", name, st, "%.3f ms" % ms)
         tuning = {"best": per["mixed"]["best"], "sample_k": len(ws), "per_precision": per}
         log("autotune", tuning)
         wl.h.set_imag_points(np.tile(wl.kp, (len(wl.y), 1)), np.repeat(wl.y, wl.nk), np.outer(wl.wy, wl.wk).ravel())
@@ -316,7 +319,10 @@ def main():
     c3m = int(best.get("contract_3m", 0))
     if f16:
         # fp16 MFMA on two-term splits: 4 real products x 3 split products per complex entry
-        if mx["split"]:
+        if mx["split"] and int(best.get("contract_tiles", 1)):
+            kernel = ("contract_h2e_kernel (fp16 MFMA 16x16x32 on hi+lo splits written by the Green function "
+                      "kernels, 8 waves with 2-D register tiles)")
+        elif mx["split"]:
             kernel = "contract_h2p_kernel (fp16 MFMA 16x16x32 on hi+lo splits written by the Green function kernels)"
         else:
             kernel = "contract_h2_kernel (fp16 MFMA 16x16x32 on hi+lo splits of the FP32 panels)"

@@ -278,6 +278,11 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    the (hi, lo) fp16 words itself and the contraction copies
  *                    them (default; the same values as 0); 0 = FP32 panels
  *                    split in the contraction
+ *   "contract_tiles" contraction of presplit panels: 1 = eight waves per
+ *                    112 x 112 tile, two per SIMD, each holding a 2-D block of
+ *                    6 or 7 of the 49 16 x 16 output blocks, panels prefetched
+ *                    two stages ahead (default); 0 = seven waves, one 16-row
+ *                    strip each.  Results are bitwise the same for both
  *   "green_f16"      TH_PREC_MIXED spectral path: 1 = V diag(d) V^H on the fp16
  *                    matrix cores with two-term splits (default); 0 = FP32 MFMA
  *   "contract_3m"    1 = the contraction forms each complex product from three

@@ -743,3 +743,32 @@ def test_spectral_two_stream_pipeline_is_bitwise(precision):
     for a, b in zip(out[2], out[0]):
         print("precision", precision, "max |two streams - one| %.3e" % np.abs(a - b).max(), flush=True)
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("q,omega", [((0.0, 0.0, 0.0), 0.00078125), ((0.1, 0.0, 0.05), 0.0)])
+def test_tiled_contraction_is_bitwise_the_strip_kernel(q, omega):
+    """The split-panel contraction with eight waves holding 2-D blocks of the
+    tile (contract_tiles = 1) equals the seven-strip kernel (contract_tiles =
+    0) bit for bit: every output block takes the same MFMAs in the same order.
+    54 x 54 bcc supercell, small ragged batches on both axes, q = 0 and
+    q != 0, and again with the cached imaginary B panels."""
+    L = _hooks()
+    spec, run = CF.system(2), CF.run_params(2)
+    m = TBModel(spec)
+    kp, wk = Ct.uniform_kmesh(m.b, 6, 6, 6)
+    y, wy = Ct.imag_epoints(1, 12)
+    x2, p2 = Ct.real_epoints(m.Ef, 0.00078125, 1, 8)
+    out = {}
+    for t in (1, 0):
+        h = m.handle()
+        h.tune(batch_mb=16, contract_tiles=t)
+        h.set_imag_points(np.tile(kp, (len(y), 1)), np.repeat(y, len(wk)), np.outer(wy, wk).ravel())
+        h.set_real_kmesh(kp, wk)
+        out[t] = [h.chi0(q, omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=B.PREC_MIXED)
+                  for _ in range(2)]
+        assert L.thx_last_split(h._h) == 1
+        h.close()
+    for a, b in zip(out[1], out[0]):
+        print("q", q, "omega", omega, "max |2-D tiles - 7 strips| %.3e" % np.abs(a - b).max(), flush=True)
+        assert np.array_equal(a, b)

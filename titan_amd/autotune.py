@@ -12,6 +12,8 @@ of the library's tuning knobs (th_tune):
                  only, 2: also the spectral kernels)
   contract_f16   mixed precision: contraction on fp16 MFMA with two-term
                  splits of the FP32 panels (1) or on FP32 MFMA (0)
+  contract_tiles split-panel contraction with eight waves holding 2-D blocks
+                 of the output tile (1) or seven 16-row strips (0)
   contract_3m    three instead of four real MFMA products per complex one
                  (FP32/FP64 MFMA contraction)
   splits         K-splits of the contraction (workgroups per output tile)
@@ -21,7 +23,7 @@ and leaves the fastest setting on the handle.  The device time of each call
 comes from th_last_timing (HIP events on the library's stream).  Tuning is
 per handle: the best shapes depend on d = 2 nOrb nA, the number of atom
 pairs and the batch sizes.  Results are bitwise identical for every
-green_variant and overlap setting (same operations per matrix entry); splits
+green_variant, overlap and contract_tiles setting (same operations per matrix entry); splits
 and batch_mb change the order of the FP64 sums of the contraction only,
 contract_3m and spectral their rounding.
 """
@@ -49,6 +51,8 @@ def _knobs(h, base, setting):
     else:
         yield "overlap", [1, 2]
     yield "contract_f16", [1, 0]
+    if setting["contract_f16"]:
+        yield "contract_tiles", [1, 0]
     yield "contract_3m", [0, 1]
     yield "splits", sorted({max(1, s0 // 2), s0, 2 * s0})
     yield "batch_mb", [256, 512, 1024]
@@ -63,7 +67,7 @@ def tune(h, q, omega, Ef, eta, etap, x2=None, p2=None, precision=B.PREC_MIXED, r
     t0 = time.perf_counter()
     base = h.tuning()
     setting = {"spectral": 1 if h.d in SPECTRAL_D else 0, "green_variant": base["green_variant"],
-               "overlap": 1, "contract_f16": 1, "contract_3m": 0, "splits": base["splits"],
+               "overlap": 1, "contract_f16": 1, "contract_tiles": 1, "contract_3m": 0, "splits": base["splits"],
                "batch_mb": base["batch_mb"]}
     table = []
 

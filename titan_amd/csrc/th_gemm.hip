@@ -520,6 +520,229 @@ __global__ __launch_bounds__(NTH) void contract_h2p_kernel(const unsigned *__res
         }
 }
 
+// Building blocks of the eight-wave contraction below: one 16 x 16 complex
+// block's operands (re/im x hi/lo fragments, transposing reads) and its 12
+// fp16 MFMAs (hi.hi + hi.lo + lo.hi per real product, Ar Br and Ai Bi in
+// separate accumulators).
+struct h2_frags { hv8 rh, rl, ih, il; };
+
+__device__ __forceinline__ h2_frags h2_load(const _Float16 *L, int o)
+{
+    // L: [re|im][hi|lo][HK * HROW] of one operand in one buffer
+    h2_frags f;
+    f.rh = tr_frag(L + o);
+    f.rl = tr_frag(L + HK * HROW + o);
+    f.ih = tr_frag(L + 2 * HK * HROW + o);
+    f.il = tr_frag(L + 3 * HK * HROW + o);
+    return f;
+}
+
+__device__ __forceinline__ void h2_mac(const h2_frags &a, const h2_frags &b, fv4 &p, fv4 &n, fv4 &i)
+{
+    p = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.rh, b.rh, p, 0, 0, 0);
+    n = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.ih, b.ih, n, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.rh, b.ih, i, 0, 0, 0);
+    p = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.rh, b.rl, p, 0, 0, 0);
+    n = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.ih, b.il, n, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.rh, b.il, i, 0, 0, 0);
+    p = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.rl, b.rh, p, 0, 0, 0);
+    n = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.il, b.ih, n, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.rl, b.ih, i, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.ih, b.rh, i, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.ih, b.rl, i, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.il, b.rh, i, 0, 0, 0);
+}
+
+// one wave's rectangle of blocks: block rows R0 .. R0 + NR - 1, block
+// columns C0 .. C0 + NC - 1 of the tile
+template <int R0, int NR, int C0, int NC> struct h2_rect {
+    fv4 p[NR][NC], n[NR][NC], i[NR][NC];
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { p[r][c][q] = 0.0f; n[r][c][q] = 0.0f; i[r][c][q] = 0.0f; }
+    }
+    template <typename TRO>
+    __device__ __forceinline__ void step(const _Float16 *LA, const _Float16 *LB, TRO tro)
+    {
+        h2_frags a[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) a[r] = h2_load(LA, tro(R0 + r));
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const h2_frags b = h2_load(LB, tro(C0 + c));
+#pragma unroll
+            for (int r = 0; r < NR; ++r) h2_mac(a[r], b, p[r][c], n[r][c], i[r][c]);
+        }
+    }
+    __device__ __forceinline__ void store(double *pr_, double *pi_, int Mpad, int r0, int c0, int lane, double back)
+    {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = r0 + (R0 + r) * 16 + 4 * (lane >> 4) + q;
+                    const int col = c0 + (C0 + c) * 16 + (lane & 15);
+                    pr_[(size_t)row * Mpad + col] = ((double)p[r][c][q] - (double)n[r][c][q]) * back;
+                    pi_[(size_t)row * Mpad + col] = (double)i[r][c][q] * back;
+                }
+    }
+};
+
+// The same contraction with eight waves, two per SIMD (one wave's waits
+// are covered by the other's MFMAs), each holding a small 2-D block of the
+// 7 x 7 tile:
+//   waves 0-5: block rows 2 (w mod 3) .. +1 x block columns 3 (w / 3) .. +2  (6 blocks)
+//   wave 6:    block rows 0-5 x block column 6                              (6)
+//   wave 7:    block row 6 x block columns 0-6                              (7)
+// SIMD s runs waves s and s + 4: 12, 12, 12 and 13 blocks.  Per stage the
+// waves read 45 operand blocks from LDS (the seven-strip layout: 56).  A
+// stage is 1792 chunks of 16 bytes, 3.5 per thread (the fourth copy is
+// done by the first 256 threads).
+constexpr int NTH8 = 512;
+
+template <int W>
+__device__ __forceinline__ void h2e_body(const unsigned *__restrict__ A, const unsigned *__restrict__ B, long np,
+                                         int Mpad, int nA2, int ntile, int splits, long kper, int eA, int eB,
+                                         double *__restrict__ part, _Float16 (*Ls)[2][2][2][HK * HROW])
+{
+    const int nwg = gridDim.x;
+    const int b = blockIdx.x;
+    const int n8 = nwg & ~7;
+    const int lid = (b < n8) ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
+    const int tile = lid % (ntile * ntile);
+    const int rest = lid / (ntile * ntile);
+    const int split = rest % splits;
+    const int ij = rest / splits;
+    const int tm = tile / ntile, tn = tile - tm * ntile;
+    const int r0 = tm * TH_TILE, c0 = tn * TH_TILE;
+    const unsigned kb = (unsigned)(split * kper);
+    const unsigned ke = (unsigned)min(np, (long)kb + kper);
+    const int tid = threadIdx.x, lane = tid & 63;
+    auto swz = [](int p) { return 4 * (p & 3) + 16 * ((p >> 3) & 1); };
+    const unsigned ps2 = (unsigned)nA2 * 2u * (unsigned)Mpad * 4u;   // bytes per pair row
+    const int nrec = (int)(ke * ps2);
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, nrec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(reinterpret_cast<const char *>(B) + (long)(c0 - r0) * 4), (short)0, nrec, 0x00020000);
+
+    // staging: chunk u * 512 + tid of the stage's 1792 (u < 3 for every
+    // thread, u = 3 for tid < 256)
+    constexpr int NU = 4;
+    const bool last = tid < 1792 - 3 * NTH8;
+    unsigned goff[NU], loff[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int q = u < 3 || last ? u * NTH8 + tid : 0;
+        const int pr = q / 56, rem = q - pr * 56, c = rem / 28, ch = rem - c * 28;
+        goff[u] = (unsigned)pr * ps2 + (((unsigned)ij * 2u + (unsigned)c) * (unsigned)Mpad + (unsigned)(r0 + 4 * ch)) * 4u;
+        loff[u] = (unsigned)c * (2 * HK * HROW) + (unsigned)(pr * HROW + 4 * (ch ^ swz(pr)));
+    }
+    // two register sets: stage t is loaded into set t & 1 three stages
+    // before it is computed and copied to LDS one stage before (global loads
+    // under this traffic take longer than one stage of MFMA work)
+    uint4 sa2[2][NU], sb2[2][NU];
+    auto gload = [&](int set, unsigned k0) {
+        uint4(&sa)[NU] = sa2[set];
+        uint4(&sb)[NU] = sb2[set];
+        const unsigned st = k0 * ps2;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            sa[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rA, (int)(goff[u] + st), 0, 0));
+            sb[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rB, (int)(goff[u] + st), 0, 0));
+        }
+        if (last) {
+            sa[3] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rA, (int)(goff[3] + st), 0, 0));
+            sb[3] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rB, (int)(goff[3] + st), 0, 0));
+        }
+    };
+    auto put = [&](_Float16 *hl, const uint4 v) {
+        const uint2 hi = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x05040100u), __builtin_amdgcn_perm(v.w, v.z, 0x05040100u));
+        const uint2 lo = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x07060302u), __builtin_amdgcn_perm(v.w, v.z, 0x07060302u));
+        *reinterpret_cast<uint2 *>(hl) = hi;
+        *reinterpret_cast<uint2 *>(hl + HK * HROW) = lo;
+    };
+    auto sstore = [&](int buf, int set) {
+        const uint4(&sa)[NU] = sa2[set];
+        const uint4(&sb)[NU] = sb2[set];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            put(&Ls[buf][0][0][0][loff[u]], sa[u]);
+            put(&Ls[buf][1][0][0][loff[u]], sb[u]);
+        }
+        if (last) {
+            put(&Ls[buf][0][0][0][loff[3]], sa[3]);
+            put(&Ls[buf][1][0][0][loff[3]], sb[3]);
+        }
+    };
+    const int trp = (8 * (lane >> 4) + ((lane & 15) >> 2)) * HROW, trk = lane & 3;
+    const int trs = swz(8 * (lane >> 4) + ((lane & 15) >> 2));
+    auto tro = [=](int blk) { return trp + 4 * (((4 * blk) ^ trs) + trk); };
+
+    // this wave's blocks
+    constexpr int R0 = W < 6 ? 2 * (W % 3) : W == 6 ? 0 : 6;
+    constexpr int NR = W < 6 ? 2 : W == 6 ? 6 : 1;
+    constexpr int C0 = W < 6 ? 3 * (W / 3) : W == 6 ? 6 : 0;
+    constexpr int NC = W < 6 ? 3 : W == 6 ? 1 : 7;
+    h2_rect<R0, NR, C0, NC> m;
+    m.zero();
+
+    const unsigned nstage = (ke > kb) ? (ke - kb + HK - 1) / HK : 0u;
+    auto stage = [&](unsigned s, int odd) {
+        // stage s + 1 (set odd ^ 1) to LDS, stage s + 3 into that set; past
+        // the last stage the loads read 0 and the copy fills a buffer no
+        // longer read
+        sstore(odd ^ 1, odd ^ 1);
+        gload(odd ^ 1, kb + (s + 3) * HK);
+        __builtin_amdgcn_sched_barrier(0);
+        m.step(&Ls[odd][0][0][0][0], &Ls[odd][1][0][0][0], tro);
+        __syncthreads();   // next buffer complete, this one free
+    };
+    if (nstage > 0) {
+        gload(0, kb);
+        sstore(0, 0);
+        gload(1, kb + HK);
+        gload(0, kb + 2 * HK);
+        __syncthreads();
+    }
+    // unrolled by two so that the register set of every copy is known at
+    // compile time
+    unsigned s = 0;
+    for (; s + 1 < nstage; s += 2) {
+        stage(s, 0);
+        stage(s + 1, 1);
+    }
+    if (s < nstage) stage(s, 0);
+    const double back = ldexp(1.0, -(eA + eB));
+    double *pr_ = part + (((size_t)split * nA2 + ij) * 2 + 0) * Mpad * Mpad;
+    double *pi_ = pr_ + (size_t)Mpad * Mpad;
+    m.store(pr_, pi_, Mpad, r0, c0, lane, back);
+}
+
+__global__ __launch_bounds__(NTH8) __attribute__((amdgpu_waves_per_eu(2, 2))) void contract_h2e_kernel(
+    const unsigned *__restrict__ A, const unsigned *__restrict__ B, long np, int Mpad, int nA2, int ntile, int splits,
+    long kper, int eA, int eB, const double *__restrict__ p2, int n_x, double areal, double *__restrict__ part)
+{
+    __shared__ __align__(16) _Float16 Ls[2][2][2][2][HK * HROW];
+    if (p2) eA = th_real_exp(p2, n_x, areal);
+    switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+    case 0: h2e_body<0>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    case 1: h2e_body<1>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    case 2: h2e_body<2>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    case 3: h2e_body<3>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    case 4: h2e_body<4>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    case 5: h2e_body<5>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    case 6: h2e_body<6>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    default: h2e_body<7>(A, B, np, Mpad, nA2, ntile, splits, kper, eA, eB, part, Ls); break;
+    }
+}
+
 // Adjoint of the contraction (reverse mode).  With a cotangent That of one
 // axis's accumulator, T = sum_p A_p B_p^T gives
 //     Ahat_p = That conj(B_p),   Bhat_p = That^T conj(A_p)
@@ -764,7 +987,9 @@ int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int
     long kper = (np + splits - 1) / splits;
     kper = ((kper + TH_KC - 1) / TH_KC) * TH_KC;
     const int nwg = ntile * ntile * splits * h->nA2;
-    const bool fits = (unsigned long)(np + 96) * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 31);
+    // the split-panel kernels read up to 127 rows past a split's end (their
+    // prefetch; such reads return 0): the buffer offsets stay below 2^31
+    const bool fits = (unsigned long)(np + 128) * h->nA2 * 2 * h->Mpad * sizeof(float) < (1ul << 31);
     if (sc && sc->presplit) {
         if (sizeof(P) != sizeof(float) || !fits) {
             th_set_error("contraction: split half-precision panels need the mixed path and < 2 GiB per batch");
@@ -772,9 +997,14 @@ int launch_contract_t(th_handle *h, const void *pA, const void *pB, long np, int
         }
         long kh = (np + splits - 1) / splits;
         kh = ((kh + HK - 1) / HK) * HK;
-        hipLaunchKernelGGL(contract_h2p_kernel, dim3(nwg), dim3(NTH), 0, h->stream, (const unsigned *)pA,
-                           (const unsigned *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, sc->eA, sc->eB, sc->p2,
-                           sc->n_x, sc->areal, h->d_part);
+        if (h->contract_tiles)
+            hipLaunchKernelGGL(contract_h2e_kernel, dim3(nwg), dim3(NTH8), 0, h->stream, (const unsigned *)pA,
+                               (const unsigned *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, sc->eA, sc->eB, sc->p2,
+                               sc->n_x, sc->areal, h->d_part);
+        else
+            hipLaunchKernelGGL(contract_h2p_kernel, dim3(nwg), dim3(NTH), 0, h->stream, (const unsigned *)pA,
+                               (const unsigned *)pB, np, h->Mpad, h->nA2, ntile, splits, kh, sc->eA, sc->eB, sc->p2,
+                               sc->n_x, sc->areal, h->d_part);
     } else if (sizeof(P) == sizeof(float) && sc && h->contract_f16 && fits) {
         long kh = (np + splits - 1) / splits;
         kh = ((kh + HK - 1) / HK) * HK;
