@@ -4,6 +4,7 @@ Reads only data files of the reference's CI example (no code is run):
   examples/nanoribbon/{input,basis,A2,A3}
   examples/nanoribbon/results_correct/TSOC/selfconsistency/*.dat
   examples/nanoribbon/results_correct/TSOC/10Sites/{HF,RPA}/chi{hf,}_i_j_*.dat
+  examples/nanoribbon/results_correct/TSOC/10Sites/{CD,SD,LD,Beff,SOT}/*.dat
 
 10-site 1D zigzag ribbon, two element types, p_x/p_y orbitals, SOC on p,
 U = 0, Ef = -0.25 (input key 'Ef', mod_self_consistency.F90:76), one
@@ -107,6 +108,26 @@ def main(ref):
                                                   for n in range(4)]}
         return out
 
+    def read_responses():
+        # per-site and total responses to the in-plane electric field (one
+        # frequency): file <name>_pos=<i>_... or <name>_total_..., columns
+        # energy, amplitude, real part, imag part, ...
+        # (mod_disturbances.F90:157-264, mod_beff.F90:114-167, mod_torques)
+        out = {}
+        for kind in ("CD", "SD", "LD", "Beff", "SOT"):
+            for f in sorted(glob.glob(os.path.join(ex, "results_correct", "TSOC", "10Sites", kind, "*.dat"))):
+                base = os.path.basename(f)
+                m = re.match(r"([A-Za-z0-9]+)_(pos=(\d+)|total)_", base)
+                if not m:
+                    continue
+                row = [_floats(l) for l in open(f) if l.strip() and not l.strip().startswith("#")][0]
+                ent = out.setdefault(m.group(1), {"pos": [None] * nA, "total": None})
+                if m.group(3):
+                    ent["pos"][int(m.group(3)) - 1] = row[2:4]
+                else:
+                    ent["total"] = row[2:4]
+        return out
+
     out = {
         "source": "examples/nanoribbon (input, basis, A2, A3, results_correct) of TITAN-FZJ/TITAN",
         "system": {
@@ -119,14 +140,16 @@ def main(ref):
         "run": {"eta": float(inp["eta"]), "etap": float(inp.get("etap", inp["eta"])), "nkpt": int(inp["nkpt"]),
                 "parts": int(inp["parts"]), "n1gl": int(inp["n1gl"]), "parts3": int(inp["parts3"]),
                 "n3gl": int(inp["n3gl"]), "emin": float(inp["emin"].replace("d", "e")),
-                "emax": float(inp["emax"].replace("d", "e")), "nEner": int(inp["nEner"]), "q": [0.0, 0.0, 0.0]},
+                "emax": float(inp["emax"].replace("d", "e")), "nEner": int(inp["nEner"]), "q": [0.0, 0.0, 0.0],
+                "ebasis": inp["ebasis"], "dirEfield": _floats(inp["dirEfield"])[:3]},
         "chihf": read_blocks("HF", "chihf"),
         "chi": read_blocks("RPA", "chi"),
+        "responses": read_responses(),
     }
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "nanoribbon.json")
     with open(dst, "w") as fh:
         json.dump(out, fh)
-    print("wrote", dst, len(out["chihf"]), len(out["chi"]))
+    print("wrote", dst, len(out["chihf"]), len(out["chi"]), sorted(out["responses"]))
 
 
 if __name__ == "__main__":
