@@ -206,7 +206,10 @@ int th_expectation_values(th_handle *h, double Ef, double eta, double *rho, doub
  * in the 16 spin-block terms of the reference, summed with the pair weights,
  * multiplied by prefactor (D x D complex128 column-major, e.g.
  * (I + chi_hf U)^-1; NULL = identity) and divided by 2 pi.
- * out: D x 4 complex128 column-major (TITAN's tchiorbiikl).  FP64; d <= 54. */
+ * out: D x 4 complex128 column-major (TITAN's tchiorbiikl).  FP64, any d: up
+ * to d = 54 one kernel per chunk of pairs holds a pair's G, Gq and products
+ * in LDS; beyond (or with the tuning key "current_general") the products go
+ * through device memory in three stages with the same sums. */
 int th_current_response(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
                         int n_x, const double *x2, const double *p2, const double *ue, const double *prefactor,
                         double *out);
@@ -304,6 +307,10 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    cache and the eigenpairs, so that the next call computes
  *                    them again into the allocations it keeps (timing from a
  *                    cold start without the one-time allocations)
+ *   "current_general" th_current_response: 1 = the three-stage form through
+ *                    device memory for every d; 0 = the one-kernel form where a
+ *                    pair fits the LDS (d <= 54, default).  The sums are the
+ *                    same up to the order of the two imaginary-axis terms
  *   "refine"         TH_PREC_MIXED with a declared U (th_set_interaction):
  *                    1 = chi0[S,S] on the support of U recomputed in FP64
  *                    (default); 0 = left as the mixed contraction gives it;

@@ -1050,6 +1050,9 @@ int th_tune(th_handle *h, const char *key, long value)
     } else if (k == "overlap") {
         if (value < 0 || value > 2) { th_set_error("th_tune: overlap is 0, 1 or 2"); return TH_E_ARG; }
         h->overlap = (int)value;
+    } else if (k == "current_general") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: current_general is 0 or 1"); return TH_E_ARG; }
+        h->current_general = (int)value;
     } else if (k == "contract_tiles") {
         if (value < 0 || value > 1) { th_set_error("th_tune: contract_tiles is 0 or 1"); return TH_E_ARG; }
         h->contract_tiles = (int)value;

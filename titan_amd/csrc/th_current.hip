@@ -200,6 +200,142 @@ __global__ __launch_bounds__(TH_CU_NT) void current_pair_kernel(current_params c
     }
 }
 
+// Any d (the kernel above keeps P, Q and Z in LDS and the D x 4 outputs in
+// registers, which holds to d = 54): the same sums in three stages through
+// device memory.  T per pair, then Z = P~ (1 (x) T) per pair and term, then
+// per chunk of pairs the diagonal atom blocks of Z Q~, with the same tiles,
+// sums and order of pairs as current_pair_kernel.
+
+// T[pl][jg][lx] = sum over hops (j, l) of t i (u.R) e^{i k.R}
+__global__ __launch_bounds__(256) void current_t_kernel(current_params c, double2 *__restrict__ Tg)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2 *ph = reinterpret_cast<double2 *>(smem);
+    const int n = c.d / 2, nO = c.nO, nA = c.nA, tid = threadIdx.x;
+    const long pl = blockIdx.x, p = c.p0 + pl;
+    double k0, k1, k2;
+    if (c.real) {
+        const long ik = p - (p / c.nk_re) * c.nk_re;
+        k0 = c.kre[3 * ik]; k1 = c.kre[3 * ik + 1]; k2 = c.kre[3 * ik + 2];
+    } else {
+        k0 = c.kim[3 * p]; k1 = c.kim[3 * p + 1]; k2 = c.kim[3 * p + 2];
+    }
+    for (int m = tid; m < c.nhop; m += 256) {
+        const double a = k0 * c.R[3 * m] + k1 * c.R[3 * m + 1] + k2 * c.R[3 * m + 2];
+        const double v = c.u0 * c.R[3 * m] + c.u1 * c.R[3 * m + 1] + c.u2 * c.R[3 * m + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[m] = make_double2(-v * sn, v * cs);
+    }
+    __syncthreads();
+    double2 *T = Tg + (size_t)pl * n * n;
+    for (int e = tid; e < n * n; e += 256) {
+        const int rg = e / n, cx = e - (e / n) * n;
+        const int j = rg / nO, g = rg - j * nO, l = cx / nO, x = cx - l * nO;
+        double2 s = make_double2(0.0, 0.0);
+        for (int m = c.hoff[j * nA + l]; m < c.hoff[j * nA + l + 1]; ++m) {
+            const double2 tv = zmul(c.t[(size_t)m * nO * nO + g + nO * x], ph[m]);
+            s.x += tv.x;
+            s.y += tv.y;
+        }
+        T[e] = s;
+    }
+}
+
+// Z[pl][r][cg n + lx] = sum_{jg} P~[r][(2 j + cg) nO + g] T[jg][lx], 2 x 2 tiles
+// (blockIdx.x = pair, blockIdx.y = group of 256 tiles)
+__global__ __launch_bounds__(256) void current_z_kernel(current_params c, const double2 *__restrict__ rawq,
+                                                        const double2 *__restrict__ Tg, int term,
+                                                        double2 *__restrict__ Zg)
+{
+    const int d = c.d, n = d / 2, nO = c.nO, h2 = d / 2;
+    const long pl = blockIdx.x;
+    const int tz = blockIdx.y * 256 + threadIdx.x;
+    if (tz >= h2 * h2) return;
+    const double2 *P = rawq + (size_t)pl * d * d, *T = Tg + (size_t)pl * n * n;
+    double2 *Z = Zg + (size_t)pl * d * d;
+    const int r0 = 2 * (tz / h2), cA = 2 * (tz - (tz / h2) * h2), cB = cA + 1;
+    const int gA = cA / n, lA = cA - gA * n, gB = cB / n, lB = cB - gB * n;
+    double2 z00 = make_double2(0.0, 0.0), z01 = z00, z10 = z00, z11 = z00;
+    for (int jg = 0; jg < n; ++jg) {
+        const int j = jg / nO, g = jg - j * nO;
+        const int colA = (2 * j + gA) * nO + g, colB = (2 * j + gB) * nO + g;
+        const double2 p0A = ptil(P, d, r0, colA, c.real, term), p1A = ptil(P, d, r0 + 1, colA, c.real, term);
+        const double2 p0B = gB == gA ? p0A : ptil(P, d, r0, colB, c.real, term);
+        const double2 p1B = gB == gA ? p1A : ptil(P, d, r0 + 1, colB, c.real, term);
+        const double2 tA = T[jg * n + lA], tB = T[jg * n + lB];
+        zfma(z00, p0A, tA);
+        zfma(z01, p0B, tB);
+        zfma(z10, p1A, tA);
+        zfma(z11, p1B, tB);
+    }
+    Z[r0 * d + cA] = z00;
+    Z[r0 * d + cB] = z01;
+    Z[(r0 + 1) * d + cA] = z10;
+    Z[(r0 + 1) * d + cB] = z11;
+}
+
+// partial[chunk][o] (+)= sum over the chunk's pairs of w O, O the diagonal
+// atom blocks of Z Q~ in 2 x 2 tiles (one tile per thread, blockIdx.y = chunk)
+__global__ __launch_bounds__(256) void current_o_kernel(current_params c, const double2 *__restrict__ Zg,
+                                                        const double2 *__restrict__ raw, int herm, int add,
+                                                        double2 *__restrict__ partial)
+{
+    const int d = c.d, n = d / 2, nO = c.nO, nA = c.nA;
+    const int D = 4 * nA * nO * nO, nout = 4 * D, ntile = nout / 4;
+    const int tt = blockIdx.x * 256 + threadIdx.x;
+    if (tt >= ntile) return;
+    const long b0 = (long)blockIdx.y * c.chunk;
+    const long b1 = b0 + c.chunk < c.np ? b0 + c.chunk : c.np;
+    int i, sp, ta, tb;
+    tile_of(tt, nO, i, sp, ta, tb);
+    const int cg = 1 - (sp & 1), rx = sp >> 1;
+    const int a0 = 2 * i * nO + 2 * ta, bq = 2 * i * nO + 2 * tb;
+    double2 acc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = make_double2(0.0, 0.0);
+    for (long pl = b0; pl < b1; ++pl) {
+        const long p = c.p0 + pl;
+        double w;
+        if (c.real) {
+            const long ie = p / c.nk_re, ik = p - (p / c.nk_re) * c.nk_re;
+            w = c.p2[ie] * c.wre[ik];
+        } else {
+            w = c.wim[p];
+        }
+        const double2 *Q = raw + (size_t)pl * d * d;
+        const double2 *z0 = Zg + (size_t)pl * d * d + (size_t)a0 * d + cg * n, *z1 = z0 + d;
+        double2 o00 = make_double2(0.0, 0.0), o01 = o00, o10 = o00, o11 = o00;
+        for (int lx = 0; lx < n; ++lx) {
+            const int l = lx / nO, x = lx - l * nO, qr = (2 * l + rx) * nO + x;
+            const double2 q0 = herm ? zconj(Q[bq * d + qr]) : Q[qr * d + bq];
+            const double2 q1 = herm ? zconj(Q[(bq + 1) * d + qr]) : Q[qr * d + bq + 1];
+            const double2 za = z0[lx], zb = z1[lx];
+            zfma(o00, za, q0);
+            zfma(o01, za, q1);
+            zfma(o10, zb, q0);
+            zfma(o11, zb, q1);
+        }
+        acc[0].x += w * o00.x; acc[0].y += w * o00.y;
+        acc[1].x += w * o01.x; acc[1].y += w * o01.y;
+        acc[2].x += w * o10.x; acc[2].y += w * o10.y;
+        acc[3].x += w * o11.x; acc[3].y += w * o11.y;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = 2 * ta + (e >> 1), b = 2 * tb + (e & 1);
+        const int rn = a / nO, nu = a - rn * nO, cm = b / nO, mu = b - cm * nO;
+        const int X = 4 * ((i * nO + mu) * nO + nu) + 2 * cm + 1 - rn;
+        double2 *o = partial + (size_t)blockIdx.y * nout + X + (size_t)D * sp;
+        if (add) {
+            o->x += acc[e].x;
+            o->y += acc[e].y;
+        } else {
+            *o = acc[e];
+        }
+    }
+}
+
 // acc[o] += sum over chunks of partial (fixed order)
 __global__ void current_sum_kernel(const double2 *partial, long nchunk, int nout, double2 *acc)
 {
@@ -248,20 +384,21 @@ extern "C" int th_current_response(th_handle *h, const double *q, double omega, 
         th_set_error("th_set_imag_points must be called before th_current_response");
         return TH_E_STATE;
     }
-    if (h->d > 54) {
-        th_set_error("th_current_response: d = " + std::to_string(h->d) + " > 54 is not supported");
-        return TH_E_UNSUPPORTED;
-    }
     const bool real_axis = std::fabs(omega) >= 1e-15 && n_x > 0;
     if (real_axis && (!h->d_kre || h->nk_re <= 0)) {
         th_set_error("th_set_real_kmesh must be called before th_current_response at omega != 0");
         return TH_E_STATE;
     }
-    const int d = h->d, D = h->D, nout = 4 * D;
+    const int d = h->d, D = h->D, nout = 4 * D, n = d / 2;
     int rc = 0;
     if (real_axis && (rc = th_upload_nodes(h, n_x, x2, p2))) return rc;
-    // pairs per batch: the two raw G (d^2 complex each) within the panel budget
-    const size_t per = (size_t)2 * d * d * sizeof(double2);
+    // the LDS kernel holds P, Q, Z and T of a pair and D x 4 outputs in
+    // registers; otherwise (or by request) the three-stage form
+    const size_t lds = sizeof(double2) * ((size_t)3 * d * d + (size_t)n * n + (size_t)std::max(1, h->nhop));
+    const bool general = h->current_general || lds > 160 * 1024 || D > TH_CU_RT * TH_CU_NT;
+    // pairs per batch: the two raw G (d^2 complex each), and in the three-stage
+    // form Z (d^2) and T ((d/2)^2), within the panel budget
+    const size_t per = (size_t)(general ? 3 : 2) * d * d * sizeof(double2) + (general ? (size_t)n * n * sizeof(double2) : 0);
     const long n_re = real_axis ? (long)n_x * h->nk_re : 0;
     const long bp = std::max<long>(1, std::min<long>(std::max(h->n_im, n_re), (long)(((size_t)h->batch_mb << 20) / per)));
     const long nch_max = std::min<long>(bp, TH_CU_WG);
@@ -276,15 +413,15 @@ extern "C" int th_current_response(th_handle *h, const double *q, double omega, 
         return TH_E_HIP;
     }
     double2 *d_rawq = d_raw, *d_rawg = d_raw + (size_t)bp * d * d;
-    const int n = d / 2;
-    const size_t lds = sizeof(double2) * ((size_t)3 * d * d + (size_t)n * n + (size_t)std::max(1, h->nhop));
-    if (lds > 160 * 1024) {
+    double2 *d_Z = d_rawg + (size_t)bp * d * d, *d_T = d_Z + (size_t)bp * d * d;   // three-stage form only
+    const size_t lds_t = sizeof(double2) * (size_t)std::max(1, h->nhop);
+    if (general && lds_t > 64 * 1024) {
         cleanup();
-        th_set_error("th_current_response: workgroup memory exceeds the CU's LDS");
+        th_set_error("th_current_response: too many hops for the phase table in LDS");
         return TH_E_UNSUPPORTED;
     }
-    if (hipFuncSetAttribute((const void *)current_pair_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-        hipSuccess) {
+    if (!general && hipFuncSetAttribute((const void *)current_pair_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds) != hipSuccess) {
         cleanup();
         (void)hipGetLastError();
         th_set_error("th_current_response: workgroup memory attribute refused");
@@ -319,8 +456,20 @@ extern "C" int th_current_response(th_handle *h, const double *q, double omega, 
             cp.np = np; cp.p0 = p0; cp.real = axis;
             cp.chunk = (np + TH_CU_WG - 1) / TH_CU_WG;
             const long nch = (np + cp.chunk - 1) / cp.chunk;
-            hipLaunchKernelGGL(current_pair_kernel, dim3((unsigned)nch), dim3(TH_CU_NT), lds, h->stream, cp,
-                               (const double2 *)d_rawq, (const double2 *)d_rawg, d_part);
+            if (general) {
+                hipLaunchKernelGGL(current_t_kernel, dim3((unsigned)np), dim3(256), lds_t, h->stream, cp, d_T);
+                const unsigned gz = (unsigned)(((d / 2) * (d / 2) + 255) / 256), go = (unsigned)((D + 255) / 256);
+                for (int term = 0; term < (axis ? 1 : 2); ++term) {
+                    hipLaunchKernelGGL(current_z_kernel, dim3((unsigned)np, gz), dim3(256), 0, h->stream, cp,
+                                       (const double2 *)d_rawq, (const double2 *)d_T, term, d_Z);
+                    hipLaunchKernelGGL(current_o_kernel, dim3(go, (unsigned)nch), dim3(256), 0, h->stream, cp,
+                                       (const double2 *)d_Z, (const double2 *)d_rawg, (axis || term) ? 1 : 0,
+                                       term, d_part);
+                }
+            } else {
+                hipLaunchKernelGGL(current_pair_kernel, dim3((unsigned)nch), dim3(TH_CU_NT), lds, h->stream, cp,
+                                   (const double2 *)d_rawq, (const double2 *)d_rawg, d_part);
+            }
             hipLaunchKernelGGL(current_sum_kernel, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, h->stream,
                                (const double2 *)d_part, nch, nout, d_acc);
             if (hipGetLastError() != hipSuccess) { rc = TH_E_HIP; th_set_error("th_current_response: kernel launch"); }

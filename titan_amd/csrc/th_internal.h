@@ -87,6 +87,7 @@ struct th_handle {
     int spectral = 1;            // use them when possible
     int spectral_chunk = 1;      // spectral G for runs of pairs per k-point (V loaded once per run)
     int contract_f16 = 1;        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
+    int current_general = 0;     // th_current_response: three-stage form for every d (1) or only where LDS cannot hold a pair (0)
     int contract_tiles = 1;      // split-panel contraction: 8 waves with 2-D register tiles (1) or 7 strips (0)
     int presplit = 1;            // mixed spectral: Green function kernels write the fp16 splits themselves
     int last_presplit = 0;       // the last chi0 did
