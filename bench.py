@@ -176,6 +176,62 @@ def current_response_cost(cfg=2, nk=16, omega=4e-3, reps=2):
                              "sample": "4 imaginary + 2 real pairs of the same workload by the oracle (numpy), %.1f s" % tc}}
 
 
+def other_configs(log_fn):
+    """Throughput of the other configurations (reported beside the headline,
+    each at its own size): cfg0, the 1-site bcc element (18 x 18 H, 16^3 k,
+    64 + 32 energies, all 16 frequencies); cfg1, the 8-layer fcc(001) slab
+    (144 x 144 H, every 4th of its 128^2 k-points, one frequency); cfg4, the
+    40-cell uneven batch (18 x 18 to 288 x 288, one frequency).  Library
+    default kernel shapes; each timed sweep after an untimed one; for cfg0
+    also the oracle on a bounded sample (the configuration the CPU baseline
+    is specified on)."""
+    from titan_amd import binding as B
+    from titan_amd import configs as CF
+    from titan_amd.batch import UnevenBatch
+    from titan_amd.chi import ChiWorkload
+    out = {}
+    for cfg, kstep, nom in ((0, 1, None), (1, 4, 1)):
+        spec, run = CF.system(cfg), CF.run_params(cfg)
+        wl = ChiWorkload(spec, run)
+        if kstep > 1:
+            ks, ws = wl.kp[::kstep], wl.wk[::kstep] * kstep
+            wl.kp, wl.wk, wl.nk = ks, ws, len(ws)
+            wl.h.set_imag_points(np.tile(ks, (len(wl.y), 1)), np.repeat(wl.y, wl.nk), np.outer(wl.wy, ws).ravel())
+            wl.h.set_real_kmesh(ks, ws)
+        om = np.asarray(run["omega"], dtype=float)
+        om = om if nom is None else om[-nom:]
+        ent = {"workload": "cfg%d: %s, H %dx%d, %d of the %s k-points, %d imag + %d real energies, %d omega" %
+                           (cfg, spec["name"], wl.model.d, wl.model.d, wl.nk, "x".join(map(str, run["nk"])),
+                            len(wl.y), wl.n_x, len(om)),
+               "pairs": wl.pairs(om)}
+        for name, prec in (("mixed", B.PREC_MIXED), ("fp64", B.PREC_FP64)):
+            wl.sweep(om[:1], precision=prec)
+            wl.h.tune(drop_caches=1)
+            t0 = time.perf_counter()
+            wl.sweep(om, precision=prec)
+            dt = time.perf_counter() - t0
+            ent[name] = {"value": ent["pairs"] / dt, "unit": "(k,E)-points/s", "seconds": dt,
+                         "contraction_tflops_alg": wl.contraction_flops(om) / dt / 1e12,
+                         "stage_ms": {k: wl.h.last_timing()[k] for k in ("green_ms", "contract_ms", "total_ms")}}
+        wl.close()
+        if cfg == 0:
+            # the CPU baseline's configuration: the oracle on a bounded sample
+            ent["cpu_baseline"] = cpu_baseline(spec, run, float(om[-1]), target_s=6.0)
+        out["cfg%d" % cfg] = ent
+        log_fn("cfg%d" % cfg, ent)
+    ub = UnevenBatch(CF.uneven_batch())
+    ub.sweep(precision=B.PREC_MIXED, omegas=[0.01])
+    _, dt, pairs = ub.sweep(precision=B.PREC_MIXED, omegas=[0.01])
+    sizes = sorted({c.model.d for c in ub.cells})
+    out["cfg4"] = {"workload": "cfg4: 40 bcc cells of 1-16 sites (H %s), per-cell k-mesh 24^3 to 8^3, 48 imag + 16 "
+                               "real energies, one omega, cells concurrent on their own streams"
+                               % "/".join(map(str, sizes)),
+                   "pairs": pairs, "mixed": {"value": pairs / dt, "unit": "(k,E)-points/s", "seconds": dt}}
+    ub.close()
+    log_fn("cfg4", out["cfg4"])
+    return out
+
+
 def _last_split(h):
     """1 when the last chi0 of h had the Green function kernels write split fp16 panels."""
     from titan_amd import binding
@@ -205,6 +261,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-current", action="store_true", help="skip the spin-current response measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default kernel shapes")
+    ap.add_argument("--no-extras", action="store_true", help="skip the cfg0 / cfg1 / cfg4 throughput section")
     ap.add_argument("--tuning", default=None,
                     help="JSON of th_tune settings to use instead of tuning (e.g. for profiler runs); "
                          "either one setting or {\"mixed\": {...}, \"fp64\": {...}}")
@@ -410,6 +467,12 @@ def main():
         except Exception as e:   # reported, never required
             log("current response failed:", e)
             out["current_response"] = None
+    if not args.no_extras:
+        try:
+            out["other_configs"] = other_configs(log)
+        except Exception as e:   # reported, never required
+            log("other configurations failed:", e)
+            out["other_configs"] = None
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(spec, run, float(om_time[-1]))

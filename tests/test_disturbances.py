@@ -61,3 +61,32 @@ def test_magnetisation_angles_and_rotation():
     # the frame of a moment along +x: L_z' = L_x (theta = 90, phi = 0)
     lp = Dz.rotated_lvec(lv, 90.0, 0.0)
     assert np.allclose(lp[2], lv[0]) and np.allclose(lp[0], -lv[2]) and np.allclose(lp[1], lv[1])
+
+
+def test_response_rows_follow_the_reference_format(tmp_path):
+    """Rows as mod_disturbances.F90:205-220 writes them: phase, real/amplitude
+    and imaginary/amplitude at or below 1e-15 written as 0, nine es16.9
+    fields; the row of the nanoribbon's Beff_z at site 1 reproduced."""
+    from titan_amd import titan_io as IO
+    z = complex(-1.326220122e-04, 1.808964766e-05)
+    row = IO.response_line(1e-6, z, 0.0, 0.0)
+    want = ("1.000000000E-06   1.338500410E-04  -1.326220122E-04   1.808964766E-05   3.006029201E+00"
+            "   0.000000000E+00   1.351486151E-01   0.000000000E+00   0.000000000E+00")
+    # the value comes from the file's rounded re/im, so the derived fields
+    # agree to the 10-digit rounding; the layout exactly
+    assert np.allclose([float(v) for v in row.split()], [float(v) for v in want.split()], rtol=1e-8, atol=0)
+    assert [f for f in row.split(" ") if f][0] == "1.000000000E-06" and len(row) == 9 * 16 + 8 * 2
+    vals = {n: (np.array([1.0 + 2.0j, 0.0]), 3.0) for _, n, _ in IO.RESPONSE_FILES}
+    paths = IO.write_response_files(str(tmp_path), [0.5], [vals], [([0.0, 0.0], [0.0, 0.0])], [(0.0, 0.0)], "_x")
+    assert len(paths) == 17 * 3
+    r = IO.read_data(str(tmp_path / "SD" / "Sz_pos=1_x.dat"))[0]
+    assert np.allclose(r[:4], [0.5, abs(1 + 2j), 1.0, 2.0])
+    assert IO.read_data(str(tmp_path / "SOT" / "XCTz_total_x.dat"))[0][2] == 3.0
+
+
+def test_field_direction_bases():
+    from titan_amd import decks as D
+    a = [[1.0, 0.0, 0.0], [0.5, np.sqrt(3) / 2, 0.0], [0.0, 0.0, 2.0]]
+    assert np.allclose(D.field_direction("cartesian", ["2", "0", "0"], a), [1, 0, 0])
+    assert np.allclose(D.field_direction("bravais", ["1", "1"], a), np.array([1.5, np.sqrt(3) / 2, 0]) / np.sqrt(3))
+    assert np.allclose(D.field_direction("spherical", ["90", "90"], a), [0, 1, 0])

@@ -13,6 +13,7 @@ import pytest
 
 from titan_amd import contour as Ct
 from titan_amd import disturbances as Dz
+from titan_amd import titan_io as IO
 from titan_amd.system import TBModel
 
 pytestmark = pytest.mark.gpu
@@ -22,7 +23,7 @@ FAMILIES = {"Cd": ("Cd",), "S": ("Sx", "Sy", "Sz"), "L": ("Lx", "Ly", "Lz"),
             "XCT": ("XCTx", "XCTy", "XCTz")}
 
 
-def test_ribbon_responses_match_reference_files(ribbon_fixture):
+def test_ribbon_responses_match_reference_files(ribbon_fixture, tmp_path):
     d = ribbon_fixture
     spec, run = d["system"], d["run"]
     m = TBModel(spec)
@@ -59,6 +60,14 @@ def test_ribbon_responses_match_reference_files(ribbon_fixture):
             "SOTx": (T[0, 0], Ttot[0, 0]), "SOTy": (T[0, 1], Ttot[0, 1]), "SOTz": (T[0, 2], Ttot[0, 2]),
             "XCTx": (T[1, 0], Ttot[1, 0]), "XCTy": (T[1, 1], Ttot[1, 1]), "XCTz": (T[1, 2], Ttot[1, 2])}
     ref = d["responses"]
+    # the files as the reference writes them, read back by the comparator's reader
+    vals = Dz.response_values(dist, dtot, beff, btot, T, Ttot)
+    paths = IO.write_response_files(str(tmp_path), [om], [vals], [(th, ph)], [(0.0, 0.0)])
+    assert len(paths) == 17 * (nA + 1)
+    for k in ("Cd", "Sz", "Lz", "Beffz"):
+        row = IO.read_data(str(tmp_path / ("SD" if k[0] == "S" else "LD" if k[0] == "L" else
+                                           "CD" if k == "Cd" else "Beff") / ("%s_pos=1.dat" % k)))[0]
+        assert np.allclose(row[2:4], ref[k]["pos"][0], rtol=1e-8, atol=1e-8 * np.abs(ref[k]["pos"][0]).max())
     checked = 0
     for fam, kinds in FAMILIES.items():
         refs = {k: (np.array([complex(*v) for v in ref[k]["pos"]]), complex(*ref[k]["total"])) for k in kinds}

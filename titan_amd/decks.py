@@ -33,6 +33,8 @@ are outside this package.
 import glob
 import os
 
+import numpy as np
+
 ORBITALS = {"s": 1, "px": 2, "py": 3, "pz": 4, "dxy": 5, "dyz": 6, "dzx": 7, "dx2": 8, "dz2": 9,
             "p": (2, 3, 4), "d": (5, 6, 7, 8, 9), "t2g": (5, 6, 7), "eg": (8, 9)}
 
@@ -255,6 +257,32 @@ def mesh_from_input(nkpt_words, sysdim):
     return dims, total
 
 
+def field_direction(ebasis, dir_efield, a):
+    """Unit vector of the in-plane electric field (mod_io.F90:297-324,
+    ElectricField.F90:20-50): 'cartesian' takes three components, 'bravais'
+    two coefficients of the first two lattice vectors, 'spherical' the polar
+    and azimuthal angles in degrees; 'neighbor' is not implemented by the
+    reference either."""
+    v = [fortran_float(x) for x in dir_efield]
+    a = np.asarray(a, dtype=float).reshape(3, 3)
+    if ebasis == "cartesian":
+        if len(v) != 3:
+            raise ValueError("dirEfield: 3 components required for ebasis = cartesian")
+        e = np.array(v, dtype=float)
+    elif ebasis == "bravais":
+        if len(v) != 2:
+            raise ValueError("dirEfield: 2 coefficients required for ebasis = bravais")
+        e = v[0] * a[0] + v[1] * a[1]
+    elif ebasis == "spherical":
+        if len(v) != 2:
+            raise ValueError("dirEfield: 2 angles required for ebasis = spherical")
+        t, ph = np.radians(v[0]), np.radians(v[1])
+        e = np.array([np.cos(ph) * np.sin(t), np.sin(ph) * np.sin(t), np.cos(t)])
+    else:
+        raise ValueError("ebasis = %s: field along atom directions is not implemented" % ebasis)
+    return (e / np.linalg.norm(e)).tolist()
+
+
 def load_deck(directory, sc_file=None):
     """{"system": TBModel parameters, "run": chi(omega) run parameters} of a
     TITAN run directory, set up as the reference does before its chi(omega)
@@ -341,6 +369,13 @@ def load_deck(directory, sc_file=None):
            "n3gl": int(_get(inp, "n3gl", "64")), "emin": fortran_float(_get(inp, "emin", "0")),
            "emax": fortran_float(_get(inp, "emax", "0")), "nEner": int(_get(inp, "nEner", "1")),
            "q": [0.0, 0.0, 0.0]}
+    # direction of the in-plane electric field (responses to it, itype 7/8)
+    ebasis = _get(inp, "ebasis", "spherical")
+    dirf = _get(inp, "dirEfield", None)
+    run["ebasis"] = ebasis
+    if dirf is not None:
+        run["dirEfield"] = [fortran_float(x) for x in dirf.split()]
+        run["efield"] = field_direction(ebasis, dirf.split(), basis["a"])
     return {"system": system, "run": run, "input": inp, "sc_file": sc_file}
 
 

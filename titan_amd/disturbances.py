@@ -130,3 +130,16 @@ def torques(tchi, nA, nOrb, orbs, lpvec, lam_p, lam_d, um, mvec, raw, hhw=None):
             T[2, 1, i] = 2.0 * (h[2] * sx - h[0] * sz)
             T[2, 2, i] = 2.0 * (h[0] * sy - h[1] * sx)
     return T, T.sum(axis=2)
+
+
+def response_values(dist, dtot, beff, btot, T, Ttot):
+    """{file name: (per-site values, total)} of one energy for
+    titan_io.write_response_files (names as the reference's files)."""
+    names = ("Cd", "Sx", "Sy", "Sz", "Lx", "Ly", "Lz")
+    out = {n: (dist[k], dtot[k]) for k, n in enumerate(names)}
+    for k, c in enumerate("0xyz"):
+        out["Beff" + c] = (beff[k], btot[k])
+    for t, kind in enumerate(("SOT", "XCT")):
+        for k, c in enumerate("xyz"):
+            out[kind + c] = (T[t, k], Ttot[t, k])
+    return out

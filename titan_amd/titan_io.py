@@ -9,6 +9,13 @@
   nA x nA transverse block chi^{+-}(i, j) = schi((1, i), (1, j)) by LAPACK's
   zgeev through numpy (mod_susceptibilities.F90:160-189), and the chi_eval /
   chi_evec rows (:342-347).
+* `response_line` / `write_response_files`: the rows of the responses to the
+  in-plane electric field, results/<SOC>/<Sites>/{CD,SD,LD}/<name>_pos=i and
+  _total (mod_disturbances.F90:55-94, 157-264), Beff/Beff{0,x,y,z}
+  (mod_beff.F90:40-70, 114-167) and SOT/{SOT,XCT}{x,y,z} (mod_torques):
+  energy, amplitude, real and imaginary part, phase, real/amplitude,
+  imaginary/amplitude (the last three written as 0 at or below 1e-15) and
+  the magnetisation angles, "(9(es16.9,2x))".
 * `read_data` / `compare_files`: the reference's comparator
   (scripts/compare_output.py), with the relative test on |v2| (the script
   divides by v2, so a negative v2 never fails the relative test).
@@ -58,6 +65,63 @@ def write_chi_files(directory, kind, energies, qval, schis, nA, suffix=""):
                 fh.write(HEADER[kind] + "\n")
                 for e, s in zip(energies, schis):
                     fh.write(chi_line(e, qval, s, i, j, nA) + "\n")
+            paths.append(p)
+    return paths
+
+
+def _response_header(name, dist):
+    if dist:   # mod_disturbances.F90:72
+        return (" #     energy    , amplitude of %s , real part of %s , imag part of %s ,  phase of %s  ,  "
+                "cosine of %s  ,  sine of %s  " % ((name,) * 6))
+    # mod_beff.F90:55, mod_torques.F90:68
+    return (" #     energy    , amplitude of %s , real part of %s , imaginary part of %s , phase of %s , "
+            "cosine of %s  ,  sine of %s  " % ((name,) * 6))
+
+
+def response_line(e, z, theta, phi):
+    """One row of a response file: the complex value z at energy e and the
+    site's magnetisation angles (mod_disturbances.F90:205-220)."""
+    z = complex(z)
+    a = abs(z)
+    if a >= 1e-15:
+        phase, sine, cosine = np.arctan2(z.imag, z.real), z.real / a, z.imag / a
+        phase = 0.0 if phase <= 1e-15 else phase
+        sine = 0.0 if sine <= 1e-15 else sine
+        cosine = 0.0 if cosine <= 1e-15 else cosine
+    else:
+        phase = sine = cosine = 0.0
+    return fields([e, a, z.real, z.imag, phase, sine, cosine, theta, phi])
+
+
+RESPONSE_FILES = (("CD", "Cd", True), ("SD", "Sx", True), ("SD", "Sy", True), ("SD", "Sz", True),
+                  ("LD", "Lx", True), ("LD", "Ly", True), ("LD", "Lz", True),
+                  ("Beff", "Beff0", False), ("Beff", "Beffx", False), ("Beff", "Beffy", False),
+                  ("Beff", "Beffz", False),
+                  ("SOT", "SOTx", False), ("SOT", "SOTy", False), ("SOT", "SOTz", False),
+                  ("SOT", "XCTx", False), ("SOT", "XCTy", False), ("SOT", "XCTz", False))
+
+
+def write_response_files(directory, energies, values, angles, total_angles, suffix=""):
+    """Files <directory>/<folder>/<name>_pos=<i><suffix>.dat and
+    <name>_total<suffix>.dat for every entry of RESPONSE_FILES, one row per
+    energy.  values[w][name] = (per-site array [nA], total) at energy w;
+    angles[w] = (theta [nA], phi [nA]) and total_angles[w] = (theta, phi) in
+    degrees.  Returns the paths."""
+    paths = []
+    nA = len(values[0]["Cd"][0])
+    for folder, name, dist in RESPONSE_FILES:
+        os.makedirs(os.path.join(directory, folder), exist_ok=True)
+        targets = [("_pos=%d" % (i + 1), i) for i in range(nA)] + [("_total", None)]
+        for tag, i in targets:
+            p = os.path.join(directory, folder, "%s%s%s.dat" % (name, tag, suffix))
+            with open(p, "w") as fh:
+                fh.write(_response_header(name, dist) + "\n")
+                for w, e in enumerate(energies):
+                    site, tot = values[w][name]
+                    if i is None:
+                        fh.write(response_line(e, tot, *total_angles[w]) + "\n")
+                    else:
+                        fh.write(response_line(e, site[i], angles[w][0][i], angles[w][1][i]) + "\n")
             paths.append(p)
     return paths
 
