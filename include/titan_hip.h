@@ -343,6 +343,14 @@ int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, d
 int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out);
 int thx_last_split(th_handle *h);
 int thx_last_refined(th_handle *h);
+/* thx_last_spectral: 1 when the last chi0 took its Green functions from the
+ * eigenpairs of H(k) (spectral kernels), 0 when from elimination. */
+int thx_last_spectral(th_handle *h);
+/* thx_eigenpairs: the eigenpairs of H(k + q) at the handle's unique k-points
+ * (the imaginary list's, then the real mesh's new ones) as the spectral
+ * kernels use them: L_out n_kuniq x d, V_out n_kuniq x d x d complex, V[r][m]
+ * = component r of eigenvector m.  Computes them where they are not valid. */
+int thx_eigenpairs(th_handle *h, const double *q, double *L_out, double *V_out);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,7 @@ def _hooks():
                                   C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
     L.thx_contract.argtypes = [C.c_void_p, _dp, _dp, C.c_long, C.c_int, _dp]
     L.thx_last_split.argtypes = [C.c_void_p]
+    L.thx_last_spectral.argtypes = [C.c_void_p]
     return L
 
 
@@ -627,6 +628,53 @@ def test_spectral_green_matches_elimination_and_oracle(which, q):
         h.tune(spectral=1)
         gm = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED)
         assert rel(gm, o) < 1e-6, rel(gm, o)
+    h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which,q", [("bcc4", (0.0, 0.0, 0.0)), ("bcc4", (0.0, 0.03, 0.0)),
+                                     ("slab8", (0.0, 0.0, 0.0)), ("slab8", (0.02, 0.0, 0.0)),
+                                     ("bcc16", (0.0, 0.0, 0.0))])
+def test_spectral_large_blocks_match_elimination_and_oracle(which, q):
+    """d = 72, 144 and 288: eigenpairs of H(k) (and of H(k + q)) from the dense
+    batched solver, G from the column-tiled spectral kernel on the FP64 and
+    FP32 matrix cores: the FP64 chi0 against the elimination kernels and the
+    oracle to 1e-10, the mixed chi0 (FP32 panels) within 1e-6, and the raw G
+    of the current response against the oracle."""
+    L = _hooks()
+    spec = {"bcc4": lambda: CF.bcc_supercell(4), "slab8": lambda: CF.fcc001_slab(8),
+            "bcc16": lambda: CF.bcc_supercell(16)}[which]()
+    m, s = TBModel(spec), O.OracleSystem(spec)
+    (K, Y, W), (kp, wk) = _work(m, (2, 2, 1) if which == "slab8" else (2, 1, 1), n1gl=6)
+    h = m.handle()
+    h.set_imag_points(K, Y, W)
+    h.set_real_kmesh(kp, wk)
+    om, eta = 3e-3, 5e-4
+    x2, p2 = Ct.real_epoints(m.Ef, om, 1, 6)
+    print(which, "d", m.d, "q", q, flush=True)
+    h.tune(spectral=1)
+    gs = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2)
+    assert L.thx_last_spectral(h._h) == 1
+    h.tune(spectral=0)
+    ge = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2)
+    assert L.thx_last_spectral(h._h) == 0
+    o = s.chi0(list(q), om, eta, eta, (K, Y, W), (kp, wk), (x2, p2))
+    print("  spectral vs elimination %.2e, vs oracle %.2e" % (rel(gs, ge), rel(gs, o)), flush=True)
+    assert rel(gs, o) < 1e-10 and rel(gs, ge) < 1e-10
+    h.tune(spectral=1)
+    gm = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED)
+    assert L.thx_last_spectral(h._h) == 1
+    print("  mixed vs oracle %.2e" % rel(gm, o), flush=True)
+    assert rel(gm, o) < 1e-6
+    if which != "bcc16" and not any(q):
+        # the eigenpairs are in place: the current response's raw G comes from
+        # the tiled spectral kernel
+        from oracle import current as OC
+        ue = np.array([0.6, 0.8, 0.0])
+        cur = h.current_response(list(q), om, m.Ef, eta, eta, ue, x2, p2)
+        want = OC.current_response(s, q, om, eta, eta, ue, (K, Y, W), (kp, wk), (x2, p2))
+        print("  current response vs oracle %.2e" % rel(cur, want), flush=True)
+        assert rel(cur, want) < 1e-10
     h.close()
 
 

@@ -36,7 +36,7 @@ from titan_amd import binding as B
 # elimination kernel shapes built per d, and the d with a spectral kernel
 # (th_green.hip)
 VARIANTS = {36: (0, 2), 54: (0, 2, 3), 72: (0, 3)}
-SPECTRAL_D = (18, 36, 54)
+SPECTRAL_D = (18, 36, 54, 72, 144, 288)
 
 
 def _knobs(h, base, setting):

@@ -51,7 +51,7 @@ def build_hip(verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lrocsolver", "-lrocblas"])
     os.replace(tmp, LIB)
     return LIB
 

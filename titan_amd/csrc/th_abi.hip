@@ -221,12 +221,13 @@ static int rebuild_kunion(th_handle *h)
 }
 
 // Eigenpairs for the sides of th_chi0 at this q: set 0 (q = 0) always, set
-// 1 for k + q when q != 0.  Skipped (the elimination kernels are used) when
-// H is not Hermitian, d has no spectral kernel, or the sets would take more
-// than 30 % of free HBM.
+// 1 for k + q when q != 0 (Jacobi in LDS for d <= 54, LAPACK's divide and
+// conquer from rocSOLVER for d = 72, 144 and 288).  Skipped (the elimination
+// kernels are used) when H is not Hermitian, d has no spectral kernel, or
+// the sets would take more than 30 % of free HBM.
 int th_ensure_eig(th_handle *h, const double *q)
 {
-    if (!h->spectral || !h->hermitian || (h->d != 18 && h->d != 36 && h->d != 54) || h->n_kuniq <= 0) return TH_OK;
+    if (!h->spectral || !h->hermitian || !th_spectral_d(h->d) || h->n_kuniq <= 0) return TH_OK;
     const bool qz = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
     const size_t bytes = (size_t)h->n_kuniq * h->d * (h->d * sizeof(double2) + sizeof(double));
     // the handle's cap on its caches (both eigenpair sets count against it)
@@ -441,6 +442,7 @@ int chi0_launch(th_handle *h, const double *q, double omega, const double *omega
     // Green functions (their fp16 variant is issue-bound on the vector ALUs,
     // the contraction on the matrix cores and LDS)
     const bool spectral_on = h->spectral && h->eig_valid[0];
+    h->last_spectral = spectral_on ? 1 : 0;
     const bool ov = h->overlap == 2 || (h->overlap == 1 && !spectral_on);
     hipStream_t gs = ov ? h->stream2 : h->stream;
     void *pA[2] = {h->d_pA, ov ? h->d_pA2 : h->d_pA}, *pB[2] = {h->d_pB, ov ? h->d_pB2 : h->d_pB};
@@ -1098,6 +1100,7 @@ int th_destroy(th_handle *h)
 {
     if (!h) return TH_OK;
     hipStreamSynchronize(h->stream);
+    th_eig_release(h);
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
                     h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk, h->d_kuniq, h->d_kid_im, h->d_kid_re,
@@ -1162,6 +1165,22 @@ int thx_green_panel(th_handle *h, int mode, long p0, long np, const double *q, d
 
 int thx_last_split(th_handle *h) { return h ? h->last_presplit : 0; }
 int thx_last_refined(th_handle *h) { return h ? h->last_refined : 0; }
+int thx_last_spectral(th_handle *h) { return h ? h->last_spectral : 0; }
+
+int thx_eigenpairs(th_handle *h, const double *q, double *L_out, double *V_out)
+{
+    if (!h || bad(q) || bad(L_out) || bad(V_out)) { th_set_error("thx_eigenpairs: bad arguments"); return TH_E_ARG; }
+    int rc = th_ensure_eig(h, q);
+    if (rc) return rc;
+    const bool qz = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
+    const int which = qz ? 0 : 1;
+    if (!h->eig_valid[which]) { th_set_error("thx_eigenpairs: no eigenpairs (spectral path off)"); return TH_E_STATE; }
+    const size_t nk = (size_t)h->n_kuniq, d = (size_t)h->d;
+    TH_HIP(hipMemcpyAsync(L_out, h->eig[which].L, sizeof(double) * nk * d, hipMemcpyDeviceToHost, h->stream));
+    TH_HIP(hipMemcpyAsync(V_out, h->eig[which].V, sizeof(double2) * nk * d * d, hipMemcpyDeviceToHost, h->stream));
+    TH_HIP(hipStreamSynchronize(h->stream));
+    return TH_OK;
+}
 
 int thx_contract(th_handle *h, const double *A, const double *B, long np, int prec, double *acc_out)
 {

@@ -13,6 +13,9 @@
 // faster); the mixed mode rounds the finished G once to FP32 when it writes
 // the panel.  For the adjoint the kernel can also write the unpermuted G
 // (FP64, row-major d x d per pair) to a second buffer.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "th_internal.h"
 
 namespace {
@@ -1228,6 +1231,170 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_chunk_ke
     }
 }
 
+// Spectral Green functions for d above 64, where V^H of a k-point no longer
+// fits in LDS: the columns of G = V diag(d) V^H in tiles of TC, one workgroup
+// per (chunk of CH pairs, column tile) and one wave per 16 rows.  V^H of the
+// tile sits in LDS (reloaded when the k-point changes), the diagonal d of
+// the current pair too; the A operand V[r][m] d_m is read from device memory
+// per K-step (a k-point's V stays in L2 over its run of pairs).  The product
+// runs on the 16x16x4 matrix cores in the panels' precision; panels as the
+// run kernel writes them, or, where the caller asks for raw G, G itself (the
+// launcher then forms the panels from it).
+template <typename P, int DD, int TC, int RPW, int CH>
+__global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green_spectral_tiled_kernel(green_params g)
+{
+    typedef P R;
+    // NB row blocks of 16, RPW of them per wave (wave w owns blocks w, w + NW, ...)
+    constexpr int NB = (DD + 15) / 16, NW = (NB + RPW - 1) / RPW, NT = 64 * NW;
+    constexpr int KS = (DD + 3) / 4, KP = 4 * KS, TB = TC / 16;
+    static_assert(TC % 16 == 0, "column tile of whole 16-wide blocks");
+    __shared__ cplx<R> Vh[KP * TC];      // Vh[m][c - c0] = conj(V[c][m]); zero outside the matrix
+    __shared__ cplx<double> dg[KP];      // diagonal of the current pair
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int c0 = (int)blockIdx.y * TC;
+    const long s0 = (long)blockIdx.x * CH;
+    const long s1 = s0 + CH < g.np ? s0 + CH : g.np;
+    const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
+    const bool bside = g.mode == TH_G_IMAG_B || g.mode == TH_G_REAL_B;
+    const int bs = g.bs, nA = g.nA, Mpad = g.Mpad, Mb = bs * bs;
+    int rI[RPW][4], rR[RPW][4], cI[TB], cR[TB], arow[RPW];
+    bool rv[RPW][4], cv[TB];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+        const int rb = wave + NW * i;
+        arow[i] = 16 * rb + (lane & 15);         // row of V in this lane's A operand
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * rb + smfma<R>::row(lane >> 4, q);
+            rv[i][q] = rb < NB && r < DD;
+            rI[i][q] = r / bs;
+            rR[i][q] = (r - rI[i][q] * bs) * bs;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < TB; ++j) {
+        const int c = c0 + 16 * j + (lane & 15);
+        cv[j] = c < DD;
+        cI[j] = c / bs;
+        cR[j] = c - cI[j] * bs;
+    }
+    long cur = -1;
+    const double2 *Vk = nullptr;
+    for (long pl = s0; pl < s1; ++pl) {
+        const long p = g.p0 + pl;
+        double k0, k1, k2, er, ei, w;
+        pair_point(g, p, k0, k1, k2, er, ei, w);
+        long kb;
+        if (imag) {
+            kb = g.kid_im[p];
+        } else {
+            long ie, ik;
+            real_split(g, p, ie, ik);
+            kb = g.kid_re[ik];
+        }
+        __syncthreads();                        // the previous pair's reads of dg (and Vh) are done
+        if (kb != cur) {                        // same kb in every thread
+            Vk = g.eigV + (size_t)kb * DD * DD;
+            for (int e = tid; e < KP * TC; e += NT) {
+                const int m = e / TC, cc = e - (e / TC) * TC, c = c0 + cc;
+                cplx<R> x = {R(0), R(0)};
+                if (m < DD && c < DD) {
+                    const double2 v = Vk[(size_t)c * DD + m];
+                    x = {(R)v.x, (R)(-v.y)};
+                }
+                Vh[e] = x;
+            }
+            cur = kb;
+        }
+        // diagonal d = (ca Re f + cb Im f) + i cc Im f, f = 1 / (z - lambda_m);
+        // raw output takes G itself (d = f), the panels are then made from it
+        double ca = 1.0, cb = 0.0, cc = 1.0;                  // imag B: f
+        if (!g.raw) {
+            if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }    // w f
+            if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }   // 2 w Im f
+            if (g.mode == TH_G_REAL_B) cc = -1.0;             // conj(f)
+        }
+        for (int m = tid; m < KP; m += NT) {
+            cplx<double> v = {0.0, 0.0};
+            if (m < DD) {
+                const double a = er - g.eigL[(size_t)kb * DD + m], n = a * a + ei * ei;
+                const double fr = a / n, fi = -ei / n;
+                v = {ca * fr + cb * fi, cc * fi};
+            }
+            dg[m] = v;
+        }
+        __syncthreads();
+        typename smfma<R>::acc accr[RPW][TB], acci[RPW][TB];
+#pragma unroll
+        for (int i = 0; i < RPW; ++i)
+#pragma unroll
+            for (int j = 0; j < TB; ++j)
+                for (int q = 0; q < 4; ++q) { accr[i][j][q] = 0; acci[i][j][q] = 0; }
+#pragma unroll(RPW == 1 ? 2 : 1)
+        for (int ks = 0; ks < KS; ++ks) {
+            const int m = 4 * ks + (lane >> 4);
+            const cplx<double> dm = dg[m];
+            R ar[RPW], ai[RPW], nai[RPW];
+#pragma unroll
+            for (int i = 0; i < RPW; ++i) {
+                double2 v = make_double2(0.0, 0.0);
+                if (arow[i] < DD && m < DD) v = Vk[(size_t)arow[i] * DD + m];
+                const cplx<double> x = cmul(cplx<double>{v.x, v.y}, dm);
+                ar[i] = (R)x.x;
+                ai[i] = (R)x.y;
+                nai[i] = -ai[i];
+            }
+#pragma unroll
+            for (int j = 0; j < TB; ++j) {
+                const cplx<R> b = Vh[m * TC + 16 * j + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < RPW; ++i) {
+                    accr[i][j] = smfma<R>::mac(ar[i], b.x, accr[i][j]);
+                    accr[i][j] = smfma<R>::mac(nai[i], b.y, accr[i][j]);
+                    acci[i][j] = smfma<R>::mac(ar[i], b.y, acci[i][j]);
+                    acci[i][j] = smfma<R>::mac(ai[i], b.x, acci[i][j]);
+                }
+            }
+        }
+        if (g.panel && !g.raw) {
+            P *out = reinterpret_cast<P *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
+#pragma unroll
+            for (int i = 0; i < RPW; ++i)
+#pragma unroll
+                for (int j = 0; j < TB; ++j)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (rv[i][q] && cv[j]) {
+                            const int blk = bside ? cI[j] * nA + rI[i][q] : rI[i][q] * nA + cI[j];
+                            const size_t o = (size_t)blk * 2 * Mpad + rR[i][q] + cR[j];
+                            out[o] = (P)accr[i][j][q];
+                            out[o + Mpad] = (P)acci[i][j][q];
+                        }
+            if (blockIdx.y == 0) {
+                const int npad = Mpad - Mb;
+                for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+                    const int part = e / npad;
+                    out[(size_t)part * Mpad + Mb + (e - part * npad)] = P(0);
+                }
+            }
+        }
+        if (g.raw) {
+            double2 *o = g.raw + (size_t)pl * DD * DD;
+#pragma unroll
+            for (int i = 0; i < RPW; ++i)
+#pragma unroll
+                for (int j = 0; j < TB; ++j)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (rv[i][q] && cv[j]) {
+                            const int r = 16 * (wave + NW * i) + smfma<R>::row(lane >> 4, q);
+                            const int c = c0 + 16 * j + (lane & 15);
+                            o[(size_t)r * DD + c] = make_double2((double)accr[i][j][q], (double)acci[i][j][q]);
+                        }
+        }
+    }
+}
+
 // Mixed precision variant of the run kernel on the fp16 matrix cores
 // (v_mfma_f32_16x16x32_f16, K = 32 eigen-indices per instruction).  The
 // operands are split in two fp16 parts (hi + lo, 22 significant bits):
@@ -1652,6 +1819,27 @@ int launch_green_spectral(th_handle *h, const th_green_args &a, const green_para
     return TH_OK;
 }
 
+template <typename P, int DD, int TC, int RPW>
+int launch_green_spectral_tiled(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    if (a.presplit) {
+        th_set_error("split half-precision panels come from the half-precision spectral kernel only");
+        return TH_E_STATE;
+    }
+    constexpr int CH = TH_RUN, NT = 64 * (((DD + 15) / 16 + RPW - 1) / RPW);
+    const dim3 grid((unsigned)((a.np + CH - 1) / CH), (unsigned)((DD + TC - 1) / TC));
+    hipLaunchKernelGGL((green_spectral_tiled_kernel<P, DD, TC, RPW, CH>), grid, dim3(NT), 0, gstream(h, a), g);
+    TH_HIP(hipGetLastError());
+    if (g.raw && g.panel) {   // the panels of the pair's mode from the raw G just written
+        green_params gp = g;
+        gp.raw = nullptr;
+        hipLaunchKernelGGL((panel_from_global_kernel<P, 256>), dim3((unsigned)a.np), dim3(256), 0, gstream(h, a), gp,
+                           (const double2 *)g.raw);
+        TH_HIP(hipGetLastError());
+    }
+    return TH_OK;
+}
+
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
@@ -1685,6 +1873,9 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
         case 18: return launch_green_spectral<P, 18>(h, a, g);
         case 36: return launch_green_spectral<P, 36>(h, a, g);
         case 54: return launch_green_spectral<P, 54>(h, a, g);
+        case 72: return launch_green_spectral_tiled<P, 72, 80, 1>(h, a, g);
+        case 144: return launch_green_spectral_tiled<P, 144, 48, 1>(h, a, g);
+        case 288: return launch_green_spectral_tiled<P, 288, 32, 2>(h, a, g);
         default: break;
         }
     }
@@ -1796,6 +1987,206 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     return launch_green_p<float>(h, a, g);
 }
 
+namespace {
+
+// H(k + q) of the unique k-points k0 .. k0 + gridDim.x - 1, column-major
+// (LAPACK's layout), assembled as the Jacobi kernel assembles it
+__global__ __launch_bounds__(256) void assemble_hk_kernel(green_params g, const double *klist, long k0,
+                                                          const int *kidx, double q0, double q1, double q2,
+                                                          double2 *Hcm)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double> *ph = reinterpret_cast<cplx<double> *>(smem);
+    const int tid = threadIdx.x, d = g.d, nO = g.nOrb, nO2 = nO * nO;
+    const long kb = kidx ? (long)kidx[blockIdx.x] : k0 + blockIdx.x;
+    const double ka = klist[3 * kb] + q0, kbb = klist[3 * kb + 1] + q1, kc = klist[3 * kb + 2] + q2;
+    for (int n = tid; n < g.nhop; n += blockDim.x) {
+        const double a = ka * g.R[3 * n] + kbb * g.R[3 * n + 1] + kc * g.R[3 * n + 2];
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        ph[n] = {cs, sn};
+    }
+    __syncthreads();
+    double2 *o = Hcm + (size_t)blockIdx.x * d * d;
+    for (int e = tid; e < d * d; e += blockDim.x) {
+        const int c = e / d, r = e - (e / d) * d;   // column-major: e = r + c d
+        const double2 hv = g.h0[r + (size_t)c * d];
+        cplx<double> x = {hv.x, hv.y};
+        const int ja = r / (2 * nO), rsp = (r / nO) & 1, nu = r - (r / nO) * nO;
+        const int ia = c / (2 * nO), csp = (c / nO) & 1, mu = c - (c / nO) * nO;
+        if (rsp == csp) {
+            const int h1 = g.hoff[ja * g.nA + ia], h2 = g.hoff[ja * g.nA + ia + 1];
+            for (int n = h1; n < h2; ++n) {
+                const double2 tv = g.t[(size_t)n * nO2 + nu + nO * mu];
+                const cplx<double> tp = cmul(cplx<double>{tv.x, tv.y}, ph[n]);
+                x.x += tp.x;
+                x.y += tp.y;
+            }
+        }
+        o[e] = make_double2(x.x, x.y);
+    }
+}
+
+// V[k][r d + m] = A[k][r + m d]: eigenvectors from LAPACK's columns into the
+// row-major layout of the spectral kernels (matrix k of A to slot kidx[k] of
+// V where an index list is given, else to slot k)
+__global__ __launch_bounds__(256) void transpose_v_kernel(const double2 *A, long n, int d, const int *kidx,
+                                                          double2 *V)
+{
+    const size_t dd = (size_t)d * d;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (size_t)n * dd) return;
+    const size_t k = e / dd;
+    const int rem = (int)(e - k * dd), r = rem / d, m = rem - (rem / d) * d;
+    const size_t slot = kidx ? (size_t)kidx[k] : k;
+    V[slot * dd + (size_t)r * d + m] = A[k * dd + r + (size_t)m * d];
+}
+
+// flag[k] = 1 where eigenvalues or eigenvectors of matrix k are not finite
+__global__ __launch_bounds__(256) void eig_finite_kernel(const double2 *A, const double *L, int d, int *flag)
+{
+    const size_t dd = (size_t)d * d;
+    const double2 *a = A + (size_t)blockIdx.x * dd;
+    const double *l = L + (size_t)blockIdx.x * d;
+    int bad = 0;
+    for (size_t e = threadIdx.x; e < dd; e += 256) bad |= !(isfinite(a[e].x) && isfinite(a[e].y));
+    for (int e = threadIdx.x; e < d; e += 256) bad |= !isfinite(l[e]);
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) flag[blockIdx.x] = bad;
+}
+
+// Eigenpairs where d is too large for the LDS-resident Jacobi kernel: H(k + q)
+// assembled on the device, rocSOLVER's batched divide-and-conquer zheevd on
+// the handle's stream in chunks of at most 2 GiB of matrices, eigenvectors
+// transposed into the set's layout.  zheevd has returned non-finite
+// eigenvectors with info = 0 for strongly degenerate spectra (H at Gamma of
+// a 16-site supercell, d = 288); such matrices are found by a finiteness
+// check and decomposed again by rocSOLVER's Jacobi zheevj.  Once per sweep
+// and set.
+int launch_eig_dense(th_handle *h, int which, const double *q)
+{
+    const int d = h->d;
+    const size_t dd = (size_t)d * d;
+    th_eig_set &es = h->eig[which];
+    if (!h->rb) {
+        rocblas_handle rb = nullptr;
+        if (rocblas_create_handle(&rb) != rocblas_status_success) {
+            th_set_error("eigen-decomposition: rocblas_create_handle failed");
+            return TH_E_HIP;
+        }
+        h->rb = rb;
+    }
+    rocblas_handle rb = static_cast<rocblas_handle>(h->rb);
+    if (rocblas_set_stream(rb, h->stream) != rocblas_status_success) {
+        th_set_error("eigen-decomposition: rocblas_set_stream failed");
+        return TH_E_HIP;
+    }
+    const long chunk = std::max<long>(1, std::min<long>(h->n_kuniq, (long)(((size_t)2 << 30) / (dd * sizeof(double2)))));
+    double2 *A = nullptr;
+    double *E = nullptr, *W = nullptr, *res = nullptr;
+    rocblas_int *info = nullptr, *nsw = nullptr;
+    int *flag = nullptr, *kidx = nullptr;
+    auto cleanup = [&]() {
+        hipFree(A); hipFree(E); hipFree(W); hipFree(res); hipFree(info); hipFree(nsw); hipFree(flag); hipFree(kidx);
+    };
+    if (hipMalloc(&A, chunk * dd * sizeof(double2)) != hipSuccess || hipMalloc(&E, chunk * d * sizeof(double)) != hipSuccess ||
+        hipMalloc(&W, chunk * d * sizeof(double)) != hipSuccess || hipMalloc(&res, chunk * sizeof(double)) != hipSuccess ||
+        hipMalloc(&info, chunk * sizeof(rocblas_int)) != hipSuccess ||
+        hipMalloc(&nsw, chunk * sizeof(rocblas_int)) != hipSuccess || hipMalloc(&flag, chunk * sizeof(int)) != hipSuccess ||
+        hipMalloc(&kidx, chunk * sizeof(int)) != hipSuccess) {
+        cleanup();
+        (void)hipGetLastError();
+        th_set_error("eigen-decomposition: out of device memory");
+        return TH_E_ALLOC;
+    }
+    green_params g{};
+    g.nA = h->nA; g.nOrb = h->nOrb; g.bs = h->bs; g.d = d; g.nhop = h->nhop;
+    g.R = h->d_R; g.jb = h->d_jb; g.ib = h->d_ib; g.t = h->d_t; g.h0 = h->d_h0; g.hoff = h->d_hoff;
+    std::vector<rocblas_int> hinfo(chunk);
+    std::vector<int> hflag(chunk);
+    const size_t lds = sizeof(double2) * std::max(1, h->nhop);
+    int rc = TH_OK;
+    auto fail = [&](const char *what) { th_set_error(std::string("eigen-decomposition: ") + what); return TH_E_HIP; };
+    for (long k0 = 0; k0 < h->n_kuniq && !rc; k0 += chunk) {
+        const long n = std::min(chunk, h->n_kuniq - k0);
+        double *Lk = es.L + (size_t)k0 * d;
+        hipLaunchKernelGGL(assemble_hk_kernel, dim3((unsigned)n), dim3(256), lds, h->stream, g,
+                           (const double *)h->d_kuniq, k0, (const int *)nullptr, q[0], q[1], q[2], A);
+        if (hipGetLastError() != hipSuccess) { rc = fail("H assembly"); break; }
+        if (rocsolver_zheevd_strided_batched(rb, rocblas_evect_original, rocblas_fill_upper, d,
+                                             reinterpret_cast<rocblas_double_complex *>(A), d, (rocblas_stride)dd, Lk,
+                                             d, E, d, info, (rocblas_int)n) != rocblas_status_success) {
+            rc = fail("rocsolver_zheevd_strided_batched failed");
+            break;
+        }
+        hipLaunchKernelGGL(eig_finite_kernel, dim3((unsigned)n), dim3(256), 0, h->stream, (const double2 *)A,
+                           (const double *)Lk, d, flag);
+        hipLaunchKernelGGL(transpose_v_kernel, dim3((unsigned)((n * dd + 255) / 256)), dim3(256), 0, h->stream,
+                           (const double2 *)A, n, d, (const int *)nullptr, es.V + (size_t)k0 * dd);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hinfo.data(), info, n * sizeof(rocblas_int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipMemcpyAsync(hflag.data(), flag, n * sizeof(int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = fail("transpose or status copy");
+            break;
+        }
+        // the matrices zheevd did not decompose, again by Jacobi rotations
+        std::vector<int> redo;
+        for (long i = 0; i < n; ++i)
+            if (hinfo[i] != 0 || hflag[i]) redo.push_back((int)(k0 + i));
+        if (redo.empty()) continue;
+        const long nr = (long)redo.size();
+        if (hipMemcpyAsync(kidx, redo.data(), nr * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            rc = fail("index copy");
+            break;
+        }
+        hipLaunchKernelGGL(assemble_hk_kernel, dim3((unsigned)nr), dim3(256), lds, h->stream, g,
+                           (const double *)h->d_kuniq, 0L, (const int *)kidx, q[0], q[1], q[2], A);
+        if (rocsolver_zheevj_strided_batched(rb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, d,
+                                             reinterpret_cast<rocblas_double_complex *>(A), d, (rocblas_stride)dd, 0.0,
+                                             res, 100, nsw, W, d, info, (rocblas_int)nr) != rocblas_status_success) {
+            rc = fail("rocsolver_zheevj_strided_batched failed");
+            break;
+        }
+        hipLaunchKernelGGL(eig_finite_kernel, dim3((unsigned)nr), dim3(256), 0, h->stream, (const double2 *)A,
+                           (const double *)W, d, flag);
+        hipLaunchKernelGGL(transpose_v_kernel, dim3((unsigned)((nr * dd + 255) / 256)), dim3(256), 0, h->stream,
+                           (const double2 *)A, nr, d, (const int *)kidx, es.V);
+        for (long i = 0; i < nr && !rc; ++i)
+            if (hipMemcpyAsync(es.L + (size_t)redo[i] * d, W + (size_t)i * d, d * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+                rc = fail("eigenvalue copy");
+        if (rc) break;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hinfo.data(), info, nr * sizeof(rocblas_int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipMemcpyAsync(hflag.data(), flag, nr * sizeof(int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = fail("Jacobi status copy");
+            break;
+        }
+        for (long i = 0; i < nr; ++i)
+            if (hinfo[i] != 0 || hflag[i]) {
+                rc = TH_E_HIP;
+                th_set_error("eigen-decomposition of H(k) failed at unique k-point " + std::to_string(redo[i]));
+                break;
+            }
+    }
+    cleanup();
+    return rc;
+}
+
+}  // namespace
+
+bool th_spectral_d(int d) { return d == 18 || d == 36 || d == 54 || d == 72 || d == 144 || d == 288; }
+
+void th_eig_release(th_handle *h)
+{
+    if (h && h->rb) {
+        rocblas_destroy_handle(static_cast<rocblas_handle>(h->rb));
+        h->rb = nullptr;
+    }
+}
+
 // Eigenpairs of H(k + q) for the handle's unique k-points into set 'which'
 int th_launch_eig(th_handle *h, int which, const double *q)
 {
@@ -1817,8 +2208,12 @@ int th_launch_eig(th_handle *h, int which, const double *q)
         hipLaunchKernelGGL((eig_jacobi_kernel<54, 256>), dim3((unsigned)h->n_kuniq), dim3(256), lds, h->stream, g,
                            (const double *)h->d_kuniq, q[0], q[1], q[2], es.V, es.L);
         break;
+    case 72:
+    case 144:
+    case 288:
+        return launch_eig_dense(h, which, q);
     default:
-        th_set_error("spectral Green functions are built for d = 18, 36, 54");
+        th_set_error("spectral Green functions are built for d = 18, 36, 54, 72, 144, 288");
         return TH_E_UNSUPPORTED;
     }
     TH_HIP(hipGetLastError());

@@ -87,6 +87,7 @@ struct th_handle {
     int spectral = 1;            // use them when possible
     int spectral_chunk = 1;      // spectral G for runs of pairs per k-point (V loaded once per run)
     int contract_f16 = 1;        // mixed: contraction on fp16 MFMA with two-term splits of the FP32 panels
+    void *rb = nullptr;          // rocblas handle of the dense eigen-decompositions (d > 64), made on first use
     int current_general = 0;     // th_current_response: three-stage form for every d (1) or only where LDS cannot hold a pair (0)
     int contract_tiles = 1;      // split-panel contraction: 8 waves with 2-D register tiles (1) or 7 strips (0)
     int presplit = 1;            // mixed spectral: Green function kernels write the fp16 splits themselves
@@ -123,6 +124,7 @@ struct th_handle {
     // FP64 chi0[S,S] of the mixed path (th_refine.hip)
     int refine = 1;              // recompute chi0 on the support of U in FP64 (mixed precision)
     int last_refined = 0;        // the last chi0 did
+    int last_spectral = 0;       // the last chi0 took its Green functions from eigenpairs
     std::vector<int> ref_S;      // support of the declared U (chi indices, ascending)
     std::vector<int> ref_built_S;   // support the device tables below were built for
     long ref_built_nk = -1;
@@ -231,6 +233,10 @@ bool th_green_presplit_ok(const th_handle *h, const double *q);
 // part-empty last wave
 long th_green_wave_pairs(th_handle *h);
 int th_launch_eig(th_handle *h, int which, const double *q);
+// d with spectral Green function kernels (and an eigen-decomposition for them)
+bool th_spectral_d(int d);
+// releases the library handle of the dense eigen-decompositions
+void th_eig_release(th_handle *h);
 
 // Row of global pair b in panels stored per batch of bpc pairs with each
 // batch's rows in the contraction's summation order (the runs of TH_RUN pairs
