@@ -159,12 +159,18 @@ def test_green_panels_match_numpy(which, order, path, fe, cfg2):
 
 
 @pytest.mark.parametrize("f16", [0, 1])
-@pytest.mark.parametrize("which", ["fe", "cfg2"])
+@pytest.mark.parametrize("which", ["fe", "cfg2", "bcc4", "slab8"])
 def test_mixed_spectral_panels_match_fp64(which, f16, fe, cfg2):
-    """Mixed-precision panels of the run kernel (FP32 MFMA, or fp16 MFMA on
-    two-term splits with the per-pair resolvent-bound scale) against the FP64
-    panels of the same pairs, every mode, both axes, eta down to 5e-4."""
-    m, _ = {"fe": fe, "cfg2": cfg2}[which]
+    """Mixed-precision panels of the spectral kernels (FP32 MFMA, or fp16 MFMA
+    on two-term splits with the per-pair resolvent-bound scale: the run kernel
+    for d = 18 and 54, the column-tiled kernel for d = 72 and 144) against the
+    FP64 panels of the same pairs, every mode, both axes, eta down to 5e-4."""
+    if which == "bcc4":
+        m = TBModel(CF.bcc_supercell(4))
+    elif which == "slab8":
+        m = TBModel(CF.fcc001_slab(8))
+    else:
+        m, _ = {"fe": fe, "cfg2": cfg2}[which]
     L = _hooks()
     h = m.handle()
     h.tune(spectral=1, spectral_chunk=1, green_f16=f16)
@@ -637,10 +643,12 @@ def test_spectral_green_matches_elimination_and_oracle(which, q):
                                      ("bcc16", (0.0, 0.0, 0.0))])
 def test_spectral_large_blocks_match_elimination_and_oracle(which, q):
     """d = 72, 144 and 288: eigenpairs of H(k) (and of H(k + q)) from the dense
-    batched solver, G from the column-tiled spectral kernel on the FP64 and
-    FP32 matrix cores: the FP64 chi0 against the elimination kernels and the
-    oracle to 1e-10, the mixed chi0 (FP32 panels) within 1e-6, and the raw G
-    of the current response against the oracle."""
+    batched solver, G from the column-tiled spectral kernels: the FP64 chi0
+    against the elimination kernels and the oracle to 1e-10; the mixed chi0
+    within 1e-6 of the oracle, for d = 72 and 144 from the fp16 tiled kernel
+    writing split panels, writing FP32 panels and from the FP32 matrix-core
+    kernel, for d = 288 from the latter; and the raw G of the current response
+    against the oracle."""
     L = _hooks()
     spec = {"bcc4": lambda: CF.bcc_supercell(4), "slab8": lambda: CF.fcc001_slab(8),
             "bcc16": lambda: CF.bcc_supercell(16)}[which]()
@@ -664,8 +672,17 @@ def test_spectral_large_blocks_match_elimination_and_oracle(which, q):
     h.tune(spectral=1)
     gm = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED)
     assert L.thx_last_spectral(h._h) == 1
-    print("  mixed vs oracle %.2e" % rel(gm, o), flush=True)
+    print("  mixed vs oracle %.2e (split panels %d)" % (rel(gm, o), L.thx_last_split(h._h)), flush=True)
     assert rel(gm, o) < 1e-6
+    if m.d in (72, 144):
+        assert L.thx_last_split(h._h) == 1
+        for knob in ({"presplit": 0}, {"presplit": 1, "green_f16": 0}):
+            h.tune(**knob)
+            gv = h.chi0(list(q), om, m.Ef, eta, eta, x2, p2, precision=B.PREC_MIXED)
+            print("  mixed", knob, "vs oracle %.2e, vs split panels %.2e" % (rel(gv, o), rel(gv, gm)), flush=True)
+            assert L.thx_last_split(h._h) == 0
+            assert rel(gv, o) < 1e-6
+        h.tune(green_f16=1)
     if which != "bcc16" and not any(q):
         # the eigenpairs are in place: the current response's raw G comes from
         # the tiled spectral kernel

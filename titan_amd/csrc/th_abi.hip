@@ -229,7 +229,8 @@ int th_ensure_eig(th_handle *h, const double *q)
 {
     if (!h->spectral || !h->hermitian || !th_spectral_d(h->d) || h->n_kuniq <= 0) return TH_OK;
     const bool qz = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
-    const size_t bytes = (size_t)h->n_kuniq * h->d * (h->d * sizeof(double2) + sizeof(double));
+    const size_t sh = th_eig_split_halves(h->d);
+    const size_t bytes = (size_t)h->n_kuniq * (h->d * (h->d * sizeof(double2) + sizeof(double)) + 2 * sh);
     // the handle's cap on its caches (both eigenpair sets count against it)
     if (h->cache_mb > 0 && (qz ? 1 : 2) * bytes > ((size_t)h->cache_mb << 20)) return TH_OK;
     for (int which = 0; which < (qz ? 1 : 2); ++which) {
@@ -237,16 +238,24 @@ int th_ensure_eig(th_handle *h, const double *q)
         const double qq[3] = {which ? q[0] : 0.0, which ? q[1] : 0.0, which ? q[2] : 0.0};
         if (h->eig_valid[which] && es.q[0] == qq[0] && es.q[1] == qq[1] && es.q[2] == qq[2]) continue;
         h->eig_valid[which] = 0;
+        if (es.V && es.cap < h->n_kuniq) {   // the k-point list grew: buffers of the new size
+            TH_HIP(hipStreamSynchronize(h->stream));
+            hipFree(es.V); hipFree(es.L); hipFree(es.S);
+            es.V = nullptr; es.L = nullptr; es.S = nullptr;
+            es.cap = 0;
+        }
         if (!es.V) {
             size_t freeb = 0, totb = 0;
             if (hipMemGetInfo(&freeb, &totb) != hipSuccess || bytes > (size_t)(0.3 * (double)freeb)) return TH_OK;
             if (hipMalloc(&es.V, (size_t)h->n_kuniq * h->d * h->d * sizeof(double2)) != hipSuccess ||
-                hipMalloc(&es.L, (size_t)h->n_kuniq * h->d * sizeof(double)) != hipSuccess) {
-                hipFree(es.V); hipFree(es.L);
-                es.V = nullptr; es.L = nullptr;
+                hipMalloc(&es.L, (size_t)h->n_kuniq * h->d * sizeof(double)) != hipSuccess ||
+                (sh && hipMalloc(&es.S, (size_t)h->n_kuniq * sh * 2) != hipSuccess)) {
+                hipFree(es.V); hipFree(es.L); hipFree(es.S);
+                es.V = nullptr; es.L = nullptr; es.S = nullptr;
                 (void)hipGetLastError();
                 return TH_OK;
             }
+            es.cap = h->n_kuniq;
         }
         int rc = th_launch_eig(h, which, qq);
         if (rc) return rc;
@@ -302,7 +311,8 @@ int ensure_bcache(th_handle *h, double Ef, double etap, int prec, bool presplit,
     if (h->cache_mb > 0) {
         size_t held = 0;
         for (const th_eig_set &es : h->eig)
-            if (es.V) held += (size_t)h->n_kuniq * h->d * (h->d * sizeof(double2) + sizeof(double));
+            if (es.V)
+                held += (size_t)es.cap * (h->d * (h->d * sizeof(double2) + sizeof(double)) + 2 * th_eig_split_halves(h->d));
         if (need + held > ((size_t)h->cache_mb << 20)) return TH_OK;
     }
     if (hipMalloc(&h->d_bcache, need) != hipSuccess) {
@@ -896,7 +906,8 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
     // while every one of them is still where it was at capture
     auto buffers = [h]() {
         return std::vector<const void *>{h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_bcache, h->d_x2,
-                                         h->d_p2, h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L,
+                                         h->d_p2, h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L, h->eig[0].S,
+                                         h->eig[1].S,
                                          h->d_ref_int, h->d_ref_part};
     };
     std::vector<const void *> ge_buf[2];
@@ -1104,7 +1115,7 @@ int th_destroy(th_handle *h)
     void *bufs[] = {h->d_hoff, h->d_R, h->d_jb, h->d_ib, h->d_t, h->d_h0, h->d_kim, h->d_yim, h->d_wim, h->d_kre,
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
                     h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk, h->d_kuniq, h->d_kid_im, h->d_kid_re,
-                    h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L, h->d_kim_ptr, h->d_kim_idx,
+                    h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L, h->eig[0].S, h->eig[1].S, h->d_kim_ptr, h->d_kim_idx,
                     h->d_kre_ptr, h->d_kre_idx, h->d_ref_int, h->d_ref_part};
     for (void *b : bufs)
         if (b) hipFree(b);

@@ -64,6 +64,7 @@ struct green_params {
     // the k-point index of every pair (imaginary list, real mesh)
     const double2 *eigV;
     const double *eigL;
+    const _Float16 *eigS;        // the split planes of eigV (th_eig_set::S), when kept
     const int *kid_im, *kid_re;
 };
 
@@ -1642,6 +1643,291 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
 }
 
 
+// Column-tiled spectral kernel on the fp16 matrix cores for d = 72 and 144
+// (mixed precision): G = V (diag(d) V^H) for a run of up to CH pairs on one
+// k-point, one workgroup per (run, tile of TC columns), one wave per 16-row
+// block.  The diagonal rides on the B side, so only the B tile depends on the
+// pair: A = 2^SA V (the wave's 16 rows, every eigen-index, two-term fp16
+// split, the scale keeping the lo parts of small entries normal) stays in
+// registers for the run, and so do each thread's groups of 8 eigen-indices of
+// conj(V) for the tile, in FP32.  Per pair a thread multiplies its groups by
+// 2^es d_m (the pair's diagonal with the power-of-two scale of the resolvent
+// bound folded in), splits them and writes the B tile into LDS (two tiles by
+// pair parity, one barrier per pair).  Products and output as in the run
+// kernel: hi.hi + hi.lo + lo.hi per real product, Ar Br and Ai Bi in separate
+// accumulators, panels as split words or FP32.
+template <int DD, int TC, int CH>
+__global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_per_eu(3))) void
+green_spectral_tiled_h2_kernel(green_params g)
+{
+    constexpr int NB = (DD + 15) / 16, NT = 64 * NB, KT = (DD + 31) / 32, KP = 32 * KT, TB = TC / 16;
+    constexpr int VST = KP + 8;                  // LDS row stride in halves (16-byte reads on distinct banks)
+    constexpr int GR = KP / 8, NG = TC * GR, GPT = (NG + NT - 1) / NT;   // 8-index groups of the tile
+    constexpr int SA = 14;                       // |2^SA V| <= 2^14 < the fp16 maximum
+    static_assert(TC % 16 == 0, "column tile of whole 16-wide blocks");
+    __shared__ __align__(16) _Float16 Bs[2][4][TC * VST];   // [pair parity][re hi|re lo|im hi|im lo][c][m]
+    __shared__ __align__(16) float2 dv[CH][KP];              // 2^es d of the workgroup's pairs
+    __shared__ double pzr[CH], pzi[CH], pzw[CH];
+    __shared__ int pkb[CH], pes[CH];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int c0 = (int)blockIdx.y * TC;
+    const long s0 = (long)blockIdx.x * CH;
+    const long s1 = s0 + CH < g.np ? s0 + CH : g.np;
+    const bool imag = g.mode == TH_G_IMAG_A || g.mode == TH_G_IMAG_B;
+    const bool bside = g.mode == TH_G_IMAG_B || g.mode == TH_G_REAL_B;
+    const int bs = g.bs, nA = g.nA, Mpad = g.Mpad, Mb = bs * bs;
+    // coefficients of d = (ca Re f + cb Im f) + i cc Im f, f = 1 / (z - lam)
+    auto coeffs = [&](double w, double &ca, double &cb, double &cc) {
+        ca = 1.0; cb = 0.0; cc = 1.0;
+        if (g.mode == TH_G_IMAG_A) { ca = w; cc = w; }
+        if (g.mode == TH_G_REAL_A) { ca = 0.0; cb = 2.0 * w; cc = 0.0; }
+        if (g.mode == TH_G_REAL_B) cc = -1.0;
+    };
+    if (tid < CH && s0 + tid < s1) {
+        const long p = g.p0 + s0 + tid;
+        double k0, k1, k2, er, ei, w;
+        pair_point(g, p, k0, k1, k2, er, ei, w);
+        long kb;
+        if (imag) {
+            kb = g.kid_im[p];
+        } else {
+            long ie, ik;
+            real_split(g, p, ie, ik);
+            kb = g.kid_re[ik];
+        }
+        pzr[tid] = er;
+        pzi[tid] = ei;
+        pzw[tid] = w;
+        pkb[tid] = (int)kb;
+        // power-of-two scale of d: |d_m| <= max(|ca|, |cb|, |cc|) / Im z = m 2^ex,
+        // m in [0.5, 1), so |2^es d_m| < 2^14 (an infinite or zero bound clamps)
+        double ca, cb, cc;
+        coeffs(w, ca, cb, cc);
+        const double bound = fmax(fmax(fabs(ca), fabs(cb)), fabs(cc)) / fabs(ei);
+        const int ex = (int)((__double_as_longlong(bound) >> 52) & 0x7ff) - 1022;
+        pes[tid] = min(100, max(-100, 14 - ex));
+    }
+    int rI[4], rR[4], cI[TB], cR[TB];
+    bool rv[4], cv[TB];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 16 * wave + 4 * (lane >> 4) + q;
+        rv[q] = r < DD;
+        rI[q] = r / bs;
+        rR[q] = (r - rI[q] * bs) * bs;
+    }
+#pragma unroll
+    for (int j = 0; j < TB; ++j) {
+        const int c = c0 + 16 * j + (lane & 15);
+        cv[j] = c < DD;
+        cI[j] = c / bs;
+        cR[j] = c - cI[j] * bs;
+    }
+    // valid 16-column blocks of this tile (the same in every thread)
+    const int nbv = min(TB, (DD - c0 + 15) / 16);
+    const int arow = 16 * wave + (lane & 15);   // row of V in this lane's A operand
+    const int kofs = 8 * (lane >> 4);            // first of the lane's 8 eigen-indices per K step
+    unsigned rof[4], cof[TB];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        rof[q] = rv[q] ? 4u * (unsigned)((bside ? rI[q] : rI[q] * nA) * 2 * Mpad + rR[q]) : TH_G_OOB;
+#pragma unroll
+    for (int j = 0; j < TB; ++j)
+        cof[j] = cv[j] ? 4u * (unsigned)((bside ? cI[j] * nA : cI[j]) * 2 * Mpad + cR[j]) : TH_G_OOB;
+    const int rowbytes = 4 * g.nA2 * 2 * Mpad, imoff = 4 * Mpad;
+    const int eP = (g.presplit && g.pareal > 0.0) ? th_real_exp(g.p2, g.pnx, g.pareal) : g.pexp;
+    ghv8 arh[KT], arl[KT], aih[KT], ail[KT];
+    float bsr[GPT][8], bsi[GPT][8];
+    // B values of this thread's groups for the next pair, built while the
+    // current pair's products run
+    ghv8 nbh[GPT], nbl[GPT], nih[GPT], nil[GPT];
+    // element i of group s for pair j: 2^es d_m conj(V[c][m]) = hi + lo (the lo
+    // part from one fused multiply-add of the exact FP32 remainder); a group
+    // past the tile reads row j of dv in range and is never stored
+    auto prep = [&](int j, int s, int i) {
+        const int gi = tid + NT * s, m0 = 8 * (gi - (gi / GR) * GR);
+        const float2 d = dv[j][m0 + i];
+        const float xr = fmaf(d.x, bsr[s][i], -d.y * bsi[s][i]);
+        const float xi = fmaf(d.x, bsi[s][i], d.y * bsr[s][i]);
+        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+        nbh[s][i] = hr;
+        nbl[s][i] = (_Float16)fmaf(xr, 1.0f, -(float)hr);
+        nih[s][i] = hi_;
+        nil[s][i] = (_Float16)fmaf(xi, 1.0f, -(float)hi_);
+    };
+    auto store = [&](int par) {
+#pragma unroll
+        for (int s = 0; s < GPT; ++s) {
+            const int gi = tid + NT * s;
+            if (gi < NG) {
+                const int cl = gi / GR, o = cl * VST + 8 * (gi - cl * GR);
+                *reinterpret_cast<ghv8 *>(&Bs[par][0][o]) = nbh[s];
+                *reinterpret_cast<ghv8 *>(&Bs[par][1][o]) = nbl[s];
+                *reinterpret_cast<ghv8 *>(&Bs[par][2][o]) = nih[s];
+                *reinterpret_cast<ghv8 *>(&Bs[par][3][o]) = nil[s];
+            }
+        }
+    };
+    constexpr int PE = (8 + KT - 1) / KT;       // next-pair elements per K step
+    long cur = -1;
+    __syncthreads();                            // the pairs' scalars
+    for (long pl = s0; pl < s1; ++pl) {
+        const int jj = (int)(pl - s0);
+        const long kb = pkb[jj];
+        if (kb != cur) {                        // same kb in every thread; the last
+                                                // pair ended with a barrier
+            // A and the B sources from the k-point's split planes of 2^SA V (rows
+            // and eigen-indices padded with zeros): sixteen bytes per plane and
+            // load, straight into the operand registers
+            constexpr int RP = 16 * NB, PL = RP * KP / 8;       // 8-half vectors per plane
+            const ghv8 *Sk = reinterpret_cast<const ghv8 *>(g.eigS) + (size_t)kb * 4 * PL;
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int o = (arow * KP + 32 * t + kofs) / 8;
+                arh[t] = Sk[o];
+                arl[t] = Sk[PL + o];
+                aih[t] = Sk[2 * PL + o];
+                ail[t] = Sk[3 * PL + o];
+            }
+            const float us = ldexpf(1.0f, -SA);
+#pragma unroll
+            for (int s = 0; s < GPT; ++s) {
+                const int gi = tid + NT * s, cl = gi / GR, m0 = 8 * (gi - cl * GR);
+                const bool in = gi < NG;                        // rows c0 + cl < RP
+                const int o = ((in ? c0 + cl : 0) * KP + m0) / 8;
+                const ghv8 rh = Sk[o], rl = Sk[PL + o], ih = Sk[2 * PL + o], il = Sk[3 * PL + o];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {   // conj(V[c][m]) to 22 bits
+                    bsr[s][i] = in ? ((float)rh[i] + (float)rl[i]) * us : 0.0f;
+                    bsi[s][i] = in ? -((float)ih[i] + (float)il[i]) * us : 0.0f;
+                }
+            }
+            // 2^es d for every later pair of the workgroup on this k-point
+            const double *Lk = g.eigL + (size_t)kb * DD;
+            for (int e = tid; e < CH * KP; e += NT) {
+                const int j = e / KP, m = e - (e / KP) * KP;
+                const double lam = Lk[m < DD ? m : DD - 1];
+                if (j >= jj && s0 + j < s1 && pkb[j] == kb) {
+                    float2 v = make_float2(0.0f, 0.0f);
+                    if (m < DD) {
+                        double ca, cb, cc;
+                        coeffs(pzw[j], ca, cb, cc);
+                        const double ei = pzi[j], a = pzr[j] - lam, n = a * a + ei * ei;
+                        const double fr = a / n, fi = -ei / n;
+                        v = make_float2((float)ldexp(ca * fr + cb * fi, pes[j]), (float)ldexp(cc * fi, pes[j]));
+                    }
+                    dv[j][m] = v;
+                }
+            }
+            cur = kb;
+            __syncthreads();                    // the run's diagonals
+#pragma unroll
+            for (int s = 0; s < GPT; ++s)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) prep(jj, s, i);
+            store(jj & 1);                      // last read two pairs ago
+            __syncthreads();                    // the run's first B tile
+        }
+        // B of this pair is in tile par; B of the next pair on this k-point goes
+        // to the other tile (last read by the previous pair, whose barrier every
+        // thread has passed)
+        const int par = jj & 1;
+        const bool nxt = pl + 1 < s1 && pkb[jj + 1] == kb;
+        sfv4 acp[TB], acn[TB], aci[TB];
+#pragma unroll
+        for (int j = 0; j < TB; ++j)
+            for (int q = 0; q < 4; ++q) { acp[j][q] = 0.0f; acn[j][q] = 0.0f; aci[j][q] = 0.0f; }
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            if (j >= nbv) break;
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int bo = (16 * j + (lane & 15)) * VST + 32 * t + kofs;
+                const ghv8 brh = *reinterpret_cast<const ghv8 *>(&Bs[par][0][bo]);
+                const ghv8 brl = *reinterpret_cast<const ghv8 *>(&Bs[par][1][bo]);
+                const ghv8 bih = *reinterpret_cast<const ghv8 *>(&Bs[par][2][bo]);
+                const ghv8 bil = *reinterpret_cast<const ghv8 *>(&Bs[par][3][bo]);
+                acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], brh, acp[j], 0, 0, 0);
+                acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], bih, acn[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], bih, aci[j], 0, 0, 0);
+                acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], brl, acp[j], 0, 0, 0);
+                acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], bil, acn[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[t], bil, aci[j], 0, 0, 0);
+                acp[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[t], brh, acp[j], 0, 0, 0);
+                acn[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], bih, acn[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[t], bih, aci[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], brh, aci[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], brl, aci[j], 0, 0, 0);
+                aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], brh, aci[j], 0, 0, 0);
+                if (j == 0 && nxt) {            // the next pair's B, a few elements per step
+#pragma unroll
+                    for (int s = 0; s < GPT; ++s)
+#pragma unroll
+                        for (int e = 0; e < PE; ++e)
+                            if (PE * t + e < 8) prep(jj + 1, s, PE * t + e);
+                }
+            }
+        }
+        if (nxt) store(par ^ 1);
+        // accumulators = 2^(es + SA) G
+        const int es = pes[jj] + SA;
+        void *row = reinterpret_cast<unsigned *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
+        const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(row, (short)0, rowbytes, 0x00020000);
+        auto put = [&](int j, int q, unsigned wr, unsigned wi) {
+            const unsigned o = rof[q] + cof[j];
+            __builtin_amdgcn_raw_buffer_store_b32(wr, ro, o, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(wi, ro, o, imoff, 0);
+        };
+        auto pack = [](_Float16 h, _Float16 l) {   // hi in the low half
+            typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+            const h2v v = {h, l};
+            return __builtin_bit_cast(unsigned, v);
+        };
+        if (g.presplit && eP - es >= -126 && eP - es <= 127) {
+            // y 2^(eP-es) and y 2^(eP-es) - hi are exact in FP32: one fused
+            // multiply-add rounded once to fp16 gives each part of x 2^eP
+            const float fs = ldexpf(1.0f, eP - es);
+#pragma unroll
+            for (int j = 0; j < TB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float yr = acp[j][q] - acn[j][q], yi = aci[j][q];
+                    const _Float16 hr = (_Float16)fmaf(yr, fs, 0.0f), hi_ = (_Float16)fmaf(yi, fs, 0.0f);
+                    const _Float16 lr = (_Float16)fmaf(yr, fs, -(float)hr), li = (_Float16)fmaf(yi, fs, -(float)hi_);
+                    put(j, q, pack(hr, lr), pack(hi_, li));
+                }
+        } else if (g.presplit) {
+#pragma unroll
+            for (int j = 0; j < TB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float xr = ldexpf(acp[j][q] - acn[j][q], eP - es), xi = ldexpf(aci[j][q], eP - es);
+                    const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+                    const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+                    put(j, q, pack(hr, lr), pack(hi_, li));
+                }
+        } else {
+            const float back = ldexpf(1.0f, -es);
+#pragma unroll
+            for (int j = 0; j < TB; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    put(j, q, __builtin_bit_cast(unsigned, (acp[j][q] - acn[j][q]) * back),
+                        __builtin_bit_cast(unsigned, aci[j][q] * back));
+        }
+        if (blockIdx.y == 0) {
+            const int npad = Mpad - Mb;
+            for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
+                const int part = e / npad;
+                __builtin_amdgcn_raw_buffer_store_b32(0u, ro, 4u * (unsigned)(part * Mpad + Mb + (e - part * npad)), 0,
+                                                      0);
+            }
+        }
+        __syncthreads();                        // the next pair's B tile; this pair's reads done
+    }
+}
+
+
 template <typename P, int NT>
 __global__ __launch_bounds__(NT) void green_kernel(green_params g)
 {
@@ -1840,10 +2126,27 @@ int launch_green_spectral_tiled(th_handle *h, const th_green_args &a, const gree
     return TH_OK;
 }
 
+// the fp16 column-tiled kernel (mixed precision, panels as the run kernel
+// writes them; raw G and the per-pair variant stay with the FP32 kernel)
+template <int DD, int TC>
+int launch_green_spectral_tiled_h2(th_handle *h, const th_green_args &a, const green_params &g)
+{
+    if ((size_t)4 * g.nA2 * 2 * g.Mpad >= TH_G_OOB) {
+        th_set_error("half-precision spectral kernel: panel row too large");
+        return TH_E_ARG;
+    }
+    constexpr int CH = TH_RUN, NT = 64 * ((DD + 15) / 16);
+    const dim3 grid((unsigned)((a.np + CH - 1) / CH), (unsigned)((DD + TC - 1) / TC));
+    hipLaunchKernelGGL((green_spectral_tiled_h2_kernel<DD, TC, CH>), grid, dim3(NT), 0, gstream(h, a), g);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
 template <typename P>
 int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
 {
-    if (a.presplit && (h->kernel_variant != 0 || !g.eigV || (h->d != 18 && h->d != 36 && h->d != 54))) {
+    if (a.presplit && (h->kernel_variant != 0 || !g.eigV ||
+                       (h->d != 18 && h->d != 36 && h->d != 54 && h->d != 72 && h->d != 144))) {
         th_set_error("split half-precision panels come from the half-precision spectral kernel only");
         return TH_E_STATE;
     }
@@ -1869,12 +2172,17 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
         }
     }
     if (g.eigV) {
+        const bool h2 = sizeof(P) == sizeof(float) && h->green_f16 && !a.raw && h->spectral_chunk > 0 && g.eigS;
         switch (h->d) {
         case 18: return launch_green_spectral<P, 18>(h, a, g);
         case 36: return launch_green_spectral<P, 36>(h, a, g);
         case 54: return launch_green_spectral<P, 54>(h, a, g);
-        case 72: return launch_green_spectral_tiled<P, 72, 80, 1>(h, a, g);
-        case 144: return launch_green_spectral_tiled<P, 144, 48, 1>(h, a, g);
+        case 72:
+            if (h2) return launch_green_spectral_tiled_h2<72, 16>(h, a, g);
+            return launch_green_spectral_tiled<P, 72, 80, 1>(h, a, g);
+        case 144:
+            if (h2) return launch_green_spectral_tiled_h2<144, 16>(h, a, g);
+            return launch_green_spectral_tiled<P, 144, 48, 1>(h, a, g);
         case 288: return launch_green_spectral_tiled<P, 288, 32, 2>(h, a, g);
         default: break;
         }
@@ -1941,8 +2249,11 @@ long th_green_wave_pairs(th_handle *h)
 
 bool th_green_presplit_ok(const th_handle *h, const double *q)
 {
-    return h->presplit && h->kernel_variant == 0 && h->spectral_chunk > 0 && h->green_f16 &&
-           (h->d == 18 || h->d == 36 || h->d == 54) && side_eig(h, true, q) && side_eig(h, false, q);
+    if (!(h->presplit && h->kernel_variant == 0 && h->spectral_chunk > 0 && h->green_f16)) return false;
+    const th_eig_set *ea = side_eig(h, true, q), *eb = side_eig(h, false, q);
+    if (!ea || !eb) return false;
+    if (h->d == 18 || h->d == 36 || h->d == 54) return true;
+    return (h->d == 72 || h->d == 144) && ea->S && eb->S;   // the tiled kernel reads the split planes
 }
 
 int th_launch_green(th_handle *h, const th_green_args &a)
@@ -1980,6 +2291,7 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     const th_eig_set *es = side_eig(h, qside, a.q);
     g.eigV = es ? es->V : nullptr;
     g.eigL = es ? es->L : nullptr;
+    g.eigS = es ? reinterpret_cast<const _Float16 *>(es->S) : nullptr;
     g.kid_im = h->d_kid_im;
     g.kid_re = h->d_kid_re;
     h->n_launch[0] += 1;
@@ -2187,6 +2499,46 @@ void th_eig_release(th_handle *h)
     }
 }
 
+size_t th_eig_split_halves(int d)
+{
+    if (d != 72 && d != 144) return 0;
+    return (size_t)4 * (16 * ((d + 15) / 16)) * (32 * ((d + 31) / 32));
+}
+
+namespace {
+
+// the split planes of 2^14 V, one thread per (k-point, padded row, padded
+// eigen-index): hi = fp16(x), lo = fp16(x - hi) with x the FP32 value
+__global__ __launch_bounds__(256) void split_v_kernel(const double2 *V, long nk, int d, int RP, int KP,
+                                                      _Float16 *S)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, pl = (size_t)RP * KP;
+    if (e >= (size_t)nk * pl) return;
+    const size_t k = e / pl, rm = e - k * pl;
+    const int r = (int)(rm / KP), m = (int)(rm - (size_t)r * KP);
+    double2 x = make_double2(0.0, 0.0);
+    if (r < d && m < d) x = V[(k * d + r) * d + m];
+    const float xr = (float)x.x * 16384.0f, xi = (float)x.y * 16384.0f;
+    const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+    _Float16 *o = S + k * 4 * pl + rm;
+    o[0] = hr;
+    o[pl] = (_Float16)(xr - (float)hr);
+    o[2 * pl] = hi;
+    o[3 * pl] = (_Float16)(xi - (float)hi);
+}
+
+int launch_split_v(th_handle *h, const th_eig_set &es)
+{
+    const int d = h->d, RP = 16 * ((d + 15) / 16), KP = 32 * ((d + 31) / 32);
+    const size_t n = (size_t)h->n_kuniq * RP * KP;
+    hipLaunchKernelGGL(split_v_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, es.V, h->n_kuniq,
+                       d, RP, KP, reinterpret_cast<_Float16 *>(es.S));
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+}  // namespace
+
 // Eigenpairs of H(k + q) for the handle's unique k-points into set 'which'
 int th_launch_eig(th_handle *h, int which, const double *q)
 {
@@ -2210,8 +2562,11 @@ int th_launch_eig(th_handle *h, int which, const double *q)
         break;
     case 72:
     case 144:
-    case 288:
-        return launch_eig_dense(h, which, q);
+    case 288: {
+        const int rc = launch_eig_dense(h, which, q);
+        if (rc || !es.S) return rc;
+        return launch_split_v(h, es);
+    }
     default:
         th_set_error("spectral Green functions are built for d = 18, 36, 54, 72, 144, 288");
         return TH_E_UNSUPPORTED;

@@ -28,6 +28,12 @@ inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
 struct th_eig_set {
     double2 *V = nullptr;        // [n_kuniq][d][d] row-major, V[r][m] = component r of eigenvector m
     double *L = nullptr;         // [n_kuniq][d]
+    // V as the fp16 matrix-core kernels read it (d = 72 and 144, else
+    // nullptr): per k-point four planes (Re hi, Re lo, Im hi, Im lo) of the
+    // two-term split of 2^14 V, [4][RP][KP] halves with the rows padded to RP =
+    // 16 ceil(d/16) and the eigen-indices to KP = 32 ceil(d/32), zero there
+    void *S = nullptr;
+    long cap = 0;                // k-points the buffers hold
     double q[3] = {0, 0, 0};
 };
 
@@ -235,6 +241,8 @@ long th_green_wave_pairs(th_handle *h);
 int th_launch_eig(th_handle *h, int which, const double *q);
 // d with spectral Green function kernels (and an eigen-decomposition for them)
 bool th_spectral_d(int d);
+// halves per k-point of th_eig_set::S for this d (0: not kept)
+size_t th_eig_split_halves(int d);
 // releases the library handle of the dense eigen-decompositions
 void th_eig_release(th_handle *h);
 
