@@ -1399,7 +1399,8 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
 // Mixed precision variant of the run kernel on the fp16 matrix cores
 // (v_mfma_f32_16x16x32_f16, K = 32 eigen-indices per instruction).  The
 // operands are split in two fp16 parts (hi + lo, 22 significant bits):
-// V^H once per run (|V| <= 1 needs no scale), V diag(d) per pair under a
+// V^H once per run (copied from the split planes of 2^14 V kept with the
+// eigenpairs, the scale keeping small entries' lo parts normal), V diag(d) per pair under a
 // power-of-two scale from |d_m| <= max(|ca|, |cb|, |cc|) / Im z (the
 // resolvent bound of a Hermitian H), and each real product is hi.hi +
 // hi.lo + lo.hi accumulated in FP32 (Ar Br and Ai Bi in separate
@@ -1500,38 +1501,39 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
         if (g.mode == TH_G_REAL_B) cc = -1.0;
         if (kb != cur) {                        // same kb in every thread
             __syncthreads();                    // the previous run's reads of Vs and dv are done
-            const double2 *Vk = g.eigV + (size_t)kb * DD * DD;
-            for (int e = tid; e < NC * KP; e += NT) {
-                const int c = e / KP, m = e - (e / KP) * KP;
-                float xr = 0.0f, xi = 0.0f;
-                if (c < DD && m < DD) {
-                    const double2 x = Vk[(size_t)c * DD + m];
-                    xr = (float)x.x;
-                    xi = (float)(-x.y);
-                }
-                const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
-                Vs[0][0][c * VST + m] = hr;
-                Vs[0][1][c * VST + m] = (_Float16)(xr - (float)hr);
-                Vs[1][0][c * VST + m] = hi_;
-                Vs[1][1][c * VST + m] = (_Float16)(xi - (float)hi_);
+            // V^H and the A rows from the k-point's split planes of 2^14 V (rows
+            // and eigen-indices padded with zeros): unconditional 16-byte loads,
+            // all issued before the first wait
+            constexpr int PL = NC * KP / 8;      // 8-half vectors per plane
+            const ghv8 *Sk = reinterpret_cast<const ghv8 *>(g.eigS) + (size_t)kb * 4 * PL;
+            for (int e = tid; e < PL; e += NT) {
+                const int c = e / (KP / 8), m0 = 8 * (e - c * (KP / 8));
+                const ghv8 rh = Sk[e], rl = Sk[PL + e], ih = Sk[2 * PL + e], il = Sk[3 * PL + e];
+                *reinterpret_cast<ghv8 *>(&Vs[0][0][c * VST + m0]) = rh;
+                *reinterpret_cast<ghv8 *>(&Vs[0][1][c * VST + m0]) = rl;
+                *reinterpret_cast<ghv8 *>(&Vs[1][0][c * VST + m0]) = -ih;   // conj
+                *reinterpret_cast<ghv8 *>(&Vs[1][1][c * VST + m0]) = -il;
             }
+            const float us = ldexpf(1.0f, -14);
 #pragma unroll
-            for (int t = 0; t < KT; ++t)
+            for (int t = 0; t < KT; ++t) {
+                const int o = (arow * KP + 32 * t + kofs) / 8;
+                const ghv8 rh = Sk[o], rl = Sk[PL + o], ih = Sk[2 * PL + o], il = Sk[3 * PL + o];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int m = 32 * t + kofs + i;
-                    double2 x = make_double2(0.0, 0.0);
-                    if (arow < DD && m < DD) x = Vk[(size_t)arow * DD + m];
-                    avr[t][i] = (float)x.x;
-                    avi[t][i] = (float)x.y;
+                for (int i = 0; i < 8; ++i) {   // V to 22 bits
+                    avr[t][i] = ((float)rh[i] + (float)rl[i]) * us;
+                    avi[t][i] = ((float)ih[i] + (float)il[i]) * us;
                 }
+            }
             // the diagonals of every later pair of the workgroup on this k-point,
-            // so that the run needs no barrier per pair
+            // so that the run needs no barrier per pair (eigenvalue loads ahead of
+            // the branch)
             const double *Lk = g.eigL + (size_t)kb * DD;
             for (int e = tid; e < CH * KP; e += NT) {
                 const int j = e / KP, m = e - (e / KP) * KP;
+                const double lam = Lk[m < DD ? m : DD - 1];
                 if (j >= jj && s0 + j < s1 && pkb[j] == kb)
-                    dv[j][m] = m < DD ? diagonal(j, Lk[m]) : make_float2(0.0f, 0.0f);
+                    dv[j][m] = m < DD ? diagonal(j, lam) : make_float2(0.0f, 0.0f);
             }
             cur = kb;
             __syncthreads();                    // the run's V and diagonals in LDS
@@ -1585,7 +1587,9 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
                 aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[t], brl, aci[j], 0, 0, 0);
                 aci[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[t], brh, aci[j], 0, 0, 0);
             }
-        const float back = ldexpf(1.0f, -es);
+        // accumulators = 2^(es + 14) G (A carries 2^es, V^H from the planes 2^14)
+        const int eo = es + 14;
+        const float back = ldexpf(1.0f, -eo);
         const int npad = Mpad - Mb;
         void *row = reinterpret_cast<unsigned *>(g.panel) + panel_row(g, pl) * g.nA2 * 2 * Mpad;
         const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(row, (short)0, rowbytes, 0x00020000);
@@ -1599,13 +1603,13 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
             const h2v v = {h, l};
             return __builtin_bit_cast(unsigned, v);
         };
-        if (g.presplit && eP - es >= -126 && eP - es <= 127) {
+        if (g.presplit && eP - eo >= -126 && eP - eo <= 127) {
             // x 2^eP = hi + lo as the contraction would split the FP32 value
             // x = acc 2^-es; y 2^(eP-es) (y the accumulator, 2^(eP-es) a normal
             // float) and y 2^(eP-es) - hi are exact in FP32, so one fused
             // multiply-add rounded once to fp16 gives each part; hi and lo share
             // one word
-            const float fs = ldexpf(1.0f, eP - es);
+            const float fs = ldexpf(1.0f, eP - eo);
 #pragma unroll
             for (int j = 0; j < NB; ++j)
 #pragma unroll
@@ -1622,7 +1626,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
             for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const float xr = ldexpf(acp[j][q] - acn[j][q], eP - es), xi = ldexpf(aci[j][q], eP - es);
+                    const float xr = ldexpf(acp[j][q] - acn[j][q], eP - eo), xi = ldexpf(aci[j][q], eP - eo);
                     const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
                     const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
                     put(j, q, pack(hr, lr), pack(hi_, li));
@@ -2082,14 +2086,14 @@ int launch_green_block(th_handle *h, const th_green_args &a, const green_params 
 template <typename P, int DD>
 int launch_green_spectral(th_handle *h, const th_green_args &a, const green_params &g)
 {
-    if (a.presplit && !(sizeof(P) == sizeof(float) && !a.raw && h->spectral_chunk > 0 && h->green_f16)) {
+    if (a.presplit && !(sizeof(P) == sizeof(float) && !a.raw && h->spectral_chunk > 0 && h->green_f16 && g.eigS)) {
         th_set_error("split half-precision panels come from the half-precision spectral kernel only");
         return TH_E_STATE;
     }
     if (!a.raw && h->spectral_chunk > 0) {   // runs of pairs per k-point; G is not kept
         constexpr int CH = TH_RUN, NT = 64 * ((DD + 15) / 16);
         const unsigned nblk = (unsigned)((a.np + CH - 1) / CH);
-        if (sizeof(P) == sizeof(float) && h->green_f16) {
+        if (sizeof(P) == sizeof(float) && h->green_f16 && g.eigS) {
             if ((size_t)4 * g.nA2 * 2 * g.Mpad >= TH_G_OOB) {
                 th_set_error("half-precision spectral kernel: panel row too large");
                 return TH_E_ARG;
@@ -2252,8 +2256,8 @@ bool th_green_presplit_ok(const th_handle *h, const double *q)
     if (!(h->presplit && h->kernel_variant == 0 && h->spectral_chunk > 0 && h->green_f16)) return false;
     const th_eig_set *ea = side_eig(h, true, q), *eb = side_eig(h, false, q);
     if (!ea || !eb) return false;
-    if (h->d == 18 || h->d == 36 || h->d == 54) return true;
-    return (h->d == 72 || h->d == 144) && ea->S && eb->S;   // the tiled kernel reads the split planes
+    // the fp16 kernels read the split planes of V
+    return (h->d == 18 || h->d == 36 || h->d == 54 || h->d == 72 || h->d == 144) && ea->S && eb->S;
 }
 
 int th_launch_green(th_handle *h, const th_green_args &a)
@@ -2501,7 +2505,7 @@ void th_eig_release(th_handle *h)
 
 size_t th_eig_split_halves(int d)
 {
-    if (d != 72 && d != 144) return 0;
+    if (d != 18 && d != 36 && d != 54 && d != 72 && d != 144) return 0;
     return (size_t)4 * (16 * ((d + 15) / 16)) * (32 * ((d + 31) / 32));
 }
 
@@ -2572,5 +2576,5 @@ int th_launch_eig(th_handle *h, int which, const double *q)
         return TH_E_UNSUPPORTED;
     }
     TH_HIP(hipGetLastError());
-    return TH_OK;
+    return es.S ? launch_split_v(h, es) : TH_OK;
 }
