@@ -180,17 +180,19 @@ def other_configs(log_fn):
     """Throughput of the other configurations (reported beside the headline,
     each at its own size): cfg0, the 1-site bcc element (18 x 18 H, 16^3 k,
     64 + 32 energies, all 16 frequencies); cfg1, the 8-layer fcc(001) slab
-    (144 x 144 H, every 4th of its 128^2 k-points, one frequency); cfg4, the
-    40-cell uneven batch (18 x 18 to 288 x 288, one frequency).  Library
-    default kernel shapes; each timed sweep after an untimed one; for cfg0
-    also the oracle on a bounded sample (the configuration the CPU baseline
-    is specified on)."""
+    (144 x 144 H, every 4th of its 128^2 k-points, 4 of its 32 frequencies
+    spread over the grid, so that the sweep's one eigen-decomposition is
+    shared as in a sweep); cfg4, the 40-cell uneven batch (18 x 18 to
+    288 x 288, one frequency).  Library default kernel shapes; each timed
+    sweep after an untimed one (each sweep diagonalises H(k) afresh); for
+    cfg0 also the oracle on a bounded sample (the configuration the CPU
+    baseline is specified on)."""
     from titan_amd import binding as B
     from titan_amd import configs as CF
     from titan_amd.batch import UnevenBatch
     from titan_amd.chi import ChiWorkload
     out = {}
-    for cfg, kstep, nom in ((0, 1, None), (1, 4, 1)):
+    for cfg, kstep, nom in ((0, 1, None), (1, 4, 4)):
         spec, run = CF.system(cfg), CF.run_params(cfg)
         wl = ChiWorkload(spec, run)
         if kstep > 1:
@@ -199,7 +201,7 @@ def other_configs(log_fn):
             wl.h.set_imag_points(np.tile(ks, (len(wl.y), 1)), np.repeat(wl.y, wl.nk), np.outer(wl.wy, ws).ravel())
             wl.h.set_real_kmesh(ks, ws)
         om = np.asarray(run["omega"], dtype=float)
-        om = om if nom is None else om[-nom:]
+        om = om if nom is None else om[np.round(np.linspace(len(om) - 1, 0, nom)).astype(int)]
         ent = {"workload": "cfg%d: %s, H %dx%d, %d of the %s k-points, %d imag + %d real energies, %d omega" %
                            (cfg, spec["name"], wl.model.d, wl.model.d, wl.nk, "x".join(map(str, run["nk"])),
                             len(wl.y), wl.n_x, len(om)),

@@ -1249,8 +1249,12 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
     constexpr int NB = (DD + 15) / 16, NW = (NB + RPW - 1) / RPW, NT = 64 * NW;
     constexpr int KS = (DD + 3) / 4, KP = 4 * KS, TB = TC / 16;
     static_assert(TC % 16 == 0, "column tile of whole 16-wide blocks");
+    static_assert(DD % 4 == 0 && KP == DD, "eigen-indices in whole K steps");
     __shared__ cplx<R> Vh[KP * TC];      // Vh[m][c - c0] = conj(V[c][m]); zero outside the matrix
     __shared__ cplx<double> dg[KP];      // diagonal of the current pair
+    __shared__ double lam[KP];           // eigenvalues of the current run's k-point
+    __shared__ double pzr[CH], pzi[CH], pzw[CH];   // Re z, Im z and weight of the workgroup's pairs
+    __shared__ int pkb[CH];              // their eigen-decomposition
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int c0 = (int)blockIdx.y * TC;
     const long s0 = (long)blockIdx.x * CH;
@@ -1279,10 +1283,17 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
         cI[j] = c / bs;
         cR[j] = c - cI[j] * bs;
     }
-    long cur = -1;
-    const double2 *Vk = nullptr;
-    for (long pl = s0; pl < s1; ++pl) {
-        const long p = g.p0 + pl;
+    // rows past the matrix (d = 72) read row DD - 1 and are zeroed
+    int arc[RPW];
+    bool arv[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+        arv[i] = arow[i] < DD;
+        arc[i] = arv[i] ? arow[i] : DD - 1;
+    }
+    // the pairs' scalars, read once
+    if (tid < CH && s0 + tid < s1) {
+        const long p = g.p0 + s0 + tid;
         double k0, k1, k2, er, ei, w;
         pair_point(g, p, k0, k1, k2, er, ei, w);
         long kb;
@@ -1293,19 +1304,31 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
             real_split(g, p, ie, ik);
             kb = g.kid_re[ik];
         }
-        __syncthreads();                        // the previous pair's reads of dg (and Vh) are done
+        pzr[tid] = er;
+        pzi[tid] = ei;
+        pzw[tid] = w;
+        pkb[tid] = (int)kb;
+    }
+    long cur = -1;
+    const double2 *Vk = nullptr;
+    for (long pl = s0; pl < s1; ++pl) {
+        const int jj = (int)(pl - s0);
+        __syncthreads();                        // the scalars; the previous pair's reads of dg, Vh, lam done
+        const long kb = pkb[jj];
+        const double er = pzr[jj], ei = pzi[jj], w = pzw[jj];
         if (kb != cur) {                        // same kb in every thread
             Vk = g.eigV + (size_t)kb * DD * DD;
+            // consecutive threads read consecutive eigen-indices of one row of
+            // V; the index is clamped (no branch per load) and the entry zeroed
             for (int e = tid; e < KP * TC; e += NT) {
-                const int m = e / TC, cc = e - (e / TC) * TC, c = c0 + cc;
-                cplx<R> x = {R(0), R(0)};
-                if (m < DD && c < DD) {
-                    const double2 v = Vk[(size_t)c * DD + m];
-                    x = {(R)v.x, (R)(-v.y)};
-                }
-                Vh[e] = x;
+                const int cc = e / KP, m = e - (e / KP) * KP, c = c0 + cc;
+                const bool in = c < DD;
+                const double2 v = Vk[(size_t)(in ? c : DD - 1) * DD + m];
+                Vh[m * TC + cc] = in ? cplx<R>{(R)v.x, (R)(-v.y)} : cplx<R>{R(0), R(0)};
             }
+            for (int m = tid; m < KP; m += NT) lam[m] = g.eigL[(size_t)kb * DD + m];
             cur = kb;
+            __syncthreads();                    // the run's V^H and eigenvalues
         }
         // diagonal d = (ca Re f + cb Im f) + i cc Im f, f = 1 / (z - lambda_m);
         // raw output takes G itself (d = f), the panels are then made from it
@@ -1316,13 +1339,9 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
             if (g.mode == TH_G_REAL_B) cc = -1.0;             // conj(f)
         }
         for (int m = tid; m < KP; m += NT) {
-            cplx<double> v = {0.0, 0.0};
-            if (m < DD) {
-                const double a = er - g.eigL[(size_t)kb * DD + m], n = a * a + ei * ei;
-                const double fr = a / n, fi = -ei / n;
-                v = {ca * fr + cb * fi, cc * fi};
-            }
-            dg[m] = v;
+            const double a = er - lam[m], n = a * a + ei * ei;
+            const double fr = a / n, fi = -ei / n;
+            dg[m] = {ca * fr + cb * fi, cc * fi};
         }
         __syncthreads();
         typename smfma<R>::acc accr[RPW][TB], acci[RPW][TB];
@@ -1331,29 +1350,46 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
 #pragma unroll
             for (int j = 0; j < TB; ++j)
                 for (int q = 0; q < 4; ++q) { accr[i][j][q] = 0; acci[i][j][q] = 0; }
-#pragma unroll(RPW == 1 ? 2 : 1)
-        for (int ks = 0; ks < KS; ++ks) {
-            const int m = 4 * ks + (lane >> 4);
-            const cplx<double> dm = dg[m];
-            R ar[RPW], ai[RPW], nai[RPW];
+        // A = V[r][m] d_m from device memory (the k-point's V stays in L2 over
+        // its run), loaded PF K steps ahead of its use, every load unconditional
+        // (the tail re-reads the last step)
+        constexpr int PF = (sizeof(R) == 8 && RPW > 1) ? 1 : 4;   // FP64 accumulators of two row blocks leave room for 1
+        static_assert(KS >= PF, "prefetch depth");
+        double2 vq[PF][RPW];
+        const double2 *Vl = Vk + (lane >> 4);
 #pragma unroll
-            for (int i = 0; i < RPW; ++i) {
-                double2 v = make_double2(0.0, 0.0);
-                if (arow[i] < DD && m < DD) v = Vk[(size_t)arow[i] * DD + m];
-                const cplx<double> x = cmul(cplx<double>{v.x, v.y}, dm);
-                ar[i] = (R)x.x;
-                ai[i] = (R)x.y;
-                nai[i] = -ai[i];
-            }
+        for (int u = 0; u < PF; ++u)
 #pragma unroll
-            for (int j = 0; j < TB; ++j) {
-                const cplx<R> b = Vh[m * TC + 16 * j + (lane & 15)];
+            for (int i = 0; i < RPW; ++i) vq[u][i] = Vl[(size_t)arc[i] * DD + 4 * u];
+        for (int k0 = 0; k0 < KS; k0 += PF) {
 #pragma unroll
-                for (int i = 0; i < RPW; ++i) {
-                    accr[i][j] = smfma<R>::mac(ar[i], b.x, accr[i][j]);
-                    accr[i][j] = smfma<R>::mac(nai[i], b.y, accr[i][j]);
-                    acci[i][j] = smfma<R>::mac(ar[i], b.y, acci[i][j]);
-                    acci[i][j] = smfma<R>::mac(ai[i], b.x, acci[i][j]);
+            for (int u = 0; u < PF; ++u) {
+                const int ks = k0 + u;
+                if (ks < KS) {                  // the same in every thread
+                    const int m = 4 * ks + (lane >> 4);
+                    const cplx<double> dm = dg[m];
+                    const int kn = ks + PF < KS ? ks + PF : KS - 1;
+                    R ar[RPW], ai[RPW], nai[RPW];
+#pragma unroll
+                    for (int i = 0; i < RPW; ++i) {
+                        const double2 v = vq[u][i];
+                        vq[u][i] = Vl[(size_t)arc[i] * DD + 4 * kn];
+                        const cplx<double> x = cmul(cplx<double>{v.x, v.y}, dm);
+                        ar[i] = arv[i] ? (R)x.x : R(0);
+                        ai[i] = arv[i] ? (R)x.y : R(0);
+                        nai[i] = -ai[i];
+                    }
+#pragma unroll
+                    for (int j = 0; j < TB; ++j) {
+                        const cplx<R> b = Vh[m * TC + 16 * j + (lane & 15)];
+#pragma unroll
+                        for (int i = 0; i < RPW; ++i) {
+                            accr[i][j] = smfma<R>::mac(ar[i], b.x, accr[i][j]);
+                            accr[i][j] = smfma<R>::mac(nai[i], b.y, accr[i][j]);
+                            acci[i][j] = smfma<R>::mac(ar[i], b.y, acci[i][j]);
+                            acci[i][j] = smfma<R>::mac(ai[i], b.x, acci[i][j]);
+                        }
+                    }
                 }
             }
         }
