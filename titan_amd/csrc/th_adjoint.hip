@@ -275,6 +275,7 @@ __global__ void unfinalize_2(const double2 *__restrict__ chat, double *__restric
 struct assemble_params {
     int mode, nA, bs, d, Mpad, nA2;
     long p0, np, nk_re;
+    int nx;                  // real-axis nodes; real pairs k-major, p = ik nx + ie
     const double *wim, *p2, *wre;
     const double *hat;       // [np][nA2][2][Mpad]
     double2 *out;            // [np][d][d]
@@ -301,7 +302,7 @@ __global__ void assemble_kernel(assemble_params a)
     } else if (a.mode == TH_G_IMAG_B) {     // B(i,j) = G(j,i)
         v = hat(ca * nA + ra, rr * bs + cc);
     } else if (a.mode == TH_G_REAL_A) {     // A = -i w (Gq - Gq^H):  Gqhat = i w (Ahat + Ahat^H)
-        const long ie = p / a.nk_re, ik = p - ie * a.nk_re;
+        const long ik = p / a.nx, ie = p - ik * a.nx;
         const double w = a.p2[ie] * a.wre[ik];
         const double2 x = hat(ra * nA + ca, rr * bs + cc), y = hat(ca * nA + ra, cc * bs + rr);
         const double sr = x.x + y.x, si = x.y - y.y;
@@ -394,76 +395,112 @@ __device__ inline double2 ghat_entry(const assemble_params &a, const double *H, 
     return zconj(hat(ra * nA + ca, cc * bs + rr));
 }
 
-__global__ __launch_bounds__(512) void hhat_fused_kernel(assemble_params a, const double2 *__restrict__ Graw, int ppw)
+// Hhat summed over a segment of consecutive pairs on one k-point (such pairs
+// share the phases of the hop reduction): one workgroup per segment, ppw pairs
+// at a time with their G and Ghat in LDS; a thread owns a 3 x 3 block of one
+// pair's result, so each LDS value it loads feeds three complex FMAs, and sums
+// it over the pairs it takes; the ppw partial sums are added in a fixed order
+// at the end.  X = Ghat G^H is held in registers, then overwrites Ghat in LDS
+// for the second product.  seg holds the segments' first pairs (indices of
+// the whole list) and the end of the last.
+__global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const double2 *__restrict__ Graw,
+                                                       const int *__restrict__ seg, int ppw, double2 *__restrict__ out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int d = a.d, dd = d * d, nb3 = d / 3, nblk3 = nb3 * nb3;
     const int tid = threadIdx.x, NT = blockDim.x;
-    const long pbase = (long)blockIdx.x * ppw;
-    // stage G and Ghat of the workgroup's pairs
-    for (int e = tid; e < ppw * dd; e += NT) {
-        const int s = e / dd, f = e - s * dd;
-        const long pl = pbase + s;
-        double2 *G = reinterpret_cast<double2 *>(smem) + (size_t)s * 2 * dd;
-        double2 *Gh = G + dd;
-        if (pl >= a.np) {
-            G[f] = make_double2(0.0, 0.0);
-            Gh[f] = make_double2(0.0, 0.0);
-            continue;
-        }
-        const long p = a.p0 + pl;
-        double w = 0.0;
-        if (a.mode == TH_G_IMAG_A) w = a.wim[p];
-        if (a.mode == TH_G_REAL_A) {
-            const long ie = p / a.nk_re, ik = p - ie * a.nk_re;
-            w = a.p2[ie] * a.wre[ik];
-        }
-        const double *H = a.hat + (size_t)pl * a.nA2 * 2 * a.Mpad;
-        G[f] = Graw[(size_t)pl * dd + f];
-        Gh[f] = ghat_entry(a, H, f / d, f - (f / d) * d, w);
-    }
-    __syncthreads();
+    const long q0 = seg[blockIdx.x] - a.p0, q1 = seg[blockIdx.x + 1] - a.p0;   // the batch's pairs
     const int s = tid / nblk3, bidx = tid - s * nblk3;
-    const bool active = s < ppw && pbase + s < a.np;
     const int r0 = 3 * (bidx / nb3), c0 = 3 * (bidx - (bidx / nb3) * nb3);
     double2 *G = reinterpret_cast<double2 *>(smem) + (size_t)(s < ppw ? s : 0) * 2 * dd;
     double2 *Gh = G + dd;
-    double2 acc[3][3];
-    // X = Ghat G^H: X[r][c] = sum_m Ghat[r][m] conj(G[c][m])
-    if (active) {
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
-        for (int m = 0; m < d; ++m) {
-            double2 x[3], y[3];
-            for (int i = 0; i < 3; ++i) x[i] = Gh[(r0 + i) * d + m];
-            for (int j = 0; j < 3; ++j) y[j] = G[(c0 + j) * d + m];
+    double2 tot[3][3], acc[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) tot[i][j] = make_double2(0.0, 0.0);
+    for (long g0 = q0; g0 < q1; g0 += ppw) {
+        __syncthreads();                         // the previous group's reads of G, Gh done
+        for (int e = tid; e < ppw * dd; e += NT) {
+            const int sl = e / dd, f = e - sl * dd;
+            const long pl = g0 + sl;
+            double2 *Gs = reinterpret_cast<double2 *>(smem) + (size_t)sl * 2 * dd;
+            if (pl >= q1) {
+                Gs[f] = make_double2(0.0, 0.0);
+                Gs[dd + f] = make_double2(0.0, 0.0);
+                continue;
+            }
+            const long p = a.p0 + pl;
+            double w = 0.0;
+            if (a.mode == TH_G_IMAG_A) w = a.wim[p];
+            if (a.mode == TH_G_REAL_A) {
+                const long ik = p / a.nx, ie = p - ik * a.nx;
+                w = a.p2[ie] * a.wre[ik];
+            }
+            const double *H = a.hat + (size_t)pl * a.nA2 * 2 * a.Mpad;
+            Gs[f] = Graw[(size_t)pl * dd + f];
+            Gs[dd + f] = ghat_entry(a, H, f / d, f - (f / d) * d, w);
+        }
+        __syncthreads();
+        const bool active = s < ppw && g0 + s < q1;
+        // X = Ghat G^H: X[r][c] = sum_m Ghat[r][m] conj(G[c][m])
+        if (active) {
             for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j)
-                    acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
-                                             fma(x[i].y, y[j].x, fma(-x[i].x, y[j].y, acc[i][j].y)));
+                for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
+            for (int m = 0; m < d; ++m) {
+                double2 x[3], y[3];
+                for (int i = 0; i < 3; ++i) x[i] = Gh[(r0 + i) * d + m];
+                for (int j = 0; j < 3; ++j) y[j] = G[(c0 + j) * d + m];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                                 fma(x[i].y, y[j].x, fma(-x[i].x, y[j].y, acc[i][j].y)));
+            }
+        }
+        __syncthreads();
+        if (active)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) Gh[(r0 + i) * d + c0 + j] = acc[i][j];
+        __syncthreads();
+        // Hhat = G^H X: Hhat[r][c] = sum_m conj(G[m][r]) X[m][c], added to the sum
+        if (active) {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
+            for (int m = 0; m < d; ++m) {
+                double2 x[3], y[3];
+                for (int i = 0; i < 3; ++i) x[i] = G[m * d + r0 + i];
+                for (int j = 0; j < 3; ++j) y[j] = Gh[m * d + c0 + j];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                                 fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y)));
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) tot[i][j] = zadd(tot[i][j], acc[i][j]);
         }
     }
     __syncthreads();
-    if (active)
+    // the ppw partial sums (in the staging space), added in order of s
+    double2 *red = reinterpret_cast<double2 *>(smem);
+    if (s < ppw)
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) Gh[(r0 + i) * d + c0 + j] = acc[i][j];
+            for (int j = 0; j < 3; ++j) red[(size_t)s * dd + (r0 + i) * d + c0 + j] = tot[i][j];
     __syncthreads();
-    if (!active) return;
-    // Hhat = G^H X: Hhat[r][c] = sum_m conj(G[m][r]) X[m][c]
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
-    for (int m = 0; m < d; ++m) {
-        double2 x[3], y[3];
-        for (int i = 0; i < 3; ++i) x[i] = G[m * d + r0 + i];
-        for (int j = 0; j < 3; ++j) y[j] = Gh[m * d + c0 + j];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
-                                         fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y)));
+    for (int f = tid; f < dd; f += NT) {
+        double2 v = red[f];
+        for (int t = 1; t < ppw; ++t) v = zadd(v, red[(size_t)t * dd + f]);
+        out[(size_t)blockIdx.x * dd + f] = v;
     }
-    double2 *out = a.out + (size_t)(pbase + s) * dd;
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) out[(r0 + i) * d + c0 + j] = acc[i][j];
+}
+
+// Hhat of the batch's pairs summed over each segment (the generic path)
+__global__ void seg_sum_kernel(const double2 *__restrict__ Hh, const int *__restrict__ seg, long p0, int nseg, int dd,
+                               double2 *__restrict__ out)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nseg * dd) return;
+    const int sg = (int)(e / dd), f = (int)(e - (size_t)sg * dd);
+    double2 v = make_double2(0.0, 0.0);
+    for (long pl = seg[sg] - p0; pl < seg[sg + 1] - p0; ++pl) v = zadd(v, Hh[(size_t)pl * dd + f]);
+    out[e] = v;
 }
 
 // raw G of the imaginary-axis B side from the omega-independent panel cache:
@@ -495,16 +532,17 @@ __global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, in
     raw[e] = make_double2((double)b[rr * bs + cc], (double)b[Mpad + rr * bs + cc]);
 }
 
-// exp(-i k_p.R_n) for the pairs of a batch: ph[pl * nhop + n]
-__global__ void phase_kernel(long p0, long np, int nhop, int imag, int side_q, long nk_re, double q0, double q1,
+// exp(-i k.R_n) for the segments of a batch (k of the segment's first pair,
+// seg[sg] in the whole list; real pairs k-major, nx nodes): ph[sg * nhop + n]
+__global__ void phase_kernel(const int *seg, int nseg, int nhop, int imag, int side_q, int nx, double q0, double q1,
                              double q2, const double *kim, const double *kre, const double *R, double2 *ph)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)np * nhop) return;
-    const long pl = (long)(e / nhop);
-    const int n = (int)(e - (size_t)pl * nhop);
-    const long p = p0 + pl;
-    const double *k = imag ? kim + 3 * p : kre + 3 * (p % nk_re);
+    if (e >= (size_t)nseg * nhop) return;
+    const int sg = (int)(e / nhop);
+    const int n = (int)(e - (size_t)sg * nhop);
+    const long p = seg[sg];
+    const double *k = imag ? kim + 3 * p : kre + 3 * (p / nx);
     double k0 = k[0], k1 = k[1], k2 = k[2];
     if (side_q) { k0 += q0; k1 += q1; k2 += q2; }
     double sn, cs;
@@ -767,7 +805,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     const int D = h->D, d = h->d, nO = h->nOrb;
     const size_t DD = (size_t)D * D, dd = (size_t)d * d;
     const int nout = h->nhop * nO * nO + (int)dd;
-    constexpr int CHUNK = 256;
+    constexpr int CHUNK = 32;     // segments per partial sum of the hop and h0 reductions
     // pairs per batch: adjoint buffers within twice the panel budget
     const size_t per = (size_t)2 * h->nA2 * 2 * h->Mpad * sizeof(double) + 5 * dd * sizeof(double2) +
                        (size_t)nout * sizeof(double2) / CHUNK + (size_t)h->nhop * sizeof(double2) +
@@ -793,6 +831,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     need += ((size_t)bp * std::max(1, h->nhop) * sizeof(double2) + 255) & ~(size_t)255;
     need += ((size_t)bp * h->nA2 * nO * nO * sizeof(double2) + 255) & ~(size_t)255;
     need += ((size_t)3 * (h->nhop + h->nA2) * sizeof(int) + 255) & ~(size_t)255;   // hop groups
+    need += ((size_t)(h->n_im + n_re + 2) * sizeof(int) + 255) & ~(size_t)255;     // segments of both axes
     need += 4096;
     if (need > h->adj_bytes) {
         hipFree(h->d_adj);
@@ -818,6 +857,28 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     double2 *d_ph = ar.take<double2>((size_t)bp * std::max(1, h->nhop));
     double2 *d_S = ar.take<double2>((size_t)bp * h->nA2 * nO * nO);
     double2 *d_Uhat = U_hat ? ar.take<double2>(DD) : nullptr;
+    int *d_seg = ar.take<int>((size_t)(h->n_im + n_re + 2));
+
+    // segments: runs of consecutive pairs on one k-point within a batch (the
+    // stored imaginary list keeps a k-point's pairs together in runs; real
+    // pairs are taken k-major, n_x per k-point); their first pairs (indices of
+    // the whole list) and the end of the last, axis 0 then axis 1
+    std::vector<int> segv[2];
+    for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
+        const long npairs = axis == 0 ? h->n_im : n_re;
+        std::vector<int> &sv = segv[axis];
+        sv.reserve((size_t)npairs / 8 + 2);
+        for (long p = 0; p < npairs; ++p) {
+            const bool brk = p % bp == 0 || (axis == 0 ? h->kid_im_local[p] != h->kid_im_local[p - 1] : p % n_x == 0);
+            if (brk) sv.push_back((int)p);
+        }
+        sv.push_back((int)npairs);
+    }
+    const size_t seg1 = segv[0].size();
+    TH_HIP(hipMemcpyAsync(d_seg, segv[0].data(), sizeof(int) * seg1, hipMemcpyHostToDevice, h->stream));
+    if (real_axis)
+        TH_HIP(hipMemcpyAsync(d_seg + seg1, segv[1].data(), sizeof(int) * segv[1].size(), hipMemcpyHostToDevice,
+                              h->stream));
 
     // hop groups: at most HG hops of one atom pair (stored hops are grouped by pair)
     std::vector<int> hoff(h->nA2 + 1), grp;
@@ -872,11 +933,22 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
         const long npairs = axis == 0 ? h->n_im : n_re;
         const double *That = h->d_acc + (size_t)axis * accsz;
+        const std::vector<int> &sv = segv[axis];
+        const int *dsv = d_seg + (axis == 0 ? 0 : seg1);
+        size_t sb0 = 0;   // first segment of the batch
         for (long p0 = 0; p0 < npairs; p0 += bp) {
             const long np = std::min(bp, npairs - p0);
+            size_t sb1 = sb0;
+            while (sv[sb1] < p0 + np) ++sb1;   // segments never straddle a batch
+            const int nseg = (int)(sb1 - sb0);
+            const int *dseg = dsv + sb0;
             th_green_args a{};
             a.p0 = p0; a.np = np; a.q[0] = q[0]; a.q[1] = q[1]; a.q[2] = q[2];
             a.omega = omega; a.Ef = Ef; a.eta = eta; a.etap = etap; a.prec = precision;
+            if (axis == 1) {   // real pairs k-major: a k-point's nodes in one segment
+                a.kmajor_re = 1;
+                a.nx = n_x;
+            }
             id = tm.begin(0);
             a.mode = axis == 0 ? TH_G_IMAG_A : TH_G_REAL_A;
             a.panel = h->d_pA;
@@ -921,22 +993,25 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 assemble_params ap{};
                 ap.mode = axis == 0 ? (side == 0 ? TH_G_IMAG_A : TH_G_IMAG_B) : (side == 0 ? TH_G_REAL_A : TH_G_REAL_B);
                 ap.nA = h->nA; ap.bs = h->bs; ap.d = d; ap.Mpad = h->Mpad; ap.nA2 = h->nA2;
-                ap.p0 = p0; ap.np = np; ap.nk_re = h->nk_re;
+                ap.p0 = p0; ap.np = np; ap.nk_re = h->nk_re; ap.nx = n_x;
                 ap.wim = h->d_wim; ap.p2 = h->d_p2; ap.wre = h->d_wre;
                 ap.hat = side == 0 ? hatA : hatB;
                 double2 *gh = side == 0 ? gA : gB;
                 const double2 *G = side == 0 ? rawA : rawB;
                 ap.out = gh;
+                // Hhat = G^H Ghat G^H per pair, summed over each segment
+                const double2 *Hseg = gh;
                 if (d <= 54 && d % 3 == 0) {
-                    // Hhat = G^H Ghat G^H with the pairs' G, Ghat in LDS, 3 x 3 register blocks
+                    // the pairs' G, Ghat in LDS, 3 x 3 register blocks, one workgroup per segment
                     const int nblk3 = (d / 3) * (d / 3);
-                    const int ppw = std::max(1, 256 / nblk3);
+                    // at most ~42 KB of LDS (three workgroups per CU)
+                    const int ppw = std::max(1, std::min(42 * 1024 / (int)(2 * dd * sizeof(double2)), 320 / nblk3));
                     const int nt = ((ppw * nblk3 + 63) / 64) * 64;
                     const size_t lds = (size_t)ppw * 2 * dd * sizeof(double2);
-                    TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(hhat_fused_kernel),
+                    TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(hhat_seg_kernel),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    hipLaunchKernelGGL(hhat_fused_kernel, dim3((unsigned)((np + ppw - 1) / ppw)), dim3(nt), lds,
-                                       h->stream, ap, G, ppw);
+                    hipLaunchKernelGGL(hhat_seg_kernel, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, G, dseg,
+                                       ppw, gh);
                     TH_HIP(hipGetLastError());
                 } else {
                     hipLaunchKernelGGL(assemble_kernel, dim3(nblk((size_t)np * dd, 256)), dim3(256), 0, h->stream, ap);
@@ -947,28 +1022,31 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                                        d, ntile);
                     hipLaunchKernelGGL(bmm_kernel, dim3(nwg), dim3(256), 0, h->stream, G, 1, (const double2 *)tmp, 0, gh,
                                        d, ntile);
+                    hipLaunchKernelGGL(seg_sum_kernel, dim3(nblk((size_t)nseg * dd, 256)), dim3(256), 0, h->stream,
+                                       (const double2 *)gh, dseg, p0, nseg, (int)dd, tmp);
                     TH_HIP(hipGetLastError());
+                    Hseg = tmp;
                 }
-                hipLaunchKernelGGL(phase_kernel, dim3(nblk((size_t)np * std::max(1, h->nhop), 256)), dim3(256), 0,
-                                   h->stream, p0, np, h->nhop, axis == 0 ? 1 : 0, side == 0 ? 1 : 0, h->nk_re, q[0], q[1],
+                hipLaunchKernelGGL(phase_kernel, dim3(nblk((size_t)nseg * std::max(1, h->nhop), 256)), dim3(256), 0,
+                                   h->stream, dseg, nseg, h->nhop, axis == 0 ? 1 : 0, side == 0 ? 1 : 0, n_x, q[0], q[1],
                                    q[2], (const double *)h->d_kim, (const double *)h->d_kre, (const double *)h->d_R,
                                    d_ph);
                 TH_HIP(hipGetLastError());
                 hreduce_params hp{};
                 hp.side_q = side == 0 ? 1 : 0;
                 hp.imag = axis == 0 ? 1 : 0;
-                hp.p0 = p0; hp.np = np; hp.nk_re = h->nk_re;
+                hp.p0 = p0; hp.np = nseg; hp.nk_re = h->nk_re;
                 hp.chunk = CHUNK; hp.nout = nout; hp.nhop = h->nhop; hp.nOrb = nO; hp.d = d;
                 hp.q0 = q[0]; hp.q1 = q[1]; hp.q2 = q[2];
                 hp.kim = h->d_kim; hp.kre = h->d_kre; hp.R = h->d_R; hp.jb = h->d_jb; hp.ib = h->d_ib;
-                hp.Hh = gh;
+                hp.Hh = Hseg;
                 hp.nA = h->nA;
                 hp.S = d_S;
                 hp.ph = d_ph;
                 hp.partial = d_part;
-                hipLaunchKernelGGL(spin_sum_kernel, dim3(nblk((size_t)np * h->nA2 * nO * nO, 256)), dim3(256), 0,
-                                   h->stream, np, h->nA, nO, (const double2 *)gh, d_S);
-                const int nch = (int)((np + CHUNK - 1) / CHUNK);
+                hipLaunchKernelGGL(spin_sum_kernel, dim3(nblk((size_t)nseg * h->nA2 * nO * nO, 256)), dim3(256), 0,
+                                   h->stream, (long)nseg, h->nA, nO, Hseg, d_S);
+                const int nch = (nseg + CHUNK - 1) / CHUNK;
                 if (ngroups > 0)
                     hipLaunchKernelGGL(hop_reduce_kernel, dim3(nblk((size_t)ngroups * nO * nO, 64), nch), dim3(256), 0,
                                        h->stream, hp, (const int *)d_groups, ngroups);
@@ -978,6 +1056,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 TH_HIP(hipGetLastError());
             }
             tm.end(id);
+            sb0 = sb1;
         }
     }
     tm.end(idall);
