@@ -935,6 +935,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
         const double *That = h->d_acc + (size_t)axis * accsz;
         const std::vector<int> &sv = segv[axis];
         const int *dsv = d_seg + (axis == 0 ? 0 : seg1);
+        if ((rc = th_panel_adjoint_prepare(h, precision, That))) return rc;
         size_t sb0 = 0;   // first segment of the batch
         for (long p0 = 0; p0 < npairs; p0 += bp) {
             const long np = std::min(bp, npairs - p0);

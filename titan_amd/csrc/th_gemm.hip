@@ -903,6 +903,227 @@ __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict_
         }
 }
 
+// Largest |x| of a buffer as the bits of a non-negative float (atomicMax on
+// the bits: they order as the values); split words count as (hi + lo) u.
+__global__ __launch_bounds__(256) void absmax_f_kernel(const float *__restrict__ x, size_t n, float u,
+                                                       unsigned *__restrict__ mx)
+{
+    float m = 0.0f;
+    auto val = [u](float v) {
+        if (u != 0.0f) {
+            const unsigned w = __builtin_bit_cast(unsigned, v);
+            v = ((float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)) +
+                 (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16))) * u;
+        }
+        return fabsf(v);
+    };
+    // 16-byte loads (n is a multiple of 4: whole panel rows of 2 Mpad floats)
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n / 4; e += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = x4[e];
+        m = fmaxf(m, fmaxf(fmaxf(val(v.x), val(v.y)), fmaxf(val(v.z), val(v.w))));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(mx, __builtin_bit_cast(unsigned, m));
+}
+
+__global__ __launch_bounds__(256) void absmax_d_kernel(const double *__restrict__ x, size_t n, unsigned *__restrict__ mx)
+{
+    float m = 0.0f;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, (float)fabs(x[e]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(mx, __builtin_bit_cast(unsigned, m));
+}
+
+__global__ void zero_u_kernel(unsigned *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// power of two 2^e with |x| 2^e < 2^14 for |x| <= the maximum held in bits
+__device__ inline int split_exp(unsigned bits)
+{
+    const float m = __builtin_bit_cast(float, bits);
+    if (!(m > 0.0f) || !isfinite(m)) return 0;
+    return min(100, max(-100, 13 - ilogbf(m)));
+}
+
+// That of one axis split into fp16 planes in the layout the fp16 panel
+// adjoint's B operand reads, Ys[ij][re|im][hi|lo][n][k] with k padded to Kp
+// (zeros), Y[k][n] = That[n][k] (trans = 1) or That[k][n] (trans = 0), under
+// the power of two of the largest |That| (mx[1])
+__global__ __launch_bounds__(256) void presplit_y_kernel(const double *__restrict__ That, int Mpad, int Kp, int nA2,
+                                                         int trans, const unsigned *__restrict__ mx,
+                                                         _Float16 *__restrict__ Ys)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nk = (size_t)Mpad * Kp;
+    if (e >= (size_t)nA2 * 2 * nk) return;
+    const size_t blk = e / nk;                  // ij * 2 + comp
+    const int nkk = (int)(e - blk * nk), n = nkk / Kp, k = nkk - n * Kp;
+    const size_t MM = (size_t)Mpad * Mpad;
+    float v = 0.0f;
+    if (k < Mpad) {
+        const double y = trans ? That[blk * MM + (size_t)n * Mpad + k] : That[blk * MM + (size_t)k * Mpad + n];
+        v = (float)(y * ldexp(1.0, split_exp(mx[1])));
+    }
+    const _Float16 hi = (_Float16)v;
+    _Float16 *o = Ys + blk * 2 * nk + nkk;
+    o[0] = hi;
+    o[nk] = (_Float16)fmaf(v, 1.0f, -(float)hi);
+}
+
+// The adjoint's GEMM (out[p][n] = sum_k conj(X[p][k]) Y[k][n], as
+// panel_adjoint_kernel) on the fp16 matrix cores for the mixed path: X (the
+// batch's FP32 panels, or split words read as (hi + lo) u) and Y (That in
+// FP64) are split into fp16 hi + lo under power-of-two scales from their
+// largest entries (mx[0], mx[1], set by the absmax kernels; split words keep
+// their exponent, ex_fixed), conj(X) by its
+// negated imaginary part, and each real product is hi.hi + hi.lo + lo.hi in
+// FP32 accumulators (Ar Br and Ai Bi apart).  A workgroup owns 112 pairs x 112
+// columns of one atom pair, seven waves of 16 pairs x 7 column blocks; K (the
+// panel index) runs in stages of 32 through LDS, each thread splitting groups
+// of 8 consecutive k into one 16-byte LDS store per part; loads use clamped
+// indices (no branch per load) and zero what lies outside.
+constexpr int PAK = 32, PAST = PAK + 8;   // K per stage, LDS row stride in halves
+__global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__restrict__ X,
+                                                               const _Float16 *__restrict__ Ys, int Kp, long np, int Mpad,
+                                                               int nA2, int ntile, int trans, double *__restrict__ out,
+                                                               th_rowmap rows, float unsplit, int ex_fixed,
+                                                               const unsigned *__restrict__ mx)
+{
+    typedef _Float16 hv8_t __attribute__((ext_vector_type(8)));
+    __shared__ __align__(16) _Float16 Lx[2][2][TH_TILE * PAST];   // [re|im][hi|lo][pair][k]
+    __shared__ __align__(16) _Float16 Ly[2][2][TH_TILE * PAST];   // [re|im][hi|lo][n][k]
+    const long nptile = (np + TH_TILE - 1) / TH_TILE;
+    const long b = blockIdx.x;
+    const int tn = (int)(b % ntile);
+    const long rest = b / ntile;
+    const long tp = rest % nptile;
+    const int ij = (int)(rest / nptile);
+    const long p0 = tp * TH_TILE;
+    const int n0 = tn * TH_TILE;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const size_t pstride = (size_t)nA2 * 2 * Mpad;
+    // split words (unsplit != 0) keep their own exponent: (hi + lo) 2^e splits
+    // back into the same two parts
+    const int eX = ex_fixed != INT_MIN ? ex_fixed : split_exp(mx[0]), eY = split_exp(mx[1]);
+    const float sx = ldexpf(1.0f, eX);
+    const float *Xb = X + (size_t)ij * 2 * Mpad;
+    const size_t nk = (size_t)Mpad * Kp;
+    const _Float16 *Yb = Ys + (size_t)ij * 4 * nk;   // [re|im][hi|lo][n][k] of this atom pair
+    // staging: 112 rows x 2 components x 4 groups of 8 k = 896 groups per
+    // operand, two per thread
+    auto split8 = [](const float (&v)[8], hv8_t &hi, hv8_t &lo) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const _Float16 h = (_Float16)v[i];
+            hi[i] = h;
+            lo[i] = (_Float16)fmaf(v[i], 1.0f, -(float)h);
+        }
+    };
+    fv4 ap[7], an[7], ai[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+        for (int q = 0; q < 4; ++q) { ap[j][q] = 0.0f; an[j][q] = 0.0f; ai[j][q] = 0.0f; }
+    long xrow[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int g = tid + 448 * u, m = g % TH_TILE;
+        const long pp = p0 + m < np ? p0 + m : np - 1;
+        xrow[u] = th_map_row(rows.pg0 + pp, rows);
+    }
+    // the next stage's X and Y in registers while the current one's products
+    // run: X as two 16-byte loads per group of 8 k (Mpad is a multiple of 8, so
+    // a group lies wholly inside or wholly past the panel row; past it the
+    // load reads the row's last group and the values are zeroed)
+    float4 xq[2][2];
+    hv8_t yq[2][2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int g = tid + 448 * u, m = g % TH_TILE, r = g / TH_TILE, comp = r >> 2, kg = 8 * (r & 3);
+            const int ks = k0 + kg < Mpad ? k0 + kg : Mpad - 8;
+            const float4 *xp = reinterpret_cast<const float4 *>(Xb + (size_t)xrow[u] * pstride + (size_t)comp * Mpad + ks);
+            xq[u][0] = xp[0];
+            xq[u][1] = xp[1];
+            const size_t yo = (size_t)comp * 2 * nk + (size_t)(n0 + m) * Kp + k0 + kg;
+            yq[u][0] = *reinterpret_cast<const hv8_t *>(Yb + yo);
+            yq[u][1] = *reinterpret_cast<const hv8_t *>(Yb + yo + nk);
+        }
+    };
+    gload(0);
+    for (int k0 = 0; k0 < Mpad; k0 += PAK) {
+        __syncthreads();                          // the previous stage's reads done
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int g = tid + 448 * u, m = g % TH_TILE, r = g / TH_TILE, comp = r >> 2, kg = 8 * (r & 3);
+            const bool in = p0 + m < np && k0 + kg < Mpad;
+            const float xs[8] = {xq[u][0].x, xq[u][0].y, xq[u][0].z, xq[u][0].w,
+                                 xq[u][1].x, xq[u][1].y, xq[u][1].z, xq[u][1].w};
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                float x = xs[i];
+                if (unsplit != 0.0f) {
+                    const unsigned w = __builtin_bit_cast(unsigned, x);
+                    x = ((float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)) +
+                         (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16))) * unsplit;
+                }
+                v[i] = in ? (comp ? -x : x) * sx : 0.0f;   // conj(X)
+            }
+            hv8_t hi, lo;
+            split8(v, hi, lo);
+            *reinterpret_cast<hv8_t *>(&Lx[comp][0][m * PAST + kg]) = hi;
+            *reinterpret_cast<hv8_t *>(&Lx[comp][1][m * PAST + kg]) = lo;
+            *reinterpret_cast<hv8_t *>(&Ly[comp][0][m * PAST + kg]) = yq[u][0];
+            *reinterpret_cast<hv8_t *>(&Ly[comp][1][m * PAST + kg]) = yq[u][1];
+        }
+        // the next stage's loads (the last stage re-reads in-range data)
+        gload(k0 + PAK < Mpad ? k0 + PAK : k0);
+        __syncthreads();
+        const int ao = (16 * wave + (lane & 15)) * PAST + 8 * (lane >> 4);
+        const hv8_t arh = *reinterpret_cast<const hv8_t *>(&Lx[0][0][ao]);
+        const hv8_t arl = *reinterpret_cast<const hv8_t *>(&Lx[0][1][ao]);
+        const hv8_t aih = *reinterpret_cast<const hv8_t *>(&Lx[1][0][ao]);
+        const hv8_t ail = *reinterpret_cast<const hv8_t *>(&Lx[1][1][ao]);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const int bo = (16 * j + (lane & 15)) * PAST + 8 * (lane >> 4);
+            const hv8_t brh = *reinterpret_cast<const hv8_t *>(&Ly[0][0][bo]);
+            const hv8_t brl = *reinterpret_cast<const hv8_t *>(&Ly[0][1][bo]);
+            const hv8_t bih = *reinterpret_cast<const hv8_t *>(&Ly[1][0][bo]);
+            const hv8_t bil = *reinterpret_cast<const hv8_t *>(&Ly[1][1][bo]);
+            ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh, ap[j], 0, 0, 0);
+            an[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih, an[j], 0, 0, 0);
+            ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih, ai[j], 0, 0, 0);
+            ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl, ap[j], 0, 0, 0);
+            an[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil, an[j], 0, 0, 0);
+            ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bil, ai[j], 0, 0, 0);
+            ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh, ap[j], 0, 0, 0);
+            an[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih, an[j], 0, 0, 0);
+            ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bih, ai[j], 0, 0, 0);
+            ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh, ai[j], 0, 0, 0);
+            ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl, ai[j], 0, 0, 0);
+            ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh, ai[j], 0, 0, 0);
+        }
+    }
+    const double back = ldexp(1.0, -(eX + eY));
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long pp = p0 + wave * 16 + 4 * (lane >> 4) + q;
+            const int col = n0 + j * 16 + (lane & 15);
+            if (pp < np) {
+                double *o = out + (size_t)pp * pstride + (size_t)ij * 2 * Mpad;
+                o[col] = ((double)ap[j][q] - (double)an[j][q]) * back;
+                o[Mpad + col] = (double)ai[j][q] * back;
+            }
+        }
+}
+
+
 // acc += sum_s part[s], elementwise, fixed order
 __global__ void reduce_kernel(const double *__restrict__ part, double *__restrict__ acc, int splits, size_t n)
 {
@@ -1071,6 +1292,36 @@ int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const dou
     const long nptile = (np + TH_TILE - 1) / TH_TILE;
     const long nwg = nptile * ntile * h->nA2;
     const dim3 grid((unsigned)nwg), blk(NTH);
+    if (prec != TH_PREC_FP64 && h->contract_f16) {
+        // fp16 matrix cores: That split by th_panel_adjoint_prepare, X scaled by
+        // its largest entry (FP32 panels of the batch) or its split exponent
+        if (h->pa_that != That) {
+            th_set_error("panel adjoint: That was not prepared (th_panel_adjoint_prepare)");
+            return TH_E_STATE;
+        }
+        unsigned *mx = static_cast<unsigned *>(h->d_pamax);
+        hipLaunchKernelGGL(zero_u_kernel, dim3(1), dim3(64), 0, h->stream, mx, 1);
+        const size_t ps = (size_t)h->nA2 * 2 * h->Mpad;
+        if (split_e == INT_MIN) {
+            // the rows the batch reads: pg0 .. pg0 + np - 1, or through a row map
+            // the whole cache batches that hold them (a wider set, still a bound)
+            long r0 = rows.pg0, r1 = rows.pg0 + np;
+            if (rows.bpc > 0) {
+                r0 = (rows.pg0 / rows.bpc) * rows.bpc;
+                r1 = std::min(rows.ntot, ((rows.pg0 + np + rows.bpc - 1) / rows.bpc) * rows.bpc);
+            }
+            hipLaunchKernelGGL(absmax_f_kernel, dim3(1024), dim3(256), 0, h->stream,
+                               (const float *)panel + (size_t)r0 * ps, (size_t)(r1 - r0) * ps, 0.0f, mx);
+        }
+        const int Kp = ((h->Mpad + PAK - 1) / PAK) * PAK;
+        const size_t ny = (size_t)h->nA2 * 2 * h->Mpad * Kp;
+        const _Float16 *Ys = static_cast<const _Float16 *>(h->d_pays) + (size_t)trans * ny * 2;
+        hipLaunchKernelGGL(panel_adjoint_h2_kernel, grid, dim3(448), 0, h->stream, (const float *)panel,
+                           (const _Float16 *)Ys, Kp, np, h->Mpad, h->nA2, ntile, trans, out, rows, unsplit, split_e,
+                           (const unsigned *)mx);
+        TH_HIP(hipGetLastError());
+        return TH_OK;
+    }
     if (prec == TH_PREC_FP64) {
         if (h->contract_3m)
             hipLaunchKernelGGL((panel_adjoint_kernel<double, true>), grid, blk, 0, h->stream, (const double *)panel,
@@ -1087,5 +1338,25 @@ int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const dou
                                That, np, h->Mpad, h->nA2, ntile, trans, out, rows, unsplit);
     }
     TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+int th_panel_adjoint_prepare(th_handle *h, int prec, const double *That)
+{
+    if (prec == TH_PREC_FP64 || !h->contract_f16) return TH_OK;
+    if (!h->d_pamax) TH_HIP(hipMalloc(&h->d_pamax, 2 * sizeof(unsigned)));
+    const int Kp = ((h->Mpad + PAK - 1) / PAK) * PAK;
+    const size_t ny = (size_t)h->nA2 * 2 * h->Mpad * Kp;
+    if (!h->d_pays) TH_HIP(hipMalloc(&h->d_pays, 2 * ny * 2 * sizeof(_Float16)));
+    unsigned *mx = static_cast<unsigned *>(h->d_pamax);
+    hipLaunchKernelGGL(zero_u_kernel, dim3(1), dim3(64), 0, h->stream, mx, 2);
+    hipLaunchKernelGGL(absmax_d_kernel, dim3(1024), dim3(256), 0, h->stream, That,
+                       (size_t)h->nA2 * 2 * h->Mpad * h->Mpad, mx + 1);
+    for (int trans = 0; trans < 2; ++trans)
+        hipLaunchKernelGGL(presplit_y_kernel, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, h->stream, That,
+                           h->Mpad, Kp, h->nA2, trans, (const unsigned *)mx,
+                           static_cast<_Float16 *>(h->d_pays) + (size_t)trans * ny * 2);
+    TH_HIP(hipGetLastError());
+    h->pa_that = That;
     return TH_OK;
 }

@@ -121,6 +121,9 @@ struct th_handle {
     // adjoint work space and the caller's hop order (hops are stored sorted)
     std::vector<int> hop_order;  // stored hop n came from caller's entry hop_order[n]
     void *d_adj = nullptr;
+    void *d_pamax = nullptr;     // two words: the largest |X| and |That| of the fp16 panel adjoint
+    void *d_pays = nullptr;      // That of the fp16 panel adjoint split into fp16 planes (both transpositions)
+    const double *pa_that = nullptr;   // the That d_pays holds
     size_t adj_bytes = 0;
     void *d_blk = nullptr;       // work space of the block elimination (d = 288)
     size_t blk_bytes = 0;
@@ -304,6 +307,11 @@ int th_launch_finalize(th_handle *h, double2 *chi0);
 // split_e != INT_MIN: the panel holds split (hi, lo) words of x 2^split_e (FP32 only)
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
                             double *out, const th_rowmap &rows = th_rowmap(), int split_e = INT_MIN);
+// For the fp16 panel adjoint (mixed precision): That of one axis split once
+// into the fp16 layouts of both transpositions before the axis's batches
+// (its largest |That| kept on the device); later launches with this That use
+// them.  FP64, or contract_f16 off: nothing to do.
+int th_panel_adjoint_prepare(th_handle *h, int prec, const double *That);
 
 // ---- FP64 chi0 on the support of U (th_refine.hip) ----
 constexpr int TH_REFINE_MAX_S = 1024;   // larger supports are not refined
