@@ -402,19 +402,28 @@ __device__ inline double2 ghat_entry(const assemble_params &a, const double *H, 
 // it over the pairs it takes; the ppw partial sums are added in a fixed order
 // at the end.  X = Ghat G^H is held in registers, then overwrites Ghat in LDS
 // for the second product.  seg holds the segments' first pairs (indices of
-// the whole list) and the end of the last.
+// the whole list) and the end of the last.  The products run in R (FP32 for
+// the mixed path, at twice the FP64 rate and half the LDS traffic), the
+// segment sums in FP64.
+template <typename R> struct c2of;
+template <> struct c2of<double> { typedef double2 T; };
+template <> struct c2of<float> { typedef float2 T; };
+
+template <typename R>
 __global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const double2 *__restrict__ Graw,
                                                        const int *__restrict__ seg, int ppw, double2 *__restrict__ out)
 {
+    typedef typename c2of<R>::T c2;
     extern __shared__ __align__(16) unsigned char smem[];
     const int d = a.d, dd = d * d, nb3 = d / 3, nblk3 = nb3 * nb3;
     const int tid = threadIdx.x, NT = blockDim.x;
     const long q0 = seg[blockIdx.x] - a.p0, q1 = seg[blockIdx.x + 1] - a.p0;   // the batch's pairs
     const int s = tid / nblk3, bidx = tid - s * nblk3;
     const int r0 = 3 * (bidx / nb3), c0 = 3 * (bidx - (bidx / nb3) * nb3);
-    double2 *G = reinterpret_cast<double2 *>(smem) + (size_t)(s < ppw ? s : 0) * 2 * dd;
-    double2 *Gh = G + dd;
-    double2 tot[3][3], acc[3][3];
+    c2 *G = reinterpret_cast<c2 *>(smem) + (size_t)(s < ppw ? s : 0) * 2 * dd;
+    c2 *Gh = G + dd;
+    double2 tot[3][3];
+    c2 acc[3][3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) tot[i][j] = make_double2(0.0, 0.0);
     for (long g0 = q0; g0 < q1; g0 += ppw) {
@@ -422,10 +431,10 @@ __global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const 
         for (int e = tid; e < ppw * dd; e += NT) {
             const int sl = e / dd, f = e - sl * dd;
             const long pl = g0 + sl;
-            double2 *Gs = reinterpret_cast<double2 *>(smem) + (size_t)sl * 2 * dd;
+            c2 *Gs = reinterpret_cast<c2 *>(smem) + (size_t)sl * 2 * dd;
             if (pl >= q1) {
-                Gs[f] = make_double2(0.0, 0.0);
-                Gs[dd + f] = make_double2(0.0, 0.0);
+                Gs[f] = c2{R(0), R(0)};
+                Gs[dd + f] = c2{R(0), R(0)};
                 continue;
             }
             const long p = a.p0 + pl;
@@ -436,23 +445,25 @@ __global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const 
                 w = a.p2[ie] * a.wre[ik];
             }
             const double *H = a.hat + (size_t)pl * a.nA2 * 2 * a.Mpad;
-            Gs[f] = Graw[(size_t)pl * dd + f];
-            Gs[dd + f] = ghat_entry(a, H, f / d, f - (f / d) * d, w);
+            const double2 gv = Graw[(size_t)pl * dd + f];
+            const double2 hv = ghat_entry(a, H, f / d, f - (f / d) * d, w);
+            Gs[f] = c2{(R)gv.x, (R)gv.y};
+            Gs[dd + f] = c2{(R)hv.x, (R)hv.y};
         }
         __syncthreads();
         const bool active = s < ppw && g0 + s < q1;
         // X = Ghat G^H: X[r][c] = sum_m Ghat[r][m] conj(G[c][m])
         if (active) {
             for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
+                for (int j = 0; j < 3; ++j) acc[i][j] = c2{R(0), R(0)};
             for (int m = 0; m < d; ++m) {
-                double2 x[3], y[3];
+                c2 x[3], y[3];
                 for (int i = 0; i < 3; ++i) x[i] = Gh[(r0 + i) * d + m];
                 for (int j = 0; j < 3; ++j) y[j] = G[(c0 + j) * d + m];
                 for (int i = 0; i < 3; ++i)
                     for (int j = 0; j < 3; ++j)
-                        acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
-                                                 fma(x[i].y, y[j].x, fma(-x[i].x, y[j].y, acc[i][j].y)));
+                        acc[i][j] = c2{fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                       fma(x[i].y, y[j].x, fma(-x[i].x, y[j].y, acc[i][j].y))};
             }
         }
         __syncthreads();
@@ -463,22 +474,24 @@ __global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const 
         // Hhat = G^H X: Hhat[r][c] = sum_m conj(G[m][r]) X[m][c], added to the sum
         if (active) {
             for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) acc[i][j] = make_double2(0.0, 0.0);
+                for (int j = 0; j < 3; ++j) acc[i][j] = c2{R(0), R(0)};
             for (int m = 0; m < d; ++m) {
-                double2 x[3], y[3];
+                c2 x[3], y[3];
                 for (int i = 0; i < 3; ++i) x[i] = G[m * d + r0 + i];
                 for (int j = 0; j < 3; ++j) y[j] = Gh[m * d + c0 + j];
                 for (int i = 0; i < 3; ++i)
                     for (int j = 0; j < 3; ++j)
-                        acc[i][j] = make_double2(fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
-                                                 fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y)));
+                        acc[i][j] = c2{fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                       fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y))};
             }
             for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) tot[i][j] = zadd(tot[i][j], acc[i][j]);
+                for (int j = 0; j < 3; ++j)
+                    tot[i][j] = make_double2(tot[i][j].x + (double)acc[i][j].x, tot[i][j].y + (double)acc[i][j].y);
         }
     }
     __syncthreads();
-    // the ppw partial sums (in the staging space), added in order of s
+    // the ppw partial sums (in the staging space: ppw dd FP64 values fit in the
+    // ppw 2 dd staged ones of either type), added in order of s
     double2 *red = reinterpret_cast<double2 *>(smem);
     if (s < ppw)
         for (int i = 0; i < 3; ++i)
@@ -1005,14 +1018,22 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 if (d <= 54 && d % 3 == 0) {
                     // the pairs' G, Ghat in LDS, 3 x 3 register blocks, one workgroup per segment
                     const int nblk3 = (d / 3) * (d / 3);
-                    // at most ~42 KB of LDS (three workgroups per CU)
-                    const int ppw = std::max(1, std::min(42 * 1024 / (int)(2 * dd * sizeof(double2)), 320 / nblk3));
+                    // at most ~42 KB of LDS (three workgroups per CU); FP32 products
+                    // for the mixed path
+                    const bool f32 = precision != TH_PREC_FP64;
+                    const size_t esz = f32 ? sizeof(float2) : sizeof(double2);
+                    const int ppw = std::max(1, std::min(42 * 1024 / (int)(2 * dd * esz), 320 / nblk3));
                     const int nt = ((ppw * nblk3 + 63) / 64) * 64;
-                    const size_t lds = (size_t)ppw * 2 * dd * sizeof(double2);
-                    TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(hhat_seg_kernel),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    hipLaunchKernelGGL(hhat_seg_kernel, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, G, dseg,
-                                       ppw, gh);
+                    const size_t lds = std::max((size_t)ppw * 2 * dd * esz, (size_t)ppw * dd * sizeof(double2));
+                    const void *kfn = f32 ? reinterpret_cast<const void *>(hhat_seg_kernel<float>)
+                                          : reinterpret_cast<const void *>(hhat_seg_kernel<double>);
+                    TH_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    if (f32)
+                        hipLaunchKernelGGL(hhat_seg_kernel<float>, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, G,
+                                           dseg, ppw, gh);
+                    else
+                        hipLaunchKernelGGL(hhat_seg_kernel<double>, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap,
+                                           G, dseg, ppw, gh);
                     TH_HIP(hipGetLastError());
                 } else {
                     hipLaunchKernelGGL(assemble_kernel, dim3(nblk((size_t)np * dd, 256)), dim3(256), 0, h->stream, ap);
