@@ -150,8 +150,9 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
  * not read.  The Green functions are recomputed, not stored for the
  * backward (the omega-independent imaginary-axis ones are read from the
  * forward's panel cache when it holds them); 'precision' selects the panel
- * precision of the two adjoint contractions as in th_chi0 (the adjoint of H
- * and every sum are FP64). */
+ * precision of the two adjoint contractions as in th_chi0 (TH_PREC_MIXED
+ * with "contract_f16": fp16 matrix cores on two-term splits; the products
+ * G^H Ghat G^H in FP32) while every sum is FP64. */
 int th_chi_adjoint(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
                    int n_x, const double *x2, const double *p2, int precision,
                    const double *chi0, const double *U, const double *chi, const double *chi_hat,
@@ -238,16 +239,18 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *   "green_variant"  inversion kernel shape: 0 = 2-D register-blocked (default
  *                    tiles), 1 = 1-D ownership, 2 = one wave per matrix,
  *                    3 = two waves per matrix (2 and 3 where compiled for d)
- *   "spectral"       1 = where H(k) is Hermitian and d is 18, 36 or 54, each
- *                    distinct k is diagonalised once (Jacobi, H = V
- *                    diag(lambda) V^H; at the first frequency of every
- *                    th_chi_sweep, th_chi0 keeps the eigenpairs of earlier
- *                    calls at the same q) and G(k,z) = V diag(1/(z-lambda))
- *                    V^H is formed on the matrix cores for every z (FP64
- *                    MFMA; in TH_PREC_MIXED fp16 MFMA on two-term splits with
- *                    "green_f16" = 1, the default, else FP32 MFMA)
- *                    (default); 0 = every
- *                    G by Gauss-Jordan elimination of z-H
+ *   "spectral"       1 = where H(k) is Hermitian and d is 18, 36, 54, 72, 144
+ *                    or 288, each distinct k is diagonalised once (Jacobi in
+ *                    LDS for d <= 54, the batched dense solver above; at the
+ *                    first frequency of every th_chi_sweep, th_chi0 keeps the
+ *                    eigenpairs of earlier calls at the same q; for d <= 144
+ *                    V is also kept as split fp16 planes for the fp16
+ *                    kernels) and G(k,z) = V diag(1/(z-lambda)) V^H is formed
+ *                    on the matrix cores for every z (FP64 MFMA; in
+ *                    TH_PREC_MIXED fp16 MFMA on two-term splits with
+ *                    "green_f16" = 1, the default, for d <= 144, else FP32
+ *                    MFMA) (default); 0 = every G by Gauss-Jordan
+ *                    elimination of z-H
  *   "spectral_chunk" 1 = the spectral kernel takes runs of up to 16 pairs of
  *                    one k-point per workgroup, loads V once per run and
  *                    writes the panels from the MFMA accumulators (default);
@@ -287,7 +290,9 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    two stages ahead (default); 0 = seven waves, one 16-row
  *                    strip each.  Results are bitwise the same for both
  *   "green_f16"      TH_PREC_MIXED spectral path: 1 = V diag(d) V^H on the fp16
- *                    matrix cores with two-term splits (default); 0 = FP32 MFMA
+ *                    matrix cores with two-term splits (default; d <= 144: the
+ *                    run kernel for d <= 54, the column-tiled kernel with the
+ *                    diagonal on its B side for d = 72 and 144); 0 = FP32 MFMA
  *   "contract_3m"    1 = the contraction forms each complex product from three
  *                    real MFMA products (Ar Br, Ai Bi, (Ar+Ai)(Br+Bi)) instead
  *                    of four; 0 = four (default)
