@@ -1353,7 +1353,9 @@ __global__ __launch_bounds__(64 * (((DD + 15) / 16 + RPW - 1) / RPW)) void green
         // A = V[r][m] d_m from device memory (the k-point's V stays in L2 over
         // its run), loaded PF K steps ahead of its use, every load unconditional
         // (the tail re-reads the last step)
-        constexpr int PF = (sizeof(R) == 8 && RPW > 1) ? 1 : 4;   // FP64 accumulators of two row blocks leave room for 1
+        // FP64 accumulators of two row blocks, or six FP32 blocks (d = 288, 48
+        // columns), leave room for one step
+        constexpr int PF = (sizeof(R) == 8 && RPW > 1) || (RPW > 1 && TB > 2) ? 1 : 4;
         static_assert(KS >= PF, "prefetch depth");
         double2 vq[PF][RPW];
         const double2 *Vl = Vk + (lane >> 4);
@@ -2223,7 +2225,9 @@ int launch_green_p(th_handle *h, const th_green_args &a, const green_params &g)
         case 144:
             if (h2) return launch_green_spectral_tiled_h2<144, 16>(h, a, g);
             return launch_green_spectral_tiled<P, 144, 48, 1>(h, a, g);
-        case 288: return launch_green_spectral_tiled<P, 288, 32, 2>(h, a, g);
+        case 288:   // FP32: 48-column tiles (110 KB of LDS); FP64 keeps 32 (48 would take 221 KB)
+            if constexpr (sizeof(P) == sizeof(float)) return launch_green_spectral_tiled<P, 288, 48, 2>(h, a, g);
+            else return launch_green_spectral_tiled<P, 288, 32, 2>(h, a, g);
         default: break;
         }
     }
