@@ -126,6 +126,48 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
     return out
 
 
+def loss_cost(cfg=3):
+    """The adjoint workload as specified: the scalar loss sum over the
+    configuration's 32 frequencies of (Im chi^{+-}(omega) - seeded target)^2
+    (the target of the model's own magnitude)
+    on its 24^3 k-mesh and 64 + 32 energies, through torch.autograd
+    (titan_amd.autodiff.chi_pm_loss): forward (32 chi on one handle) and
+    backward (32 th_chi_adjoint calls) wall time in both precisions, after an
+    untimed forward at the first frequency (the eigenpairs and the
+    omega-independent panels are then in place for both)."""
+    import torch
+    from titan_amd import binding as B
+    from titan_amd import configs as CF
+    from titan_amd.autodiff import ChiOperator, TorchModel, chi_pm_loss, nominal_chi_pm, target_curve
+    spec, run = CF.system(cfg), CF.run_params(cfg)
+    oms = list(run["omega"])
+    tm = TorchModel(spec)
+    tgt = None
+    out = {"workload": "cfg%d: %s, k-mesh %s, %d imag + %d real energies, loss over %d omega" %
+                       (cfg, spec["name"], "x".join(map(str, run["nk"])), run["n1gl"] * run["parts"],
+                        run["n3gl"] * run["parts3"], len(oms))}
+    for name, prec in (("fp64", B.PREC_FP64), ("mixed", B.PREC_MIXED)):
+        op = ChiOperator(tm, tuple(run["nk"]), n1gl=run["n1gl"], parts=run["parts"], n3gl=run["n3gl"],
+                         parts3=run["parts3"], eta=run["eta"], etap=run["etap"], q=run["q"], precision=prec)
+        if tgt is None:   # the target scaled to the model's response (untimed)
+            tgt = target_curve(oms, np.max(np.abs(nominal_chi_pm(op, tm, oms))))
+        else:
+            with torch.no_grad():
+                chi_pm_loss(op, tm.params(requires_grad=False), oms[:1], tgt[:1])
+        p = tm.params()
+        t0 = time.perf_counter()
+        L = chi_pm_loss(op, p, oms, tgt)
+        t1 = time.perf_counter()
+        L.backward()
+        t2 = time.perf_counter()
+        g = np.concatenate([[float(p["U"].grad[0, 1]), float(p["lam"].grad[0, 1])], p["sk"].grad[0, 0].numpy()])
+        out[name] = {"loss": float(L.detach()), "forward_s": t1 - t0, "backward_s": t2 - t1,
+                     "ratio": (t2 - t1) / (t1 - t0), "grad_U_lambda_sk": [float(x) for x in g]}
+    gf, gm = np.asarray(out["fp64"]["grad_U_lambda_sk"]), np.asarray(out["mixed"]["grad_U_lambda_sk"])
+    out["mixed_grad_rel_err_vs_fp64"] = float(np.linalg.norm(gm - gf) / max(np.linalg.norm(gf), 1e-300))
+    return out
+
+
 def current_response_cost(cfg=2, nk=16, omega=4e-3, reps=2):
     """Throughput of th_current_response (the spin-current response of
     eintshe) on the cfg system with an nk^3 k-mesh and the config's energies,
@@ -462,6 +504,12 @@ def main():
         except Exception as e:   # reported, never required
             log("adjoint cost failed:", e)
             out["adjoint_cost"] = None
+        try:
+            out["adjoint_loss"] = loss_cost()
+            log("adjoint loss", out["adjoint_loss"])
+        except Exception as e:   # reported, never required
+            log("adjoint loss failed:", e)
+            out["adjoint_loss"] = None
     if not args.no_current:
         try:
             out["current_response"] = current_response_cost()

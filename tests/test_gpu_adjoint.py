@@ -151,3 +151,67 @@ def test_parameter_gradients_match_finite_differences():
         fd = (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * h)
         print(key, float(g), fd, flush=True)
         assert abs(float(g) - fd) <= 1e-6 * abs(fd), (key, float(g), fd)
+
+
+def test_cfg3_loss_gradients():
+    """The adjoint workload's loss, sum over its 32 frequencies of
+    (Im chi^{+-}(omega) - seeded target)^2 on the 1-site bcc system (reduced
+    mesh: 3^3 k-points, 8 + 8 energies): the value against the oracle; the
+    gradients with respect to U_d, lambda_d and two Slater-Koster integrals
+    against five-point central differences (h = 1e-5) of the oracle's loss
+    (1e-6 relative); the mixed-precision gradient of U_d, lambda_d and the
+    ten first-shell integrals within 1e-5 of the FP64 one."""
+    from titan_amd.autodiff import ChiOperator, TorchModel, chi_pm_loss, nominal_chi_pm, target_curve
+    spec = CF.system(3)
+    run = CF.run_params(3)
+    oms = list(run["omega"])
+    assert len(oms) == 32
+    nk, n1, n3, eta = 3, 8, 8, 5e-3
+    tm = TorchModel(spec)
+    nom = nominal_chi_pm(ChiOperator(tm, (nk, nk, nk), n1gl=n1, n3gl=n3, eta=eta), tm, oms)
+    tgt = target_curve(oms, np.max(np.abs(nom)))
+    print("largest |Im chi+-| %.4e" % np.max(np.abs(nom)), flush=True)
+    grads = {}
+    for prec in (B.PREC_FP64, B.PREC_MIXED):
+        op = ChiOperator(tm, (nk, nk, nk), n1gl=n1, n3gl=n3, eta=eta, precision=prec)
+        p = tm.params()
+        L = chi_pm_loss(op, p, oms, tgt)
+        L.backward()
+        grads[prec] = (float(L.detach()), p)
+    Lv, p = grads[B.PREC_FP64]
+    op = ChiOperator(tm, (nk, nk, nk), n1gl=n1, n3gl=n3, eta=eta)
+    kp, wk = op.kp, op.wk
+    imag = (op.kim, op.yim, op.wim)
+
+    def L_oracle(sp):
+        so = O.OracleSystem(sp)
+        U = so.umat()
+        tot = 0.0
+        for om, tg in zip(oms, tgt):
+            re_k = (kp, wk) if om else None
+            re_e = Ct.real_epoints(so.Ef, om, 1, n3) if om else None
+            c0 = so.chi0([0, 0, 0], om, eta, eta, imag, re_k, re_e)
+            s = O.trace(1, tm.m.nOrb, O.dyson(c0, U))
+            tot += (s[0, 0].imag - tg) ** 2
+        return tot
+
+    Lo = L_oracle(spec)
+    print("loss GPU %.12e oracle %.12e" % (Lv, Lo), flush=True)
+    assert abs(Lv - Lo) <= 1e-10 * abs(Lo)
+    h = 1e-5
+    checks = ((("U", 1), p["U"].grad[0, 1]), (("lam", 1), p["lam"].grad[0, 1]),
+              (("sk", 0, 4), p["sk"].grad[0, 0, 4]), (("sk", 0, 9), p["sk"].grad[0, 0, 9]))
+    for key, g in checks:
+        vals = []
+        for k in (2, 1, -1, -2):
+            sp = copy.deepcopy(spec)
+            _set_param(sp, key, k * h)
+            vals.append(L_oracle(sp))
+        fd = (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * h)
+        print(key, float(g), fd, flush=True)
+        assert abs(float(g) - fd) <= 1e-6 * abs(fd), (key, float(g), fd)
+    pm = grads[B.PREC_MIXED][1]
+    g64 = np.concatenate([[float(p["U"].grad[0, 1]), float(p["lam"].grad[0, 1])], p["sk"].grad[0, 0].numpy()])
+    gmx = np.concatenate([[float(pm["U"].grad[0, 1]), float(pm["lam"].grad[0, 1])], pm["sk"].grad[0, 0].numpy()])
+    print("mixed vs FP64 gradient %.2e" % _rel(gmx, g64), flush=True)
+    assert _rel(gmx, g64) < 1e-5

@@ -174,6 +174,7 @@ class ChiOperator:
         self.parts3, self.n3gl = parts3, n3gl
         self.precision = precision
         self.adjoint_precision = precision if adjoint_precision is None else adjoint_precision
+        self._h, self._key = None, None
         y, wy = Ct.imag_epoints(parts, n1gl)
         self.kp, self.wk = Ct.uniform_kmesh(m.b, *kmesh)
         self.kim = np.tile(self.kp, (len(y), 1))
@@ -194,10 +195,20 @@ class ChiOperator:
         return Ct.real_epoints(self.Ef, omega, self.parts3, self.n3gl)
 
     def handle(self, t, h0):
+        """A handle of the model (t, h0): the one of the previous call while the
+        tables are unchanged, so that the frequencies of one loss share its
+        eigen-decomposition and omega-independent panels (a handle a pending
+        backward still holds stays alive through that reference)."""
         m = self.tm.m
-        h = B.Handle(m.nA, m.nOrb, m.R, m.jb, m.ib, np.asarray(t, dtype=complex), h0)
+        t = np.asarray(t, dtype=complex)
+        h0 = np.asarray(h0, dtype=complex)
+        key = (t.tobytes(), h0.tobytes())
+        if self._h is not None and self._key == key:
+            return self._h
+        h = B.Handle(m.nA, m.nOrb, m.R, m.jb, m.ib, t, h0)
         h.set_imag_points(self.kim, self.yim, self.wim)
         h.set_real_kmesh(self.kp, self.wk)
+        self._h, self._key = h, key
         return h
 
     def __call__(self, p, omega):
@@ -207,3 +218,44 @@ class ChiOperator:
     def trace(self, chi):
         """schi = S chi S^T (4nA x 4nA, index sigma*nA + i)."""
         return self.S @ chi @ self.S.T
+
+    def chi_pm(self, chi):
+        """The transverse susceptibility chi^{+-} summed over the sites: the
+        (+, +) block of schi (sigma = + for both indices in the reference's
+        labelling, its chi_i_j files), sum_ij schi[(+, i), (+, j)]."""
+        nA = self.tm.m.nA
+        s = self.trace(chi)
+        return torch.sum(s[0:nA, 0:nA])
+
+
+def target_curve(omegas, scale, seed=3):
+    """A seeded target for Im chi^{+-}(omega) of magnitude 'scale' (callers
+    take the largest |Im chi^{+-}| of the model at its nominal parameters): a
+    Lorentzian of seeded position, width and sign over the frequency window
+    plus a seeded linear background, zero at omega = 0."""
+    rng = np.random.default_rng(seed)
+    om = np.asarray(omegas, dtype=float)
+    top = max(float(om.max()), 1e-12)
+    w0 = top * rng.uniform(0.2, 0.8)
+    gam = top * rng.uniform(0.05, 0.2)
+    sign = -1.0 if rng.uniform() < 0.5 else 1.0
+    slope = rng.uniform(-0.3, 0.3)
+    lor = gam * om / ((om - w0) ** 2 + gam ** 2)
+    lor = lor / max(float(np.max(np.abs(lor))), 1e-300)
+    return float(scale) * (sign * lor + slope * om / top)
+
+
+def nominal_chi_pm(op, tm, omegas):
+    """Im chi^{+-}(omega) of the model at its nominal parameters (no graph)."""
+    with torch.no_grad():
+        p = tm.params(requires_grad=False)
+        return np.array([float(torch.imag(op.chi_pm(op(p, float(om))))) for om in omegas])
+
+
+def chi_pm_loss(op, p, omegas, target):
+    """The adjoint workload's scalar loss: sum over omega of
+    (Im chi^{+-}(omega) - target(omega))^2, differentiable in p."""
+    L = torch.zeros((), dtype=torch.float64)
+    for om, tg in zip(omegas, target):
+        L = L + (torch.imag(op.chi_pm(op(p, float(om)))) - float(tg)) ** 2
+    return L
