@@ -9,7 +9,9 @@ of titan_amd.autodiff.
     on-site energy through torch.autograd against central finite differences
     (h = 1e-5, five-point stencil) of the C oracle's chi built from the
     perturbed parameter set, to 1e-6 relative.
-  * bitwise reproducibility of the adjoint; the three-product contraction.
+  * bitwise reproducibility of the adjoint; the three-product contraction;
+    the Green functions by inversion (Hhat from each pair's G instead of the
+    spectral form).
   The FP64 calls follow an FP64 forward, so they read the imaginary-axis B
   side from the forward's panel cache; the mixed calls find an FP64 cache
   and recompute it.
@@ -50,6 +52,8 @@ CASES = {
     # name: (spec, nk, n1gl, n3gl, omega, q, with U)
     "bcc1_real_q_U": (CF.bcc_1site, 3, 6, 6, 3e-3, (0.05, -0.02, 0.1), True),
     "bcc1_imag_U": (CF.bcc_1site, 3, 6, 6, 0.0, (0.0, 0.0, 0.0), True),
+    # 20 real nodes: the real pairs' grouped order (blocks of 16 nodes, then 4)
+    "bcc1_real_nx20_U": (CF.bcc_1site, 2, 4, 20, 3e-3, (0.05, -0.02, 0.1), True),
     "bcc2_real_chi0": (lambda: CF.bcc_supercell(2), 2, 4, 4, 2e-3, (0.0, 0.0, 0.07), False),
     "bcc2_imag_U": (lambda: CF.bcc_supercell(2), 2, 4, 4, 0.0, (0.0, 0.0, 0.0), True),
     "pchain_real_U0": (lambda: CF.chain_p(), 5, 6, 6, 2e-3, (0.1, 0.0, 0.0), True),
@@ -92,6 +96,14 @@ def test_adjoint_matches_fp64_reference(name):
     h.tune(contract_3m=1)
     th4, hh4, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"], p2=c["p2"])
     assert _rel(th4, rt) <= 1e-10 and _rel(hh4, rh) <= 1e-10
+    # Green functions by inversion: Hhat from each pair's G (the spectral
+    # form V [sum_p ...] V^H needs the eigenpairs), both precisions
+    h.tune(contract_3m=0, spectral=0)
+    th5, hh5, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"], p2=c["p2"])
+    assert _rel(th5, rt) <= 1e-10 and _rel(hh5, rh) <= 1e-10
+    th6, hh6, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"],
+                            p2=c["p2"], precision=B.PREC_MIXED)
+    assert _rel(th6, rt) <= 1e-5 and _rel(hh6, rh) <= 1e-5
     h.close()
 
 

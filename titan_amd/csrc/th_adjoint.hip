@@ -275,7 +275,7 @@ __global__ void unfinalize_2(const double2 *__restrict__ chat, double *__restric
 struct assemble_params {
     int mode, nA, bs, d, Mpad, nA2;
     long p0, np, nk_re;
-    int nx;                  // real-axis nodes; real pairs k-major, p = ik nx + ie
+    int nx;                  // real-axis nodes; real pairs grouped k-major (th_real_grouped)
     const double *wim, *p2, *wre;
     const double *hat;       // [np][nA2][2][Mpad]
     double2 *out;            // [np][d][d]
@@ -302,7 +302,8 @@ __global__ void assemble_kernel(assemble_params a)
     } else if (a.mode == TH_G_IMAG_B) {     // B(i,j) = G(j,i)
         v = hat(ca * nA + ra, rr * bs + cc);
     } else if (a.mode == TH_G_REAL_A) {     // A = -i w (Gq - Gq^H):  Gqhat = i w (Ahat + Ahat^H)
-        const long ik = p / a.nx, ie = p - ik * a.nx;
+        unsigned ie, ik;
+        th_real_grouped((unsigned)p, (unsigned)a.nk_re, (unsigned)a.nx, ie, ik);
         const double w = a.p2[ie] * a.wre[ik];
         const double2 x = hat(ra * nA + ca, rr * bs + cc), y = hat(ca * nA + ra, cc * bs + rr);
         const double sr = x.x + y.x, si = x.y - y.y;
@@ -441,7 +442,8 @@ __global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const 
             double w = 0.0;
             if (a.mode == TH_G_IMAG_A) w = a.wim[p];
             if (a.mode == TH_G_REAL_A) {
-                const long ik = p / a.nx, ie = p - ik * a.nx;
+                unsigned ie, ik;
+                th_real_grouped((unsigned)p, (unsigned)a.nk_re, (unsigned)a.nx, ie, ik);
                 w = a.p2[ie] * a.wre[ik];
             }
             const double *H = a.hat + (size_t)pl * a.nA2 * 2 * a.Mpad;
@@ -504,6 +506,387 @@ __global__ __launch_bounds__(512) void hhat_seg_kernel(assemble_params a, const 
     }
 }
 
+// The same with the order D known at compile time (D = 18, 36, 54: one to
+// three atoms of nine spin orbitals).  A thread stages the nine entries
+// f = bidx + i (D/3)^2 of its own pair slot, whose offsets in the cotangent
+// panel are fixed for the launch and computed once, with its pair's weight
+// once per group and all nine entries' global loads issued together; the
+// product loops unroll with immediate LDS offsets.
+template <typename R, int D>
+__global__ __launch_bounds__(512) void hhat_seg_d_kernel(assemble_params a, const double2 *__restrict__ Graw,
+                                                         const int *__restrict__ seg, int ppw,
+                                                         double2 *__restrict__ out)
+{
+    typedef typename c2of<R>::T c2;
+    constexpr int DD = D * D, NB3 = D / 3, NBLK = NB3 * NB3, NE = DD / NBLK;
+    static_assert(NE == 9, "nine staged entries per thread");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const long q0 = seg[blockIdx.x] - a.p0, q1 = seg[blockIdx.x + 1] - a.p0;   // the batch's pairs
+    const int s = tid / NBLK, bidx = tid - s * NBLK;
+    const bool slot = s < ppw;
+    const int r0 = 3 * (bidx / NB3), c0 = 3 * (bidx - (bidx / NB3) * NB3);
+    c2 *G = reinterpret_cast<c2 *>(smem) + (size_t)(slot ? s : 0) * 2 * DD;
+    c2 *Gh = G + DD;
+    const int mode = a.mode, bs = a.bs, nA = a.nA, Mp = a.Mpad, M2 = 2 * a.Mpad;
+    const float rbs = 1.0f / (float)bs;   // (x + 0.5) / bs truncates to x / bs for x < D <= 54
+    double2 tot[3][3];
+    c2 acc[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) tot[i][j] = make_double2(0.0, 0.0);
+    for (long g0 = q0; g0 < q1; g0 += ppw) {
+        const long pl = g0 + s;
+        const bool active = slot && pl < q1;
+        __syncthreads();                         // the previous group's reads of G, Gh done
+        if (active) {
+            const int p = (int)(a.p0 + pl);
+            double w = 1.0;
+            if (mode == TH_G_IMAG_A) w = a.wim[p];
+            if (mode == TH_G_REAL_A) {
+                unsigned ie, ik;
+                th_real_grouped((unsigned)p, (unsigned)a.nk_re, (unsigned)a.nx, ie, ik);
+                w = a.p2[ie] * a.wre[ik];
+            }
+            const double *H = a.hat + (size_t)pl * a.nA2 * M2;
+            const double2 *Gp = Graw + (size_t)pl * DD;
+            // three chunks of three entries: the loads of a chunk in flight together
+#pragma unroll 1
+            for (int i0 = 0; i0 < NE; i0 += 3) {
+                double2 gv[3], hv[3], yv[3];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int f = bidx + (i0 + u) * NBLK, r = f / D, c = f - (f / D) * D;
+                    const int ra = (int)(((float)r + 0.5f) * rbs), ca = (int)(((float)c + 0.5f) * rbs);
+                    const int rr = r - ra * bs, cc = c - ca * bs;
+                    // block (i,j) entry [rr][cc], its transpose (j,i) [cc][rr]
+                    const int ox = mode == TH_G_IMAG_B ? (ca * nA + ra) * M2 + rr * bs + cc
+                                 : mode == TH_G_REAL_B ? (ra * nA + ca) * M2 + cc * bs + rr
+                                 : (ra * nA + ca) * M2 + rr * bs + cc;
+                    const int oy = (ca * nA + ra) * M2 + cc * bs + rr;
+                    gv[u] = Gp[f];
+                    hv[u] = make_double2(H[ox], H[ox + Mp]);
+                    yv[u] = mode == TH_G_REAL_A ? make_double2(H[oy], H[oy + Mp]) : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    double2 v = hv[u];
+                    if (mode == TH_G_REAL_A)             // Gqhat = i w (Ahat + Ahat^H)
+                        v = make_double2(-w * (v.y - yv[u].y), w * (v.x + yv[u].x));
+                    else if (mode == TH_G_IMAG_A)        // Gqhat = w Ahat
+                        v = make_double2(w * v.x, w * v.y);
+                    else if (mode == TH_G_REAL_B)        // Ghat(i,j)[cc][rr] = conj(Bhat(i,j)[rr][cc])
+                        v.y = -v.y;
+                    const int f = bidx + (i0 + u) * NBLK;
+                    G[f] = c2{(R)gv[u].x, (R)gv[u].y};
+                    Gh[f] = c2{(R)v.x, (R)v.y};
+                }
+            }
+        }
+        __syncthreads();
+        // X = Ghat G^H: X[r][c] = sum_m Ghat[r][m] conj(G[c][m])
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[i][j] = c2{R(0), R(0)};
+#pragma unroll 6
+            for (int m = 0; m < D; ++m) {
+                c2 x[3], y[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) x[i] = Gh[(r0 + i) * D + m];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) y[j] = G[(c0 + j) * D + m];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[i][j] = c2{fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                       fma(x[i].y, y[j].x, fma(-x[i].x, y[j].y, acc[i][j].y))};
+            }
+        }
+        __syncthreads();
+        if (active)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Gh[(r0 + i) * D + c0 + j] = acc[i][j];
+        __syncthreads();
+        // Hhat = G^H X: Hhat[r][c] = sum_m conj(G[m][r]) X[m][c], added to the sum
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[i][j] = c2{R(0), R(0)};
+#pragma unroll 6
+            for (int m = 0; m < D; ++m) {
+                c2 x[3], y[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) x[i] = G[m * D + r0 + i];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) y[j] = Gh[m * D + c0 + j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[i][j] = c2{fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                       fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y))};
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    tot[i][j] = make_double2(tot[i][j].x + (double)acc[i][j].x, tot[i][j].y + (double)acc[i][j].y);
+        }
+    }
+    __syncthreads();
+    // the ppw partial sums, added in order of s (threads of slots never
+    // active hold zeros)
+    double2 *red = reinterpret_cast<double2 *>(smem);
+    if (slot)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) red[(size_t)s * DD + (r0 + i) * D + c0 + j] = tot[i][j];
+    __syncthreads();
+    for (int f = tid; f < DD; f += blockDim.x) {
+        double2 v = red[f];
+        for (int t = 1; t < ppw; ++t) v = zadd(v, red[(size_t)t * DD + f]);
+        out[(size_t)blockIdx.x * DD + f] = v;
+    }
+}
+
+// Spectral form of the segment sums.  With G_p = V diag(g_p) V^H, g_p,m =
+// 1/(z_p - lambda_m), and V the eigenvectors of the segment's k-point,
+//   sum_p G_p^H Ghat_p G_p^H = V [ sum_p (conj g_p conj g_p^T) o (V^H Ghat_p V) ] V^H,
+// so no pair's G is formed or read: the workgroup stages V and lambda once,
+// per pair only Ghat_p and conj g_p.  Y_p = V^H (Ghat_p V) runs in R with a
+// thread's 3 x 3 register block as in hhat_seg_d_kernel; W sums the scaled
+// Y_p in FP64; the slots' W are added in order of slot, and V W V^H is formed
+// in FP64 from the FP64 eigenvectors by every thread, an entry at a time.
+struct hspec_params {
+    const double2 *V;        // [nk][D][D] row-major, V[r][m] = component r of eigenvector m
+    const double *L;         // [nk][D]
+    const int *kid;          // unique k-point of an imaginary pair, or of a real-axis mesh point
+    const double *yim, *x2;
+    double er0, ei0;         // z = er0 + x2[ie] (real axis) + i (ei0 + yim[p] (imaginary axis))
+};
+
+template <typename R, int D>
+__global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec_params sp,
+                                                        const int *__restrict__ seg, int ppw,
+                                                        double2 *__restrict__ out)
+{
+    typedef typename c2of<R>::T c2;
+    constexpr int DD = D * D, NB3 = D / 3, NBLK = NB3 * NB3, NE = DD / NBLK;
+    static_assert(NE == 9, "nine staged entries per thread");
+    // LDS: V (R) | W (FP64) | lambda | conj g per slot (FP64) | Ghat, then X, per slot (R)
+    extern __shared__ __align__(16) unsigned char smem[];
+    c2 *Vs = reinterpret_cast<c2 *>(smem);
+    double2 *Ws = reinterpret_cast<double2 *>(Vs + DD);
+    double *lam = reinterpret_cast<double *>(Ws + DD);
+    double2 *gs = reinterpret_cast<double2 *>(lam + D);
+    c2 *slots = reinterpret_cast<c2 *>(gs + (size_t)ppw * D);
+    const int tid = threadIdx.x;
+    const bool imag = a.mode == TH_G_IMAG_A || a.mode == TH_G_IMAG_B;
+    const int pfirst = seg[blockIdx.x];
+    const long q0 = pfirst - a.p0, q1 = seg[blockIdx.x + 1] - a.p0;   // the batch's pairs
+    const int s = tid / NBLK, bidx = tid - s * NBLK;
+    const bool slot = s < ppw;
+    const int r0 = 3 * (bidx / NB3), c0 = 3 * (bidx - (bidx / NB3) * NB3);
+    c2 *Gh = slots + (size_t)(slot ? s : 0) * DD;
+    double2 *gp = gs + (size_t)(slot ? s : 0) * D;
+    const int mode = a.mode, bs = a.bs, nA = a.nA, Mp = a.Mpad, M2 = 2 * a.Mpad;
+    const float rbs = 1.0f / (float)bs;   // (x + 0.5) / bs truncates to x / bs for x < D <= 54
+    // the segment's k-point (all its pairs share it)
+    int kb;
+    if (imag) {
+        kb = sp.kid[pfirst];
+    } else {
+        unsigned ie, ik;
+        th_real_grouped((unsigned)pfirst, (unsigned)a.nk_re, (unsigned)a.nx, ie, ik);
+        kb = sp.kid[ik];
+    }
+    const double2 *Vk = sp.V + (size_t)kb * DD;
+    for (int e = tid; e < DD; e += blockDim.x) {
+        const double2 v = Vk[e];
+        Vs[e] = c2{(R)v.x, (R)v.y};
+    }
+    for (int m = tid; m < D; m += blockDim.x) lam[m] = sp.L[(size_t)kb * D + m];
+    double2 tot[3][3];
+    c2 acc[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) tot[i][j] = make_double2(0.0, 0.0);
+    for (long g0 = q0; g0 < q1; g0 += ppw) {
+        const long pl = g0 + s;
+        const bool active = slot && pl < q1;
+        __syncthreads();                         // V, lambda staged; the previous group's reads done
+        if (active) {
+            const int p = (int)(a.p0 + pl);
+            double w = 1.0, er = sp.er0, ei = sp.ei0;
+            if (imag) {
+                ei += sp.yim[p];
+                if (mode == TH_G_IMAG_A) w = a.wim[p];
+            } else {
+                unsigned ie, ik;
+                th_real_grouped((unsigned)p, (unsigned)a.nk_re, (unsigned)a.nx, ie, ik);
+                er += sp.x2[ie];
+                if (mode == TH_G_REAL_A) w = a.p2[ie] * a.wre[ik];
+            }
+            if (bidx < D) {                      // conj g_m = conj(1 / (z - lambda_m))
+                const double x = er - lam[bidx], n = 1.0 / (x * x + ei * ei);
+                gp[bidx] = make_double2(x * n, ei * n);
+            }
+            const double *H = a.hat + (size_t)pl * a.nA2 * M2;
+#pragma unroll 1
+            for (int i0 = 0; i0 < NE; i0 += 3) {
+                double2 hv[3], yv[3];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int f = bidx + (i0 + u) * NBLK, r = f / D, c = f - (f / D) * D;
+                    const int ra = (int)(((float)r + 0.5f) * rbs), ca = (int)(((float)c + 0.5f) * rbs);
+                    const int rr = r - ra * bs, cc = c - ca * bs;
+                    const int ox = mode == TH_G_IMAG_B ? (ca * nA + ra) * M2 + rr * bs + cc
+                                 : mode == TH_G_REAL_B ? (ra * nA + ca) * M2 + cc * bs + rr
+                                 : (ra * nA + ca) * M2 + rr * bs + cc;
+                    const int oy = (ca * nA + ra) * M2 + cc * bs + rr;
+                    hv[u] = make_double2(H[ox], H[ox + Mp]);
+                    yv[u] = mode == TH_G_REAL_A ? make_double2(H[oy], H[oy + Mp]) : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    double2 v = hv[u];
+                    if (mode == TH_G_REAL_A)             // Gqhat = i w (Ahat + Ahat^H)
+                        v = make_double2(-w * (v.y - yv[u].y), w * (v.x + yv[u].x));
+                    else if (mode == TH_G_IMAG_A)        // Gqhat = w Ahat
+                        v = make_double2(w * v.x, w * v.y);
+                    else if (mode == TH_G_REAL_B)        // Ghat(i,j)[cc][rr] = conj(Bhat(i,j)[rr][cc])
+                        v.y = -v.y;
+                    Gh[bidx + (i0 + u) * NBLK] = c2{(R)v.x, (R)v.y};
+                }
+            }
+        }
+        __syncthreads();
+        // X = Ghat V: X[r][c] = sum_m Ghat[r][m] V[m][c]
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[i][j] = c2{R(0), R(0)};
+#pragma unroll 6
+            for (int m = 0; m < D; ++m) {
+                c2 x[3], y[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) x[i] = Gh[(r0 + i) * D + m];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) y[j] = Vs[m * D + c0 + j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[i][j] = c2{fma(x[i].x, y[j].x, fma(-x[i].y, y[j].y, acc[i][j].x)),
+                                       fma(x[i].x, y[j].y, fma(x[i].y, y[j].x, acc[i][j].y))};
+            }
+        }
+        __syncthreads();
+        if (active)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Gh[(r0 + i) * D + c0 + j] = acc[i][j];
+        __syncthreads();
+        // Y = V^H X: Y[r][c] = sum_m conj(V[m][r]) X[m][c]; W += conj(g_r) conj(g_c) Y
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[i][j] = c2{R(0), R(0)};
+#pragma unroll 6
+            for (int m = 0; m < D; ++m) {
+                c2 x[3], y[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) x[i] = Vs[m * D + r0 + i];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) y[j] = Gh[m * D + c0 + j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[i][j] = c2{fma(x[i].x, y[j].x, fma(x[i].y, y[j].y, acc[i][j].x)),
+                                       fma(x[i].x, y[j].y, fma(-x[i].y, y[j].x, acc[i][j].y))};
+            }
+            double2 gr[3], gc[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) { gr[i] = gp[r0 + i]; gc[i] = gp[c0 + i]; }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double2 f = zmul(gr[i], gc[j]);
+                    const double yr = (double)acc[i][j].x, yi = (double)acc[i][j].y;
+                    tot[i][j] = make_double2(tot[i][j].x + f.x * yr - f.y * yi, tot[i][j].y + f.x * yi + f.y * yr);
+                }
+        }
+    }
+    // W = the slots' sums, added in order of slot (slots never active hold zeros)
+    for (int t = 0; t < ppw; ++t) {
+        __syncthreads();
+        if (s == t)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double2 &o = Ws[(r0 + i) * D + c0 + j];
+                    o = t == 0 ? tot[i][j] : zadd(o, tot[i][j]);
+                }
+    }
+    __syncthreads();
+    // U = W V^H, then Hhat = V U, in FP64 with V in FP64 in the slot space,
+    // entries e = tid + k blockDim.x over every thread
+    constexpr int NU = (DD + 64 * ((NBLK + 63) / 64) - 1) / (64 * ((NBLK + 63) / 64));   // fewest threads: one slot
+    double2 *V64 = reinterpret_cast<double2 *>(slots);
+    for (int e = tid; e < DD; e += blockDim.x) V64[e] = Vk[e];
+    __syncthreads();
+    double2 u[NU];
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        const int e = tid + k * (int)blockDim.x, r = e / D, c = e - (e / D) * D;
+        double2 v = make_double2(0.0, 0.0);
+        if (e < DD)
+#pragma unroll 3
+            for (int m = 0; m < D; ++m) v = zadd(v, zmul(Ws[r * D + m], zconj(V64[c * D + m])));
+        u[k] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        const int e = tid + k * (int)blockDim.x;
+        if (e < DD) Ws[e] = u[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        const int e = tid + k * (int)blockDim.x, r = e / D, c = e - (e / D) * D;
+        if (e < DD) {
+            double2 v = make_double2(0.0, 0.0);
+#pragma unroll 3
+            for (int m = 0; m < D; ++m) v = zadd(v, zmul(V64[r * D + m], Ws[m * D + c]));
+            out[(size_t)blockIdx.x * DD + e] = v;
+        }
+    }
+}
+
+// LDS bytes of hhat_spec_kernel (the slot space also holds V in FP64 at the end)
+inline size_t hspec_lds(int D, int ppw, bool f32)
+{
+    const size_t c2 = f32 ? sizeof(float2) : sizeof(double2), dd = (size_t)D * D;
+    return dd * (c2 + sizeof(double2)) + sizeof(double) * D + sizeof(double2) * ppw * D +
+           std::max(c2 * ppw * dd, sizeof(double2) * dd);
+}
+
 // Hhat of the batch's pairs summed over each segment (the generic path)
 __global__ void seg_sum_kernel(const double2 *__restrict__ Hh, const int *__restrict__ seg, long p0, int nseg, int dd,
                                double2 *__restrict__ out)
@@ -546,16 +929,19 @@ __global__ void raw_from_bpanel(const P *__restrict__ panel, long np, int nA, in
 }
 
 // exp(-i k.R_n) for the segments of a batch (k of the segment's first pair,
-// seg[sg] in the whole list; real pairs k-major, nx nodes): ph[sg * nhop + n]
-__global__ void phase_kernel(const int *seg, int nseg, int nhop, int imag, int side_q, int nx, double q0, double q1,
-                             double q2, const double *kim, const double *kre, const double *R, double2 *ph)
+// seg[sg] in the whole list; real pairs grouped k-major over nk k-points and
+// nx nodes): ph[sg * nhop + n]
+__global__ void phase_kernel(const int *seg, int nseg, int nhop, int imag, int side_q, int nk, int nx, double q0,
+                             double q1, double q2, const double *kim, const double *kre, const double *R, double2 *ph)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nseg * nhop) return;
     const int sg = (int)(e / nhop);
     const int n = (int)(e - (size_t)sg * nhop);
     const long p = seg[sg];
-    const double *k = imag ? kim + 3 * p : kre + 3 * (p / nx);
+    unsigned ie, ik;
+    th_real_grouped((unsigned)p, (unsigned)nk, (unsigned)nx, ie, ik);
+    const double *k = imag ? kim + 3 * p : kre + 3 * (size_t)ik;
     double k0 = k[0], k1 = k[1], k2 = k[2];
     if (side_q) { k0 += q0; k1 += q1; k2 += q2; }
     double sn, cs;
@@ -874,15 +1260,25 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
 
     // segments: runs of consecutive pairs on one k-point within a batch (the
     // stored imaginary list keeps a k-point's pairs together in runs; real
-    // pairs are taken k-major, n_x per k-point); their first pairs (indices of
-    // the whole list) and the end of the last, axis 0 then axis 1
+    // pairs are taken grouped k-major, th_real_grouped: runs of up to TH_RUN
+    // nodes of one k-point); their first pairs (indices of the whole list)
+    // and the end of the last, axis 0 then axis 1
     std::vector<int> segv[2];
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
         const long npairs = axis == 0 ? h->n_im : n_re;
         std::vector<int> &sv = segv[axis];
         sv.reserve((size_t)npairs / 8 + 2);
+        unsigned ik_prev = 0;
         for (long p = 0; p < npairs; ++p) {
-            const bool brk = p % bp == 0 || (axis == 0 ? h->kid_im_local[p] != h->kid_im_local[p - 1] : p % n_x == 0);
+            bool brk = p % bp == 0;
+            if (axis == 0) {
+                brk = brk || h->kid_im_local[p] != h->kid_im_local[p - 1];
+            } else {
+                unsigned ie, ik;
+                th_real_grouped((unsigned)p, (unsigned)h->nk_re, (unsigned)n_x, ie, ik);
+                brk = brk || ik != ik_prev;
+                ik_prev = ik;
+            }
             if (brk) sv.push_back((int)p);
         }
         sv.push_back((int)npairs);
@@ -942,6 +1338,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     // split words in the cache are read back as (hi + lo) 2^-e
     const int split_e = (use_bcache && h->bcache_presplit) ? h->bcache_pexp : INT_MIN;
     const int ntile = (d + 31) / 32;
+    // Hhat in the spectral form when both sides' eigenpairs are ready (then
+    // neither side's G is formed: the Green stage writes panels only)
+    const bool spec_h = (d == 18 || d == 36 || d == 54) && th_side_eig(h, true, q) && th_side_eig(h, false, q);
     // 3. pair batches of each axis
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
         const long npairs = axis == 0 ? h->n_im : n_re;
@@ -966,7 +1365,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             id = tm.begin(0);
             a.mode = axis == 0 ? TH_G_IMAG_A : TH_G_REAL_A;
             a.panel = h->d_pA;
-            a.raw = rawA;
+            a.raw = spec_h ? nullptr : rawA;
             if ((rc = th_launch_green(h, a))) return rc;
             // the imaginary-axis B side does not depend on omega: when the forward's
             // panel cache holds it (same E_F, eta', precision), read G from there
@@ -979,19 +1378,21 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 brows.pg0 = p0;
                 brows.bpc = h->bcache_bp;
                 brows.ntot = h->n_im;
-                const unsigned g = nblk((size_t)np * dd, 256);
-                const float unsplit = split_e == INT_MIN ? 0.0f : ldexpf(1.0f, -split_e);
-                if (precision == TH_PREC_FP64)
-                    hipLaunchKernelGGL(raw_from_bpanel<double>, dim3(g), dim3(256), 0, h->stream, (const double *)pB,
-                                       np, h->nA, h->bs, h->Mpad, brows, 0.0f, rawB);
-                else
-                    hipLaunchKernelGGL(raw_from_bpanel<float>, dim3(g), dim3(256), 0, h->stream, (const float *)pB, np,
-                                       h->nA, h->bs, h->Mpad, brows, unsplit, rawB);
-                TH_HIP(hipGetLastError());
+                if (!spec_h) {   // G of the B side for the per-pair Hhat products
+                    const unsigned g = nblk((size_t)np * dd, 256);
+                    const float unsplit = split_e == INT_MIN ? 0.0f : ldexpf(1.0f, -split_e);
+                    if (precision == TH_PREC_FP64)
+                        hipLaunchKernelGGL(raw_from_bpanel<double>, dim3(g), dim3(256), 0, h->stream,
+                                           (const double *)pB, np, h->nA, h->bs, h->Mpad, brows, 0.0f, rawB);
+                    else
+                        hipLaunchKernelGGL(raw_from_bpanel<float>, dim3(g), dim3(256), 0, h->stream,
+                                           (const float *)pB, np, h->nA, h->bs, h->Mpad, brows, unsplit, rawB);
+                    TH_HIP(hipGetLastError());
+                }
             } else {
                 a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
                 a.panel = h->d_pB;
-                a.raw = rawB;
+                a.raw = spec_h ? nullptr : rawB;
                 if ((rc = th_launch_green(h, a))) return rc;
             }
             tm.end(id);
@@ -1015,7 +1416,35 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 ap.out = gh;
                 // Hhat = G^H Ghat G^H per pair, summed over each segment
                 const double2 *Hseg = gh;
-                if (d <= 54 && d % 3 == 0) {
+                if (spec_h) {
+                    // spectral form: V and lambda of the segment's k-point, no G
+                    const th_eig_set *es = th_side_eig(h, side == 0, q);
+                    hspec_params sp{};
+                    sp.V = es->V; sp.L = es->L;
+                    sp.kid = axis == 0 ? h->d_kid_im : h->d_kid_re;
+                    sp.yim = h->d_yim; sp.x2 = h->d_x2;
+                    // z = E_F + omega + i (y + eta), E_F + i (y + eta'), x + omega + i eta, x + i eta'
+                    sp.er0 = (axis == 0 ? Ef : 0.0) + (side == 0 ? omega : 0.0);
+                    sp.ei0 = side == 0 ? eta : etap;
+                    const bool f32 = precision != TH_PREC_FP64;
+                    const int nblk3 = (d / 3) * (d / 3);
+                    int ppw = std::max(1, 512 / nblk3);
+                    while (ppw > 1 && hspec_lds(d, ppw, f32) > 64 * 1024) --ppw;
+                    if (d == 18) ppw = std::min(ppw, f32 ? 8 : 4);   // runs of 16 and 64 pairs in whole groups
+                    const int nt = ((ppw * nblk3 + 63) / 64) * 64;
+                    const size_t lds = hspec_lds(d, ppw, f32);
+                    auto launch = [&](auto kern) {
+                        TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                        hipLaunchKernelGGL(kern, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, sp, dseg, ppw, gh);
+                        return 0;
+                    };
+                    if (d == 18) rc = f32 ? launch(hhat_spec_kernel<float, 18>) : launch(hhat_spec_kernel<double, 18>);
+                    else if (d == 36) rc = f32 ? launch(hhat_spec_kernel<float, 36>) : launch(hhat_spec_kernel<double, 36>);
+                    else rc = f32 ? launch(hhat_spec_kernel<float, 54>) : launch(hhat_spec_kernel<double, 54>);
+                    if (rc) return rc;
+                    TH_HIP(hipGetLastError());
+                } else if (d <= 54 && d % 3 == 0) {
                     // the pairs' G, Ghat in LDS, 3 x 3 register blocks, one workgroup per segment
                     const int nblk3 = (d / 3) * (d / 3);
                     // at most ~42 KB of LDS (three workgroups per CU); FP32 products
@@ -1025,15 +1454,17 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                     const int ppw = std::max(1, std::min(42 * 1024 / (int)(2 * dd * esz), 320 / nblk3));
                     const int nt = ((ppw * nblk3 + 63) / 64) * 64;
                     const size_t lds = std::max((size_t)ppw * 2 * dd * esz, (size_t)ppw * dd * sizeof(double2));
-                    const void *kfn = f32 ? reinterpret_cast<const void *>(hhat_seg_kernel<float>)
-                                          : reinterpret_cast<const void *>(hhat_seg_kernel<double>);
-                    TH_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    if (f32)
-                        hipLaunchKernelGGL(hhat_seg_kernel<float>, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, G,
-                                           dseg, ppw, gh);
-                    else
-                        hipLaunchKernelGGL(hhat_seg_kernel<double>, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap,
-                                           G, dseg, ppw, gh);
+                    auto launch = [&](auto kern) {
+                        TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                        hipLaunchKernelGGL(kern, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, G, dseg, ppw, gh);
+                        return 0;
+                    };
+                    if (d == 18) rc = f32 ? launch(hhat_seg_d_kernel<float, 18>) : launch(hhat_seg_d_kernel<double, 18>);
+                    else if (d == 36) rc = f32 ? launch(hhat_seg_d_kernel<float, 36>) : launch(hhat_seg_d_kernel<double, 36>);
+                    else if (d == 54) rc = f32 ? launch(hhat_seg_d_kernel<float, 54>) : launch(hhat_seg_d_kernel<double, 54>);
+                    else rc = f32 ? launch(hhat_seg_kernel<float>) : launch(hhat_seg_kernel<double>);
+                    if (rc) return rc;
                     TH_HIP(hipGetLastError());
                 } else {
                     hipLaunchKernelGGL(assemble_kernel, dim3(nblk((size_t)np * dd, 256)), dim3(256), 0, h->stream, ap);
@@ -1050,9 +1481,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                     Hseg = tmp;
                 }
                 hipLaunchKernelGGL(phase_kernel, dim3(nblk((size_t)nseg * std::max(1, h->nhop), 256)), dim3(256), 0,
-                                   h->stream, dseg, nseg, h->nhop, axis == 0 ? 1 : 0, side == 0 ? 1 : 0, n_x, q[0], q[1],
-                                   q[2], (const double *)h->d_kim, (const double *)h->d_kre, (const double *)h->d_R,
-                                   d_ph);
+                                   h->stream, dseg, nseg, h->nhop, axis == 0 ? 1 : 0, side == 0 ? 1 : 0, (int)h->nk_re,
+                                   n_x, q[0], q[1], q[2], (const double *)h->d_kim, (const double *)h->d_kre,
+                                   (const double *)h->d_R, d_ph);
                 TH_HIP(hipGetLastError());
                 hreduce_params hp{};
                 hp.side_q = side == 0 ? 1 : 0;

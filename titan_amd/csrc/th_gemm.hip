@@ -905,6 +905,20 @@ __global__ __launch_bounds__(NTH) void panel_adjoint_kernel(const P *__restrict_
 
 // Largest |x| of a buffer as the bits of a non-negative float (atomicMax on
 // the bits: they order as the values); split words count as (hi + lo) u.
+// The workgroup's largest value, one atomicMax per workgroup (atomics of
+// every wave on the one address serialise and would dominate the scan).
+__device__ inline void block_max_to(float m, unsigned *mx)
+{
+    __shared__ float wm[4];
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wm[w]);
+        atomicMax(mx, __builtin_bit_cast(unsigned, m));
+    }
+}
+
 __global__ __launch_bounds__(256) void absmax_f_kernel(const float *__restrict__ x, size_t n, float u,
                                                        unsigned *__restrict__ mx)
 {
@@ -917,14 +931,21 @@ __global__ __launch_bounds__(256) void absmax_f_kernel(const float *__restrict__
         }
         return fabsf(v);
     };
-    // 16-byte loads (n is a multiple of 4: whole panel rows of 2 Mpad floats)
+    auto vmax = [&](float4 v) { return fmaxf(fmaxf(val(v.x), val(v.y)), fmaxf(val(v.z), val(v.w))); };
+    // 16-byte loads (n is a multiple of 4: whole panel rows of 2 Mpad floats),
+    // four of them in flight per thread
     const float4 *x4 = reinterpret_cast<const float4 *>(x);
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n / 4; e += (size_t)gridDim.x * blockDim.x) {
-        const float4 v = x4[e];
-        m = fmaxf(m, fmaxf(fmaxf(val(v.x), val(v.y)), fmaxf(val(v.z), val(v.w))));
+    const size_t n4 = n / 4, st = (size_t)gridDim.x * blockDim.x;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; e + 3 * st < n4; e += 4 * st) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = x4[e + u * st];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m = fmaxf(m, vmax(v[u]));
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(mx, __builtin_bit_cast(unsigned, m));
+    for (; e < n4; e += st) m = fmaxf(m, vmax(x4[e]));
+    block_max_to(m, mx);
 }
 
 __global__ __launch_bounds__(256) void absmax_d_kernel(const double *__restrict__ x, size_t n, unsigned *__restrict__ mx)
@@ -932,8 +953,7 @@ __global__ __launch_bounds__(256) void absmax_d_kernel(const double *__restrict_
     float m = 0.0f;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         m = fmaxf(m, (float)fabs(x[e]));
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(mx, __builtin_bit_cast(unsigned, m));
+    block_max_to(m, mx);
 }
 
 __global__ void zero_u_kernel(unsigned *p, int n)

@@ -45,7 +45,7 @@ struct green_params {
     int mode, prec;
     int nA, nOrb, bs, d, nhop, Mpad, nA2;
     long p0, np, nk_re;
-    int nx, kmajor_re;       // real axis: nx nodes; pair p = ik * nx + ie when kmajor_re, else ie * nk_re + ik
+    int nx, kmajor_re;       // real axis: nx nodes; pairs grouped k-major when kmajor_re (th_real_grouped), else p = ie * nk_re + ik
     double q0, q1, q2, omega, Ef, eta, etap;
     const double *omega_p;   // when set, omega is read from device memory (graph replays)
     const double *R;
@@ -154,20 +154,15 @@ __device__ void gj_invert(cplx<R> *M, int d, cplx<R> *colk, cplx<R> *prow, int *
 // (k, z, weight) of pair p for the kernel mode
 // node ie and k-point ik of real-axis pair p (pair counts below 2^31,
 // checked on the host, so the divisions are 32-bit).  Node-major:
-// p = ie * nk_re + ik.  Grouped (kmajor_re): the nodes split into blocks of
-// TH_RUN (the last one shorter), block b holds its nodes for every k-point,
-// k-point by k-point: p = b TH_RUN nk_re + ik * len(b) + (ie - b TH_RUN).
+// p = ie * nk_re + ik; grouped k-major (kmajor_re): th_real_grouped.
 __device__ inline void real_split(const green_params &g, long p, long &ie, long &ik)
 {
     // both layouts computed, the result selected by value (no branch-dependent
     // stores through the references, which the compiler would keep in memory)
     const unsigned up = (unsigned)p, nk = (unsigned)g.nk_re;
     const unsigned a0 = up / nk, ie0 = a0, ik0 = up - a0 * nk;
-    const unsigned nx = g.nx > 0 ? (unsigned)g.nx : 1u, full = (unsigned)TH_RUN * nk;
-    const unsigned nb = (nx + TH_RUN - 1) / TH_RUN, bq = up / full, b = bq < nb ? bq : nb - 1;
-    const unsigned off = up - b * full, rest = nx - (unsigned)TH_RUN * b;
-    const unsigned bl = rest < (unsigned)TH_RUN ? rest : (unsigned)TH_RUN;
-    const unsigned a1 = off / bl, ik1 = a1, ie1 = (unsigned)TH_RUN * b + (off - a1 * bl);
+    unsigned ie1, ik1;
+    th_real_grouped(up, nk, g.nx > 0 ? (unsigned)g.nx : 1u, ie1, ik1);
     ie = (long)(g.kmajor_re ? ie1 : ie0);
     ik = (long)(g.kmajor_re ? ik1 : ik0);
 }
@@ -2260,7 +2255,7 @@ int launch_green_t(th_handle *h, const th_green_args &a, const green_params &g)
 int *th_status_word(th_handle *h);
 
 // Eigenpair set the spectral kernels use for one side (nullptr: inversion)
-static const th_eig_set *side_eig(const th_handle *h, bool qside, const double *q)
+const th_eig_set *th_side_eig(const th_handle *h, bool qside, const double *q)
 {
     const bool qzero = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;
     if (h->spectral && h->eig_valid[0] && (!qside || qzero)) return &h->eig[0];
@@ -2294,7 +2289,7 @@ long th_green_wave_pairs(th_handle *h)
 bool th_green_presplit_ok(const th_handle *h, const double *q)
 {
     if (!(h->presplit && h->kernel_variant == 0 && h->spectral_chunk > 0 && h->green_f16)) return false;
-    const th_eig_set *ea = side_eig(h, true, q), *eb = side_eig(h, false, q);
+    const th_eig_set *ea = th_side_eig(h, true, q), *eb = th_side_eig(h, false, q);
     if (!ea || !eb) return false;
     // the fp16 kernels read the split planes of V
     return (h->d == 18 || h->d == 36 || h->d == 54 || h->d == 72 || h->d == 144) && ea->S && eb->S;
@@ -2332,7 +2327,7 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.status = th_status_word(h);
     // spectral Green functions when the eigenpairs of this side's k-points are ready
     const bool qside = a.mode == TH_G_IMAG_A || a.mode == TH_G_REAL_A;
-    const th_eig_set *es = side_eig(h, qside, a.q);
+    const th_eig_set *es = th_side_eig(h, qside, a.q);
     g.eigV = es ? es->V : nullptr;
     g.eigL = es ? es->L : nullptr;
     g.eigS = es ? reinterpret_cast<const _Float16 *>(es->S) : nullptr;

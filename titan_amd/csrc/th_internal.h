@@ -24,6 +24,21 @@ constexpr int TH_MAX_NA = 32;
 
 inline int th_mpad(int M) { return ((M + TH_TILE - 1) / TH_TILE) * TH_TILE; }
 
+// Node ie and k-point ik of real-axis pair p in the grouped k-major order
+// (the run-per-k Green kernels need TH_RUN consecutive pairs of one k-point):
+// the nx nodes split into blocks of TH_RUN, the last one shorter; block b
+// holds its nodes for every k-point, k-point by k-point, so
+// p = b TH_RUN nk + ik len(b) + (ie - b TH_RUN).  Pair counts stay below 2^31.
+__host__ __device__ inline void th_real_grouped(unsigned p, unsigned nk, unsigned nx, unsigned &ie, unsigned &ik)
+{
+    const unsigned full = (unsigned)TH_RUN * nk, nb = (nx + TH_RUN - 1) / TH_RUN;
+    const unsigned bq = p / full, b = bq < nb ? bq : nb - 1;
+    const unsigned off = p - b * full, rest = nx - (unsigned)TH_RUN * b;
+    const unsigned bl = rest < (unsigned)TH_RUN ? rest : (unsigned)TH_RUN;
+    ik = off / bl;
+    ie = (unsigned)TH_RUN * b + (off - ik * bl);
+}
+
 // eigenpairs of H(k + q) for the unique k-points of the work lists
 struct th_eig_set {
     double2 *V = nullptr;        // [n_kuniq][d][d] row-major, V[r][m] = component r of eigenvector m
@@ -220,7 +235,7 @@ struct th_green_args {
     double2 *raw = nullptr;   // optional: G itself, [np][d][d] row-major FP64
     const double *omega_p = nullptr;   // optional: omega in device memory (graph replays)
     hipStream_t stream = nullptr;      // optional: launch stream (default: the handle's)
-    int kmajor_re = 0, nx = 0;         // real axis: pairs k-major (p = ik * nx + ie) instead of node-major
+    int kmajor_re = 0, nx = 0;         // real axis: pairs grouped k-major (th_real_grouped) instead of node-major
     // panel rows in the contraction's summation order (with pair_order: the
     // runs of TH_RUN pairs per k-point stored transposed, see panel_row)
     int sum_order = 0;
@@ -236,6 +251,8 @@ int th_launch_green(th_handle *h, const th_green_args &a);
 // The Green function kernels of both sides of chi0(q) can write split
 // half-precision panels (th_green_args::presplit)
 bool th_green_presplit_ok(const th_handle *h, const double *q);
+// the eigenpairs of one side's k-points (qside: k + q), when ready for q
+const th_eig_set *th_side_eig(const th_handle *h, bool qside, const double *q);
 // Pairs that fill every resident workgroup of the fp16 spectral Green kernel
 // once (TH_RUN pairs per workgroup x compute units x workgroups per unit), 0
 // when the handle's d has no such kernel; batches of whole multiples leave no
