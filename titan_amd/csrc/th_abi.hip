@@ -167,6 +167,7 @@ int th_upload_nodes(th_handle *h, int n_x, const double *x2, const double *p2)
 // of H are computed once per unique k) and the maps to them.
 static int rebuild_kunion(th_handle *h)
 {
+    ++h->list_version;
     h->eig_valid[0] = h->eig_valid[1] = 0;
     std::unordered_map<kkey, int, kkey_hash> ids;
     std::vector<double> ku;

@@ -1262,11 +1262,15 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     // stored imaginary list keeps a k-point's pairs together in runs; real
     // pairs are taken grouped k-major, th_real_grouped: runs of up to TH_RUN
     // nodes of one k-point); their first pairs (indices of the whole list)
-    // and the end of the last, axis 0 then axis 1
-    std::vector<int> segv[2];
-    for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
+    // and the end of the last, axis 0 then axis 1; built once per pair lists,
+    // batch size and node count (a pass over every pair on the host)
+    std::vector<int>(&segv)[2] = h->adj_seg;
+    const long skey[3] = {h->list_version, bp, real_axis ? (long)n_x : 0};
+    for (int axis = 0; axis < 2 && !std::equal(skey, skey + 3, h->adj_seg_key); ++axis) {
         const long npairs = axis == 0 ? h->n_im : n_re;
         std::vector<int> &sv = segv[axis];
+        sv.clear();
+        if (axis == 1 && !real_axis) break;
         sv.reserve((size_t)npairs / 8 + 2);
         unsigned ik_prev = 0;
         for (long p = 0; p < npairs; ++p) {
@@ -1283,6 +1287,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
         }
         sv.push_back((int)npairs);
     }
+    std::copy(skey, skey + 3, h->adj_seg_key);
     const size_t seg1 = segv[0].size();
     TH_HIP(hipMemcpyAsync(d_seg, segv[0].data(), sizeof(int) * seg1, hipMemcpyHostToDevice, h->stream));
     if (real_axis)

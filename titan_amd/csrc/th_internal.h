@@ -122,6 +122,11 @@ struct th_handle {
     th_eig_set eig[2];           // [0]: q = 0, [1]: the last q != 0
     int eig_valid[2] = {0, 0};
     std::vector<int> kid_im_local;        // imaginary point -> index in kim_uniq
+    long list_version = 0;                // bumped whenever the imaginary list or the real mesh is set
+    // the adjoint's segments (runs of pairs on one k-point within a batch) per
+    // axis, for the key {list_version, pairs per batch, real-axis nodes}
+    std::vector<int> adj_seg[2];
+    long adj_seg_key[3] = {-1, -1, -1};
     std::vector<double> kim_uniq, kre_host;
     std::vector<double> yim_host, wim_host;   // imaginary list in stored order
     double wre_absmax = 0.0;     // max |w| of the real k-mesh
