@@ -23,6 +23,7 @@
 // bitwise reproducible.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "th_internal.h"
@@ -741,33 +742,50 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
                 gp[bidx] = make_double2(x * n, ei * n);
             }
             const double *H = a.hat + (size_t)pl * a.nA2 * M2;
+            // offsets in the pair's cotangent: ox the entry read, oy the
+            // transposed one of the real-axis A side; one atom of nine spin
+            // orbitals (D = 18, Mpad = 336): block (0,0), entry [r][c] at r D + c
+            auto stage = [&](auto one_atom) {
+                constexpr bool ONE = decltype(one_atom)::value;
+                const int mp = ONE ? 336 : Mp;
 #pragma unroll 1
-            for (int i0 = 0; i0 < NE; i0 += 3) {
-                double2 hv[3], yv[3];
+                for (int i0 = 0; i0 < NE; i0 += 3) {
+                    double2 hv[3], yv[3];
 #pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    const int f = bidx + (i0 + u) * NBLK, r = f / D, c = f - (f / D) * D;
-                    const int ra = (int)(((float)r + 0.5f) * rbs), ca = (int)(((float)c + 0.5f) * rbs);
-                    const int rr = r - ra * bs, cc = c - ca * bs;
-                    const int ox = mode == TH_G_IMAG_B ? (ca * nA + ra) * M2 + rr * bs + cc
-                                 : mode == TH_G_REAL_B ? (ra * nA + ca) * M2 + cc * bs + rr
-                                 : (ra * nA + ca) * M2 + rr * bs + cc;
-                    const int oy = (ca * nA + ra) * M2 + cc * bs + rr;
-                    hv[u] = make_double2(H[ox], H[ox + Mp]);
-                    yv[u] = mode == TH_G_REAL_A ? make_double2(H[oy], H[oy + Mp]) : make_double2(0.0, 0.0);
-                }
+                    for (int u = 0; u < 3; ++u) {
+                        const int f = bidx + (i0 + u) * NBLK, r = f / D, c = f - (f / D) * D;
+                        int ox, oy;
+                        if constexpr (ONE) {
+                            oy = c * D + r;
+                            ox = mode == TH_G_REAL_B ? oy : f;
+                        } else {
+                            const int ra = (int)(((float)r + 0.5f) * rbs), ca = (int)(((float)c + 0.5f) * rbs);
+                            const int rr = r - ra * bs, cc = c - ca * bs;
+                            ox = mode == TH_G_IMAG_B ? (ca * nA + ra) * M2 + rr * bs + cc
+                               : mode == TH_G_REAL_B ? (ra * nA + ca) * M2 + cc * bs + rr
+                               : (ra * nA + ca) * M2 + rr * bs + cc;
+                            oy = (ca * nA + ra) * M2 + cc * bs + rr;
+                        }
+                        hv[u] = make_double2(H[ox], H[ox + mp]);
+                        yv[u] = mode == TH_G_REAL_A ? make_double2(H[oy], H[oy + mp]) : make_double2(0.0, 0.0);
+                    }
 #pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    double2 v = hv[u];
-                    if (mode == TH_G_REAL_A)             // Gqhat = i w (Ahat + Ahat^H)
-                        v = make_double2(-w * (v.y - yv[u].y), w * (v.x + yv[u].x));
-                    else if (mode == TH_G_IMAG_A)        // Gqhat = w Ahat
-                        v = make_double2(w * v.x, w * v.y);
-                    else if (mode == TH_G_REAL_B)        // Ghat(i,j)[cc][rr] = conj(Bhat(i,j)[rr][cc])
-                        v.y = -v.y;
-                    Gh[bidx + (i0 + u) * NBLK] = c2{(R)v.x, (R)v.y};
+                    for (int u = 0; u < 3; ++u) {
+                        double2 v = hv[u];
+                        if (mode == TH_G_REAL_A)             // Gqhat = i w (Ahat + Ahat^H)
+                            v = make_double2(-w * (v.y - yv[u].y), w * (v.x + yv[u].x));
+                        else if (mode == TH_G_IMAG_A)        // Gqhat = w Ahat
+                            v = make_double2(w * v.x, w * v.y);
+                        else if (mode == TH_G_REAL_B)        // Ghat(i,j)[cc][rr] = conj(Bhat(i,j)[rr][cc])
+                            v.y = -v.y;
+                        Gh[bidx + (i0 + u) * NBLK] = c2{(R)v.x, (R)v.y};
+                    }
                 }
-            }
+            };
+            if (D == 18 && nA == 1 && Mp == 336)
+                stage(std::true_type{});
+            else
+                stage(std::false_type{});
         }
         __syncthreads();
         // X = Ghat V: X[r][c] = sum_m Ghat[r][m] V[m][c]
@@ -1055,13 +1073,17 @@ __global__ void spin_sum_kernel(long np, int nA, int nOrb, const double2 *__rest
 }
 
 // acc[e] += sum_ch partial[ch][e], fixed order
-__global__ void hsum_kernel(const double2 *partial, int nch, int nout, double2 *acc)
+// acc[e] += sum over the nch chunks of partial[ch][e]: one wave per output,
+// lane l takes the chunks l, l + 64, ... and the lanes' sums are combined by a
+// fixed butterfly (the same order on every call)
+__global__ __launch_bounds__(256) void hsum_kernel(const double2 *partial, int nch, int nout, double2 *acc)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (e >= nout) return;
-    double2 s = acc[e];
-    for (int ch = 0; ch < nch; ++ch) s = zadd(s, partial[(size_t)ch * nout + e]);
-    acc[e] = s;
+    double2 s = make_double2(0.0, 0.0);
+    for (int ch = lane; ch < nch; ch += 64) s = zadd(s, partial[(size_t)ch * nout + e]);
+    for (int o = 32; o > 0; o >>= 1) s = make_double2(s.x + __shfl_xor(s.x, o), s.y + __shfl_xor(s.y, o));
+    if (lane == 0) acc[e] = zadd(acc[e], s);
 }
 
 __global__ void zero2_kernel(double2 *p, size_t n)
@@ -1231,6 +1253,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     need += ((size_t)bp * h->nA2 * nO * nO * sizeof(double2) + 255) & ~(size_t)255;
     need += ((size_t)3 * (h->nhop + h->nA2) * sizeof(int) + 255) & ~(size_t)255;   // hop groups
     need += ((size_t)(h->n_im + n_re + 2) * sizeof(int) + 255) & ~(size_t)255;     // segments of both axes
+    need += (((size_t)bp / TH_RUN + 1) * sizeof(float) + 255) & ~(size_t)255;     // A panel maxima per workgroup
     need += 4096;
     if (need > h->adj_bytes) {
         hipFree(h->d_adj);
@@ -1257,6 +1280,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     double2 *d_S = ar.take<double2>((size_t)bp * h->nA2 * nO * nO);
     double2 *d_Uhat = U_hat ? ar.take<double2>(DD) : nullptr;
     int *d_seg = ar.take<int>((size_t)(h->n_im + n_re + 2));
+    float *d_bmax = ar.take<float>((size_t)bp / TH_RUN + 1);
 
     // segments: runs of consecutive pairs on one k-point within a batch (the
     // stored imaginary list keeps a k-point's pairs together in runs; real
@@ -1371,7 +1395,10 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             a.mode = axis == 0 ? TH_G_IMAG_A : TH_G_REAL_A;
             a.panel = h->d_pA;
             a.raw = spec_h ? nullptr : rawA;
+            a.blkmax = precision != TH_PREC_FP64 ? d_bmax : nullptr;   // the A panel's scale for the fp16 GEMM
             if ((rc = th_launch_green(h, a))) return rc;
+            a.blkmax = nullptr;
+            const long nbmax = h->last_blkmax;
             // the imaginary-axis B side does not depend on omega: when the forward's
             // panel cache holds it (same E_F, eta', precision), read G from there
             const void *pB = h->d_pB;
@@ -1405,7 +1432,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA, brows,
                                               axis == 0 && use_bcache ? split_e : INT_MIN)))
                 return rc;
-            if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB))) return rc;
+            if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB, th_rowmap(), INT_MIN, d_bmax,
+                                              nbmax)))
+                return rc;
             h->n_launch[1] += 2;
             tm.end(id);
             id = tm.begin(2);
@@ -1509,7 +1538,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                     hipLaunchKernelGGL(hop_reduce_kernel, dim3(nblk((size_t)ngroups * nO * nO, 64), nch), dim3(256), 0,
                                        h->stream, hp, (const int *)d_groups, ngroups);
                 hipLaunchKernelGGL(h0_reduce_kernel, dim3(nblk(dd, 64), nch), dim3(256), 0, h->stream, hp);
-                hipLaunchKernelGGL(hsum_kernel, dim3(nblk(nout, 256)), dim3(256), 0, h->stream, (const double2 *)d_part,
+                hipLaunchKernelGGL(hsum_kernel, dim3(nblk(nout, 4)), dim3(256), 0, h->stream, (const double2 *)d_part,
                                    nch, nout, d_grad);
                 TH_HIP(hipGetLastError());
             }

@@ -948,6 +948,14 @@ __global__ __launch_bounds__(256) void absmax_f_kernel(const float *__restrict__
     block_max_to(m, mx);
 }
 
+// mx[0] = the bits of the largest of n non-negative floats (one workgroup)
+__global__ __launch_bounds__(256) void max_of_kernel(const float *__restrict__ x, long n, unsigned *__restrict__ mx)
+{
+    float m = 0.0f;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) m = fmaxf(m, x[e]);
+    block_max_to(m, mx);
+}
+
 __global__ __launch_bounds__(256) void absmax_d_kernel(const double *__restrict__ x, size_t n, unsigned *__restrict__ mx)
 {
     float m = 0.0f;
@@ -1304,7 +1312,7 @@ int th_launch_finalize(th_handle *h, double2 *chi0)
 }
 
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
-                            double *out, const th_rowmap &rows, int split_e)
+                            double *out, const th_rowmap &rows, int split_e, const float *blkmax, long nblkmax)
 {
     const float unsplit = split_e == INT_MIN ? 0.0f : ldexpf(1.0f, -split_e);
     if (np <= 0) return TH_OK;
@@ -1322,7 +1330,10 @@ int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const dou
         unsigned *mx = static_cast<unsigned *>(h->d_pamax);
         hipLaunchKernelGGL(zero_u_kernel, dim3(1), dim3(64), 0, h->stream, mx, 1);
         const size_t ps = (size_t)h->nA2 * 2 * h->Mpad;
-        if (split_e == INT_MIN) {
+        if (split_e == INT_MIN && blkmax && nblkmax > 0) {
+            // the largest entry from the Green kernel's per-workgroup maxima
+            hipLaunchKernelGGL(max_of_kernel, dim3(1), dim3(256), 0, h->stream, blkmax, nblkmax, mx);
+        } else if (split_e == INT_MIN) {
             // the rows the batch reads: pg0 .. pg0 + np - 1, or through a row map
             // the whole cache batches that hold them (a wider set, still a bound)
             long r0 = rows.pg0, r1 = rows.pg0 + np;

@@ -66,6 +66,7 @@ struct green_params {
     const double *eigL;
     const _Float16 *eigS;        // the split planes of eigV (th_eig_set::S), when kept
     const int *kid_im, *kid_re;
+    float *blkmax;               // th_green_args::blkmax
 };
 
 // Panel row of batch pair b = slot0 + pl.  With nrun_out = n > 0 the first
@@ -1522,6 +1523,7 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
     const int eP = (g.presplit && g.pareal > 0.0) ? th_real_exp(g.p2, g.pnx, g.pareal) : g.pexp;
     long cur = -1;
     float avr[KT][8], avi[KT][8];
+    float bmax = 0.0f;                          // largest |entry| written (FP32 panels)
     __syncthreads();                            // the pairs' scalars
     for (long pl = s0; pl < s1; ++pl) {
         const int jj = (int)(pl - s0);
@@ -1668,13 +1670,25 @@ __global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_
 #pragma unroll
             for (int j = 0; j < NB; ++j)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    put(j, q, __builtin_bit_cast(unsigned, (acp[j][q] - acn[j][q]) * back),
-                        __builtin_bit_cast(unsigned, aci[j][q] * back));
+                for (int q = 0; q < 4; ++q) {
+                    const float vr = (acp[j][q] - acn[j][q]) * back, vi = aci[j][q] * back;
+                    bmax = fmaxf(bmax, fmaxf(fabsf(vr), fabsf(vi)));   // padding entries hold zeros
+                    put(j, q, __builtin_bit_cast(unsigned, vr), __builtin_bit_cast(unsigned, vi));
+                }
         }
         for (int e = tid; e < g.nA2 * 2 * npad; e += NT) {
             const int part = e / npad;
             __builtin_amdgcn_raw_buffer_store_b32(0u, ro, 4u * (unsigned)(part * Mpad + Mb + (e - part * npad)), 0, 0);
+        }
+    }
+    if (g.blkmax && !g.presplit) {              // the workgroup's largest |entry|
+        __shared__ float wmax[NB];
+        for (int o = 32; o > 0; o >>= 1) bmax = fmaxf(bmax, __shfl_xor(bmax, o));
+        if (lane == 0) wmax[wave] = bmax;
+        __syncthreads();
+        if (tid == 0) {
+            for (int w2 = 1; w2 < NB; ++w2) bmax = fmaxf(bmax, wmax[w2]);
+            g.blkmax[blockIdx.x] = bmax;
         }
     }
 }
@@ -2132,6 +2146,7 @@ int launch_green_spectral(th_handle *h, const th_green_args &a, const green_para
                 return TH_E_ARG;
             }
             hipLaunchKernelGGL((green_spectral_h2_kernel<DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
+            if (g.blkmax && !a.presplit) h->last_blkmax = nblk;
         } else {
             hipLaunchKernelGGL((green_spectral_chunk_kernel<P, DD, CH>), dim3(nblk), dim3(NT), 0, gstream(h, a), g);
         }
@@ -2333,6 +2348,8 @@ int th_launch_green(th_handle *h, const th_green_args &a)
     g.eigS = es ? reinterpret_cast<const _Float16 *>(es->S) : nullptr;
     g.kid_im = h->d_kid_im;
     g.kid_re = h->d_kid_re;
+    g.blkmax = a.blkmax;
+    h->last_blkmax = 0;
     h->n_launch[0] += 1;
     if (a.prec == TH_PREC_FP64) return launch_green_p<double>(h, a, g);
     return launch_green_p<float>(h, a, g);

@@ -123,6 +123,7 @@ struct th_handle {
     int eig_valid[2] = {0, 0};
     std::vector<int> kid_im_local;        // imaginary point -> index in kim_uniq
     long list_version = 0;                // bumped whenever the imaginary list or the real mesh is set
+    long last_blkmax = 0;                 // workgroup maxima the last th_launch_green wrote (0: none)
     // the adjoint's segments (runs of pairs on one k-point within a batch) per
     // axis, for the key {list_version, pairs per batch, real-axis nodes}
     std::vector<int> adj_seg[2];
@@ -251,6 +252,9 @@ struct th_green_args {
     // pareal max |p2[0 .. pnx)| instead
     int presplit = 0, pexp = 0, pnx = 0;
     double pareal = 0.0;
+    // optional, FP32 panels: each workgroup's largest |entry| of what it wrote,
+    // [np / TH_RUN rounded up]; kernels that support it set th_handle::last_blkmax
+    float *blkmax = nullptr;
 };
 int th_launch_green(th_handle *h, const th_green_args &a);
 // The Green function kernels of both sides of chi0(q) can write split
@@ -327,8 +331,11 @@ int th_launch_finalize(th_handle *h, double2 *chi0);
 // (trans = 1) or conj(panel[p]) . That (trans = 0) per atom pair, FP64 out
 // [np][nA2][2][Mpad]; That is [nA2][2][Mpad][Mpad] FP64.
 // split_e != INT_MIN: the panel holds split (hi, lo) words of x 2^split_e (FP32 only)
+// blkmax / nblkmax: the panel's largest |entry| as per-workgroup maxima from
+// the Green kernel that wrote it (instead of a pass over the panel)
 int th_launch_panel_adjoint(th_handle *h, const void *panel, int prec, const double *That, long np, int trans,
-                            double *out, const th_rowmap &rows = th_rowmap(), int split_e = INT_MIN);
+                            double *out, const th_rowmap &rows = th_rowmap(), int split_e = INT_MIN,
+                            const float *blkmax = nullptr, long nblkmax = 0);
 // For the fp16 panel adjoint (mixed precision): That of one axis split once
 // into the fp16 layouts of both transpositions before the axis's batches
 // (its largest |That| kept on the device); later launches with this That use
