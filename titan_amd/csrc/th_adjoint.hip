@@ -843,9 +843,10 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    const double2 f = zmul(gr[i], gc[j]);
+                    const double fr = fma(gr[i].x, gc[j].x, -gr[i].y * gc[j].y);
+                    const double fi = fma(gr[i].x, gc[j].y, gr[i].y * gc[j].x);
                     const double yr = (double)acc[i][j].x, yi = (double)acc[i][j].y;
-                    tot[i][j] = make_double2(tot[i][j].x + f.x * yr - f.y * yi, tot[i][j].y + f.x * yi + f.y * yr);
+                    tot[i][j] = make_double2(fma(fr, yr, fma(-fi, yi, tot[i][j].x)), fma(fr, yi, fma(fi, yr, tot[i][j].y)));
                 }
         }
     }
@@ -875,7 +876,10 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
         double2 v = make_double2(0.0, 0.0);
         if (e < DD)
 #pragma unroll 3
-            for (int m = 0; m < D; ++m) v = zadd(v, zmul(Ws[r * D + m], zconj(V64[c * D + m])));
+            for (int m = 0; m < D; ++m) {   // v += W[r][m] conj(V[c][m])
+                const double2 x = Ws[r * D + m], y = V64[c * D + m];
+                v = make_double2(fma(x.x, y.x, fma(x.y, y.y, v.x)), fma(x.y, y.x, fma(-x.x, y.y, v.y)));
+            }
         u[k] = v;
     }
     __syncthreads();
@@ -891,7 +895,10 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
         if (e < DD) {
             double2 v = make_double2(0.0, 0.0);
 #pragma unroll 3
-            for (int m = 0; m < D; ++m) v = zadd(v, zmul(V64[r * D + m], Ws[m * D + c]));
+            for (int m = 0; m < D; ++m) {   // v += V[r][m] U[m][c]
+                const double2 x = V64[r * D + m], y = Ws[m * D + c];
+                v = make_double2(fma(x.x, y.x, fma(-x.y, y.y, v.x)), fma(x.x, y.y, fma(x.y, y.x, v.y)));
+            }
             out[(size_t)blockIdx.x * DD + e] = v;
         }
     }
@@ -1464,7 +1471,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                     const int nblk3 = (d / 3) * (d / 3);
                     int ppw = std::max(1, 512 / nblk3);
                     while (ppw > 1 && hspec_lds(d, ppw, f32) > 64 * 1024) --ppw;
-                    if (d == 18) ppw = std::min(ppw, f32 ? 8 : 4);   // runs of 16 and 64 pairs in whole groups
+                    if (d == 18) ppw = std::min(ppw, 4);   // segments of 16 pairs: four pairs per thread slot
                     const int nt = ((ppw * nblk3 + 63) / 64) * 64;
                     const size_t lds = hspec_lds(d, ppw, f32);
                     auto launch = [&](auto kern) {
