@@ -199,6 +199,8 @@ static int rebuild_kunion(th_handle *h)
         TH_HIP(hipMalloc(&h->d_kid_re, sizeof(int) * kid_re.size()));
         TH_HIP(hipMemcpy(h->d_kid_re, kid_re.data(), sizeof(int) * kid_re.size(), hipMemcpyHostToDevice));
     }
+    h->kid_im_host = kid_im;
+    h->kid_re_host = kid_re;
     // imaginary pairs (stored order) and real-mesh points of each unique k-point
     hipFree(h->d_kim_ptr); hipFree(h->d_kim_idx); hipFree(h->d_kre_ptr); hipFree(h->d_kre_idx);
     h->d_kim_ptr = h->d_kim_idx = h->d_kre_ptr = h->d_kre_idx = nullptr;
@@ -1117,7 +1119,7 @@ int th_destroy(th_handle *h)
                     h->d_wre, h->d_x2, h->d_p2, h->d_pA, h->d_pB, h->d_pA2, h->d_pB2, h->d_part, h->d_acc, h->d_chi0, h->d_chi,
                     h->d_bcache, h->d_sup, h->d_U, h->d_W, h->d_V, h->d_Z, h->d_X, h->d_adj, h->d_omega, h->d_norms, h->d_blk, h->d_kuniq, h->d_kid_im, h->d_kid_re,
                     h->eig[0].V, h->eig[0].L, h->eig[1].V, h->eig[1].L, h->eig[0].S, h->eig[1].S, h->d_kim_ptr, h->d_kim_idx,
-                    h->d_kre_ptr, h->d_kre_idx, h->d_ref_int, h->d_ref_part, h->d_pamax, h->d_pays};
+                    h->d_kre_ptr, h->d_kre_idx, h->d_ref_int, h->d_ref_part, h->d_pamax, h->d_pays, h->d_wk};
     for (void *b : bufs)
         if (b) hipFree(b);
     {

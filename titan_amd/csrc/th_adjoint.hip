@@ -664,8 +664,11 @@ __global__ __launch_bounds__(512) void hhat_seg_d_kernel(assemble_params a, cons
 // so no pair's G is formed or read: the workgroup stages V and lambda once,
 // per pair only Ghat_p and conj g_p.  Y_p = V^H (Ghat_p V) runs in R with a
 // thread's 3 x 3 register block as in hhat_seg_d_kernel; W sums the scaled
-// Y_p in FP64; the slots' W are added in order of slot, and V W V^H is formed
-// in FP64 from the FP64 eigenvectors by every thread, an entry at a time.
+// Y_p in FP64, the slots' W are added in order of slot, and the result is
+// added to the k-point's eigenbasis sum Wk[k] (segments of one k-point share V
+// and the phases, whichever axis they belong to); hhat_final_kernel forms
+// V Wk V^H once per k-point after the last batch.  One workgroup per segment of
+// the list sidx, whose segments have distinct k-points.
 struct hspec_params {
     const double2 *V;        // [nk][D][D] row-major, V[r][m] = component r of eigenvector m
     const double *L;         // [nk][D]
@@ -676,8 +679,8 @@ struct hspec_params {
 
 template <typename R, int D>
 __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec_params sp,
-                                                        const int *__restrict__ seg, int ppw,
-                                                        double2 *__restrict__ out)
+                                                        const int *__restrict__ seg, const int *__restrict__ sidx,
+                                                        int ppw, double2 *__restrict__ Wk)
 {
     typedef typename c2of<R>::T c2;
     constexpr int DD = D * D, NB3 = D / 3, NBLK = NB3 * NB3, NE = DD / NBLK;
@@ -691,8 +694,9 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
     c2 *slots = reinterpret_cast<c2 *>(gs + (size_t)ppw * D);
     const int tid = threadIdx.x;
     const bool imag = a.mode == TH_G_IMAG_A || a.mode == TH_G_IMAG_B;
-    const int pfirst = seg[blockIdx.x];
-    const long q0 = pfirst - a.p0, q1 = seg[blockIdx.x + 1] - a.p0;   // the batch's pairs
+    const int sg = sidx[blockIdx.x];                                   // the segment in the batch
+    const int pfirst = seg[sg];
+    const long q0 = pfirst - a.p0, q1 = seg[sg + 1] - a.p0;             // the batch's pairs
     const int s = tid / NBLK, bidx = tid - s * NBLK;
     const bool slot = s < ppw;
     const int r0 = 3 * (bidx / NB3), c0 = 3 * (bidx - (bidx / NB3) * NB3);
@@ -863,40 +867,56 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
                 }
     }
     __syncthreads();
-    // U = W V^H, then Hhat = V U, in FP64 with V in FP64 in the slot space,
-    // entries e = tid + k blockDim.x over every thread
-    constexpr int NU = (DD + 64 * ((NBLK + 63) / 64) - 1) / (64 * ((NBLK + 63) / 64));   // fewest threads: one slot
-    double2 *V64 = reinterpret_cast<double2 *>(slots);
-    for (int e = tid; e < DD; e += blockDim.x) V64[e] = Vk[e];
+    // added to the k-point's sum (no other workgroup of the launch has this
+    // k-point; launches follow the segments' order)
+    double2 *o = Wk + (size_t)kb * DD;
+    for (int e = tid; e < DD; e += blockDim.x) o[e] = zadd(o[e], Ws[e]);
+}
+
+// Hhat_k = V_k W_k V_k^H for k-points k0 .. k0 + gridDim.x - 1 (the sums the
+// spectral kernel accumulated in the eigenbasis), FP64, into out[k - k0]
+template <int D>
+__global__ __launch_bounds__(256) void hhat_final_kernel(const double2 *__restrict__ Wk, const double2 *__restrict__ V,
+                                                         long k0, double2 *__restrict__ out)
+{
+    constexpr int DD = D * D, NU = (DD + 255) / 256;
+    extern __shared__ __align__(16) unsigned char smem[];   // W | V, 2 D^2 FP64 complex
+    double2 *Ws = reinterpret_cast<double2 *>(smem), *Vs = Ws + DD;
+    const long k = k0 + blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < DD; e += 256) {
+        Ws[e] = Wk[(size_t)k * DD + e];
+        Vs[e] = V[(size_t)k * DD + e];
+    }
     __syncthreads();
     double2 u[NU];
 #pragma unroll
-    for (int k = 0; k < NU; ++k) {
-        const int e = tid + k * (int)blockDim.x, r = e / D, c = e - (e / D) * D;
+    for (int q = 0; q < NU; ++q) {
+        const int e = tid + 256 * q, r = e / D, c = e - (e / D) * D;
         double2 v = make_double2(0.0, 0.0);
         if (e < DD)
 #pragma unroll 3
             for (int m = 0; m < D; ++m) {   // v += W[r][m] conj(V[c][m])
-                const double2 x = Ws[r * D + m], y = V64[c * D + m];
+                const double2 x = Ws[r * D + m], y = Vs[c * D + m];
                 v = make_double2(fma(x.x, y.x, fma(x.y, y.y, v.x)), fma(x.y, y.x, fma(-x.x, y.y, v.y)));
             }
-        u[k] = v;
+        u[q] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NU; ++k) {
-        const int e = tid + k * (int)blockDim.x;
-        if (e < DD) Ws[e] = u[k];
+    for (int q = 0; q < NU; ++q) {
+        const int e = tid + 256 * q;
+        if (e < DD) Ws[e] = u[q];
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NU; ++k) {
-        const int e = tid + k * (int)blockDim.x, r = e / D, c = e - (e / D) * D;
+    for (int q = 0; q < NU; ++q) {
+        const int e = tid + 256 * q, r = e / D, c = e - (e / D) * D;
         if (e < DD) {
             double2 v = make_double2(0.0, 0.0);
 #pragma unroll 3
             for (int m = 0; m < D; ++m) {   // v += V[r][m] U[m][c]
-                const double2 x = V64[r * D + m], y = Ws[m * D + c];
+                const double2 x = Vs[r * D + m], y = Ws[m * D + c];
                 v = make_double2(fma(x.x, y.x, fma(-x.y, y.y, v.x)), fma(x.x, y.y, fma(x.y, y.x, v.y)));
             }
             out[(size_t)blockIdx.x * DD + e] = v;
@@ -904,12 +924,11 @@ __global__ __launch_bounds__(512) void hhat_spec_kernel(assemble_params a, hspec
     }
 }
 
-// LDS bytes of hhat_spec_kernel (the slot space also holds V in FP64 at the end)
+// LDS bytes of hhat_spec_kernel
 inline size_t hspec_lds(int D, int ppw, bool f32)
 {
     const size_t c2 = f32 ? sizeof(float2) : sizeof(double2), dd = (size_t)D * D;
-    return dd * (c2 + sizeof(double2)) + sizeof(double) * D + sizeof(double2) * ppw * D +
-           std::max(c2 * ppw * dd, sizeof(double2) * dd);
+    return dd * (c2 + sizeof(double2)) + sizeof(double) * D + sizeof(double2) * ppw * D + c2 * ppw * dd;
 }
 
 // Hhat of the batch's pairs summed over each segment (the generic path)
@@ -971,6 +990,21 @@ __global__ void phase_kernel(const int *seg, int nseg, int nhop, int imag, int s
     if (side_q) { k0 += q0; k1 += q1; k2 += q2; }
     double sn, cs;
     sincos(k0 * R[3 * n] + k1 * R[3 * n + 1] + k2 * R[3 * n + 2], &sn, &cs);
+    ph[e] = make_double2(cs, -sn);
+}
+
+// exp(-i k.R_n) for the unique k-points k0 .. k0 + nk - 1 (plus q on the q side)
+__global__ void phase_k_kernel(const double *kuniq, long k0, int nk, int nhop, int side_q, double q0, double q1,
+                               double q2, const double *R, double2 *ph)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nk * nhop) return;
+    const int i = (int)(e / nhop), n = (int)(e - (size_t)i * nhop);
+    const double *k = kuniq + 3 * (size_t)(k0 + i);
+    double k0_ = k[0], k1 = k[1], k2 = k[2];
+    if (side_q) { k0_ += q0; k1 += q1; k2 += q2; }
+    double sn, cs;
+    sincos(k0_ * R[3 * n] + k1 * R[3 * n + 1] + k2 * R[3 * n + 2], &sn, &cs);
     ph[e] = make_double2(cs, -sn);
 }
 
@@ -1259,7 +1293,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     need += ((size_t)bp * std::max(1, h->nhop) * sizeof(double2) + 255) & ~(size_t)255;
     need += ((size_t)bp * h->nA2 * nO * nO * sizeof(double2) + 255) & ~(size_t)255;
     need += ((size_t)3 * (h->nhop + h->nA2) * sizeof(int) + 255) & ~(size_t)255;   // hop groups
-    need += ((size_t)(h->n_im + n_re + 2) * sizeof(int) + 255) & ~(size_t)255;     // segments of both axes
+    need += 2 * (((size_t)(h->n_im + n_re + 2) * sizeof(int) + 255) & ~(size_t)255);   // segments, their order
     need += (((size_t)bp / TH_RUN + 1) * sizeof(float) + 255) & ~(size_t)255;     // A panel maxima per workgroup
     need += 4096;
     if (need > h->adj_bytes) {
@@ -1287,6 +1321,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     double2 *d_S = ar.take<double2>((size_t)bp * h->nA2 * nO * nO);
     double2 *d_Uhat = U_hat ? ar.take<double2>(DD) : nullptr;
     int *d_seg = ar.take<int>((size_t)(h->n_im + n_re + 2));
+    int *d_sord = ar.take<int>((size_t)(h->n_im + n_re + 2));
     float *d_bmax = ar.take<float>((size_t)bp / TH_RUN + 1);
 
     // segments: runs of consecutive pairs on one k-point within a batch (the
@@ -1318,12 +1353,63 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
         }
         sv.push_back((int)npairs);
     }
+    if (!std::equal(skey, skey + 3, h->adj_seg_key)) {
+        // rounds: per batch, a segment goes to the round after the last one
+        // holding a segment of its k-point (unique k-point index)
+        std::vector<int> cnt((size_t)h->n_kuniq, 0), kb, rr;
+        for (int axis = 0; axis < 2; ++axis) {
+            h->adj_sord[axis].clear();
+            h->adj_roff[axis].clear();
+            if (axis == 1 && !real_axis) break;
+            const long npairs = axis == 0 ? h->n_im : n_re;
+            const std::vector<int> &sv = segv[axis];
+            std::vector<int> &so = h->adj_sord[axis];
+            size_t b0 = 0;
+            for (long p0 = 0; p0 < npairs; p0 += bp) {
+                const long np = std::min(bp, npairs - p0);
+                size_t b1 = b0;
+                while (sv[b1] < p0 + np) ++b1;
+                const int ns = (int)(b1 - b0);
+                kb.assign(ns, 0);
+                rr.assign(ns, 0);
+                int nr = 0;
+                for (int i = 0; i < ns; ++i) {
+                    const long pf = sv[b0 + i];
+                    if (axis == 0) {
+                        kb[i] = h->kid_im_host[pf];
+                    } else {
+                        unsigned ie, ik;
+                        th_real_grouped((unsigned)pf, (unsigned)h->nk_re, (unsigned)n_x, ie, ik);
+                        kb[i] = h->kid_re_host[ik];
+                    }
+                    rr[i] = cnt[kb[i]]++;
+                    nr = std::max(nr, rr[i] + 1);
+                }
+                for (int i = 0; i < ns; ++i) cnt[kb[i]] = 0;
+                std::vector<long> off(nr + 1, 0);
+                for (int i = 0; i < ns; ++i) ++off[rr[i] + 1];
+                for (int r = 0; r < nr; ++r) off[r + 1] += off[r];
+                std::vector<long> fill(off.begin(), off.end() - 1);
+                const size_t base = so.size();
+                so.resize(base + ns);
+                for (int i = 0; i < ns; ++i) so[base + fill[rr[i]]++] = i;
+                for (long &o : off) o += (long)base;
+                h->adj_roff[axis].push_back(std::move(off));
+                b0 = b1;
+            }
+        }
+    }
     std::copy(skey, skey + 3, h->adj_seg_key);
     const size_t seg1 = segv[0].size();
     TH_HIP(hipMemcpyAsync(d_seg, segv[0].data(), sizeof(int) * seg1, hipMemcpyHostToDevice, h->stream));
-    if (real_axis)
+    TH_HIP(hipMemcpyAsync(d_sord, h->adj_sord[0].data(), sizeof(int) * h->adj_sord[0].size(), hipMemcpyHostToDevice,
+                          h->stream));
+    if (real_axis) {
         TH_HIP(hipMemcpyAsync(d_seg + seg1, segv[1].data(), sizeof(int) * segv[1].size(), hipMemcpyHostToDevice,
                               h->stream));
+        TH_HIP(hipMemcpyAsync(d_sord + seg1, h->adj_sord[1].data(), sizeof(int) * h->adj_sord[1].size(),
+                              hipMemcpyHostToDevice, h->stream));
+    }
 
     // hop groups: at most HG hops of one atom pair (stored hops are grouped by pair)
     std::vector<int> hoff(h->nA2 + 1), grp;
@@ -1374,9 +1460,51 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     // split words in the cache are read back as (hi + lo) 2^-e
     const int split_e = (use_bcache && h->bcache_presplit) ? h->bcache_pexp : INT_MIN;
     const int ntile = (d + 31) / 32;
+    // the hop and h0 cotangents of n segment (or k-point) sums Hs, whose
+    // phases are in d_ph: spin sums, chunk partials, then added to d_grad
+    auto hop_h0_reduce = [&](const double2 *Hs, int n) -> int {
+        hreduce_params hp{};
+        hp.np = n; hp.nk_re = h->nk_re;
+        hp.chunk = CHUNK; hp.nout = nout; hp.nhop = h->nhop; hp.nOrb = nO; hp.d = d;
+        hp.q0 = q[0]; hp.q1 = q[1]; hp.q2 = q[2];
+        hp.kim = h->d_kim; hp.kre = h->d_kre; hp.R = h->d_R; hp.jb = h->d_jb; hp.ib = h->d_ib;
+        hp.Hh = Hs;
+        hp.nA = h->nA;
+        hp.S = d_S;
+        hp.ph = d_ph;
+        hp.partial = d_part;
+        hipLaunchKernelGGL(spin_sum_kernel, dim3(nblk((size_t)n * h->nA2 * nO * nO, 256)), dim3(256), 0, h->stream,
+                           (long)n, h->nA, nO, Hs, d_S);
+        const int nch = (n + CHUNK - 1) / CHUNK;
+        if (ngroups > 0)
+            hipLaunchKernelGGL(hop_reduce_kernel, dim3(nblk((size_t)ngroups * nO * nO, 64), nch), dim3(256), 0,
+                               h->stream, hp, (const int *)d_groups, ngroups);
+        hipLaunchKernelGGL(h0_reduce_kernel, dim3(nblk(dd, 64), nch), dim3(256), 0, h->stream, hp);
+        hipLaunchKernelGGL(hsum_kernel, dim3(nblk(nout, 4)), dim3(256), 0, h->stream, (const double2 *)d_part, nch,
+                           nout, d_grad);
+        TH_HIP(hipGetLastError());
+        return TH_OK;
+    };
     // Hhat in the spectral form when both sides' eigenpairs are ready (then
     // neither side's G is formed: the Green stage writes panels only)
     const bool spec_h = (d == 18 || d == 36 || d == 54) && th_side_eig(h, true, q) && th_side_eig(h, false, q);
+    // spectral form: per side and unique k-point the eigenbasis sum W_k (FP64),
+    // and per batch its segments in rounds of distinct k-points (launch after
+    // launch, so that the sums add in the segments' order without races)
+    const size_t wk_side = (size_t)h->n_kuniq * dd;
+    std::vector<int>(&sord)[2] = h->adj_sord;
+    std::vector<std::vector<long>>(&roff)[2] = h->adj_roff;
+    if (spec_h) {
+        const size_t wb = 2 * wk_side * sizeof(double2);
+        if (h->wk_bytes < wb) {
+            hipFree(h->d_wk);
+            h->d_wk = nullptr;
+            h->wk_bytes = 0;
+            TH_HIP(hipMalloc(&h->d_wk, wb));
+            h->wk_bytes = wb;
+        }
+        if ((rc = th_launch_zero(h, reinterpret_cast<double *>(h->d_wk), 4 * wk_side))) return rc;
+    }
     // 3. pair batches of each axis
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
         const long npairs = axis == 0 ? h->n_im : n_re;
@@ -1385,7 +1513,8 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
         const int *dsv = d_seg + (axis == 0 ? 0 : seg1);
         if ((rc = th_panel_adjoint_prepare(h, precision, That))) return rc;
         size_t sb0 = 0;   // first segment of the batch
-        for (long p0 = 0; p0 < npairs; p0 += bp) {
+        int bi = 0;       // the batch's index
+        for (long p0 = 0; p0 < npairs; p0 += bp, ++bi) {
             const long np = std::min(bp, npairs - p0);
             size_t sb1 = sb0;
             while (sv[sb1] < p0 + np) ++sb1;   // segments never straddle a batch
@@ -1458,7 +1587,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 // Hhat = G^H Ghat G^H per pair, summed over each segment
                 const double2 *Hseg = gh;
                 if (spec_h) {
-                    // spectral form: V and lambda of the segment's k-point, no G
+                    // spectral form: V and lambda of the segment's k-point, no G; the
+                    // segment sums go to the k-points' eigenbasis sums (reduced after
+                    // the last batch)
                     const th_eig_set *es = th_side_eig(h, side == 0, q);
                     hspec_params sp{};
                     sp.V = es->V; sp.L = es->L;
@@ -1474,17 +1605,25 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                     if (d == 18) ppw = std::min(ppw, 4);   // segments of 16 pairs: four pairs per thread slot
                     const int nt = ((ppw * nblk3 + 63) / 64) * 64;
                     const size_t lds = hspec_lds(d, ppw, f32);
-                    auto launch = [&](auto kern) {
-                        TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                        hipLaunchKernelGGL(kern, dim3((unsigned)nseg), dim3(nt), lds, h->stream, ap, sp, dseg, ppw, gh);
-                        return 0;
-                    };
-                    if (d == 18) rc = f32 ? launch(hhat_spec_kernel<float, 18>) : launch(hhat_spec_kernel<double, 18>);
-                    else if (d == 36) rc = f32 ? launch(hhat_spec_kernel<float, 36>) : launch(hhat_spec_kernel<double, 36>);
-                    else rc = f32 ? launch(hhat_spec_kernel<float, 54>) : launch(hhat_spec_kernel<double, 54>);
-                    if (rc) return rc;
-                    TH_HIP(hipGetLastError());
+                    double2 *Wk = h->d_wk + (size_t)side * wk_side;
+                    const std::vector<long> &off = roff[axis][bi];
+                    const int *dso = d_sord + (axis == 0 ? 0 : seg1);
+                    for (size_t r = 0; r + 1 < off.size(); ++r) {
+                        const unsigned nw = (unsigned)(off[r + 1] - off[r]);
+                        auto launch = [&](auto kern) {
+                            TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                            hipLaunchKernelGGL(kern, dim3(nw), dim3(nt), lds, h->stream, ap, sp, dseg, dso + off[r], ppw,
+                                               Wk);
+                            return 0;
+                        };
+                        if (d == 18) rc = f32 ? launch(hhat_spec_kernel<float, 18>) : launch(hhat_spec_kernel<double, 18>);
+                        else if (d == 36) rc = f32 ? launch(hhat_spec_kernel<float, 36>) : launch(hhat_spec_kernel<double, 36>);
+                        else rc = f32 ? launch(hhat_spec_kernel<float, 54>) : launch(hhat_spec_kernel<double, 54>);
+                        if (rc) return rc;
+                        TH_HIP(hipGetLastError());
+                    }
+                    continue;
                 } else if (d <= 54 && d % 3 == 0) {
                     // the pairs' G, Ghat in LDS, 3 x 3 register blocks, one workgroup per segment
                     const int nblk3 = (d / 3) * (d / 3);
@@ -1526,32 +1665,39 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                                    n_x, q[0], q[1], q[2], (const double *)h->d_kim, (const double *)h->d_kre,
                                    (const double *)h->d_R, d_ph);
                 TH_HIP(hipGetLastError());
-                hreduce_params hp{};
-                hp.side_q = side == 0 ? 1 : 0;
-                hp.imag = axis == 0 ? 1 : 0;
-                hp.p0 = p0; hp.np = nseg; hp.nk_re = h->nk_re;
-                hp.chunk = CHUNK; hp.nout = nout; hp.nhop = h->nhop; hp.nOrb = nO; hp.d = d;
-                hp.q0 = q[0]; hp.q1 = q[1]; hp.q2 = q[2];
-                hp.kim = h->d_kim; hp.kre = h->d_kre; hp.R = h->d_R; hp.jb = h->d_jb; hp.ib = h->d_ib;
-                hp.Hh = Hseg;
-                hp.nA = h->nA;
-                hp.S = d_S;
-                hp.ph = d_ph;
-                hp.partial = d_part;
-                hipLaunchKernelGGL(spin_sum_kernel, dim3(nblk((size_t)nseg * h->nA2 * nO * nO, 256)), dim3(256), 0,
-                                   h->stream, (long)nseg, h->nA, nO, Hseg, d_S);
-                const int nch = (nseg + CHUNK - 1) / CHUNK;
-                if (ngroups > 0)
-                    hipLaunchKernelGGL(hop_reduce_kernel, dim3(nblk((size_t)ngroups * nO * nO, 64), nch), dim3(256), 0,
-                                       h->stream, hp, (const int *)d_groups, ngroups);
-                hipLaunchKernelGGL(h0_reduce_kernel, dim3(nblk(dd, 64), nch), dim3(256), 0, h->stream, hp);
-                hipLaunchKernelGGL(hsum_kernel, dim3(nblk(nout, 4)), dim3(256), 0, h->stream, (const double2 *)d_part,
-                                   nch, nout, d_grad);
+                if ((rc = hop_h0_reduce(Hseg, nseg))) return rc;
                 TH_HIP(hipGetLastError());
             }
             tm.end(id);
             sb0 = sb1;
         }
+    }
+    if (spec_h) {
+        // Hhat_k = V_k W_k V_k^H per side and unique k-point, then the phase,
+        // spin-sum and hop/h0 reductions over the k-points, bp at a time
+        id = tm.begin(2);
+        for (int side = 0; side < 2; ++side) {
+            const th_eig_set *es = th_side_eig(h, side == 0, q);
+            const double2 *Wk = h->d_wk + (size_t)side * wk_side;
+            for (long k0 = 0; k0 < h->n_kuniq; k0 += bp) {
+                const int nkc = (int)std::min(bp, h->n_kuniq - k0);
+                const size_t lds = 2 * dd * sizeof(double2);
+                auto fin = [&](auto kern) {
+                    TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    hipLaunchKernelGGL(kern, dim3((unsigned)nkc), dim3(256), lds, h->stream, Wk, es->V, k0, gA);
+                    return 0;
+                };
+                rc = d == 18 ? fin(hhat_final_kernel<18>) : d == 36 ? fin(hhat_final_kernel<36>) : fin(hhat_final_kernel<54>);
+                if (rc) return rc;
+                hipLaunchKernelGGL(phase_k_kernel, dim3(nblk((size_t)nkc * std::max(1, h->nhop), 256)), dim3(256), 0,
+                                   h->stream, (const double *)h->d_kuniq, k0, nkc, h->nhop, side == 0 ? 1 : 0, q[0], q[1],
+                                   q[2], (const double *)h->d_R, d_ph);
+                TH_HIP(hipGetLastError());
+                if ((rc = hop_h0_reduce(gA, nkc))) return rc;
+            }
+        }
+        tm.end(id);
     }
     tm.end(idall);
     // 4. results to the host, hops back in the caller's order

@@ -123,10 +123,17 @@ struct th_handle {
     int eig_valid[2] = {0, 0};
     std::vector<int> kid_im_local;        // imaginary point -> index in kim_uniq
     long list_version = 0;                // bumped whenever the imaginary list or the real mesh is set
+    std::vector<int> kid_im_host, kid_re_host;   // d_kid_im, d_kid_re on the host
+    double2 *d_wk = nullptr;              // adjoint: per unique k-point and side, the eigenbasis Hhat sums
+    size_t wk_bytes = 0;
     long last_blkmax = 0;                 // workgroup maxima the last th_launch_green wrote (0: none)
     // the adjoint's segments (runs of pairs on one k-point within a batch) per
     // axis, for the key {list_version, pairs per batch, real-axis nodes}
     std::vector<int> adj_seg[2];
+    // per batch its segments (index in the batch) in rounds of distinct
+    // k-points, and the rounds' offsets into adj_sord
+    std::vector<int> adj_sord[2];
+    std::vector<std::vector<long>> adj_roff[2];
     long adj_seg_key[3] = {-1, -1, -1};
     std::vector<double> kim_uniq, kre_host;
     std::vector<double> yim_host, wim_host;   // imaginary list in stored order
