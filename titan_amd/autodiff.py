@@ -211,6 +211,12 @@ class ChiOperator:
         self._h, self._key = h, key
         return h
 
+    def close(self):
+        """Frees the kept handle's device memory (a later call builds a new one)."""
+        if self._h is not None:
+            self._h.close()
+        self._h, self._key = None, None
+
     def __call__(self, p, omega):
         t, h0, U = self.tm.tables(p)
         return ChiFunction.apply(t, h0, U, self, float(omega))
