@@ -97,7 +97,11 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
     # kernel shapes tuned on every 8th k-point, as for the main workload
     ks, ws = wl.kp[::8], wl.wk[::8] * 8
     h.set_imag_points(np.tile(ks, (len(wl.y), 1)), np.repeat(wl.y, len(ws)), np.outer(wl.wy, ws).ravel())
-    best = AT.tune(h, run["q"], omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=B.PREC_MIXED, repeat=1)["best"]
+    # per precision: the FP64 contractions take the three-product form or not,
+    # which the mixed path's fp16 kernels never read
+    best = {name: AT.tune(h, run["q"], omega, m.Ef, run["eta"], run["etap"], x2, p2, precision=prec,
+                          repeat=1)["best"]
+            for name, prec in (("fp64", B.PREC_FP64), ("mixed", B.PREC_MIXED))}
     h.set_imag_points(np.tile(wl.kp, (len(wl.y), 1)), np.repeat(wl.y, wl.nk), np.outer(wl.wy, wl.wk).ravel())
     rng = np.random.default_rng(1)
     W = rng.standard_normal((m.D, m.D)) + 1j * rng.standard_normal((m.D, m.D))
@@ -105,6 +109,7 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
                        (cfg, spec["name"], "x".join(map(str, run["nk"])), len(wl.y), wl.n_x),
            "pairs": wl.pairs([omega]), "tuning": best}
     for name, prec in (("fp64", B.PREC_FP64), ("mixed", B.PREC_MIXED)):
+        h.tune(**best[name])
         tf, tb, stages = [], [], None
         for r in range(reps + 1):
             t0 = time.perf_counter()
@@ -126,7 +131,7 @@ def adjoint_cost(cfg=3, omega=4e-3, reps=2):
     return out
 
 
-def loss_cost(cfg=3):
+def loss_cost(cfg=3, tuning=None):
     """The adjoint workload as specified: the scalar loss sum over the
     configuration's 32 frequencies of (Im chi^{+-}(omega) - seeded target)^2
     (the target of the model's own magnitude)
@@ -134,7 +139,9 @@ def loss_cost(cfg=3):
     (titan_amd.autodiff.chi_pm_loss): forward (32 chi on one handle) and
     backward (32 th_chi_adjoint calls) wall time in both precisions, after an
     untimed forward at the first frequency (the eigenpairs and the
-    omega-independent panels are then in place for both)."""
+    omega-independent panels are then in place for both); tuning: th_tune
+    settings of the handles per precision, {"fp64": {...}, "mixed": {...}}
+    (the adjoint workload's autotuned shapes)."""
     import torch
     from titan_amd import binding as B
     from titan_amd import configs as CF
@@ -148,7 +155,8 @@ def loss_cost(cfg=3):
                         run["n3gl"] * run["parts3"], len(oms))}
     for name, prec in (("fp64", B.PREC_FP64), ("mixed", B.PREC_MIXED)):
         op = ChiOperator(tm, tuple(run["nk"]), n1gl=run["n1gl"], parts=run["parts"], n3gl=run["n3gl"],
-                         parts3=run["parts3"], eta=run["eta"], etap=run["etap"], q=run["q"], precision=prec)
+                         parts3=run["parts3"], eta=run["eta"], etap=run["etap"], q=run["q"], precision=prec,
+                         tuning=(tuning or {}).get(name))
         if tgt is None:   # the target scaled to the model's response (untimed)
             tgt = target_curve(oms, np.max(np.abs(nominal_chi_pm(op, tm, oms))))
         else:
@@ -517,7 +525,8 @@ def main():
             log("adjoint cost failed:", e)
             out["adjoint_cost"] = None
         try:
-            out["adjoint_loss"] = loss_cost()
+            # the loss's handles take the shapes tuned for the adjoint workload
+            out["adjoint_loss"] = loss_cost(tuning=(out.get("adjoint_cost") or {}).get("tuning"))
             log("adjoint loss", out["adjoint_loss"])
         except Exception as e:   # reported, never required
             log("adjoint loss failed:", e)

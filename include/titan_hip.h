@@ -147,12 +147,13 @@ int th_chi_sweep(th_handle *h, const double *q, int n_omega, const double *omega
  * in the same convention.  chi0 and chi are the forward results at this
  * omega (from th_chi0 and th_dyson with the same arguments).  With U == NULL
  * the Dyson step is skipped: chi_hat is then the cotangent of chi0 and chi is
- * not read.  The Green functions are recomputed, not stored for the
- * backward (the omega-independent imaginary-axis ones are read from the
- * forward's panel cache when it holds them); 'precision' selects the panel
- * precision of the two adjoint contractions as in th_chi0 (TH_PREC_MIXED
- * with "contract_f16": fp16 matrix cores on two-term splits; the products
- * G^H Ghat G^H in FP32) while every sum is FP64. */
+ * not read.  The Green functions are recomputed in the backward, not stored
+ * for it (th_tune "adjoint_bcache" = 1 reads the omega-independent
+ * imaginary-axis ones from the forward's panel cache instead); 'precision'
+ * selects the panel precision of the two adjoint contractions as in th_chi0
+ * (TH_PREC_MIXED with "contract_f16": fp16 matrix cores on two-term splits;
+ * the products G^H Ghat G^H, in the eigenbasis of H(k) when the eigenpairs
+ * are kept, in FP32) while every sum is FP64. */
 int th_chi_adjoint(th_handle *h, const double *q, double omega, double Ef, double eta, double etap,
                    int n_x, const double *x2, const double *p2, int precision,
                    const double *chi0, const double *U, const double *chi, const double *chi_hat,
@@ -327,6 +328,10 @@ int th_last_timing(th_handle *h, double *t, long *n);
  *                    per launch); a budget of at least one full wave of the
  *                    fp16 spectral kernel's workgroups (d = 18, 36, 54) is
  *                    rounded to the nearest whole number of waves
+ *   "adjoint_bcache" 0 = th_chi_adjoint recomputes every Green function
+ *                    (default); 1 = it reads the omega-independent
+ *                    imaginary-axis B panels from the forward's panel cache
+ *                    when that holds them (same E_F, eta', precision)
  * th_get_tuning reads the current values. */
 int th_tune(th_handle *h, const char *key, long value);
 int th_get_tuning(th_handle *h, long *green_variant, long *splits, long *batch_mb);

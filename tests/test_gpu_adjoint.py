@@ -12,9 +12,10 @@ of titan_amd.autodiff.
   * bitwise reproducibility of the adjoint; the three-product contraction;
     the Green functions by inversion (Hhat from each pair's G instead of the
     spectral form).
-  The FP64 calls follow an FP64 forward, so they read the imaginary-axis B
-  side from the forward's panel cache; the mixed calls find an FP64 cache
-  and recompute it.
+  Every Green function is recomputed in the backward by default; a leg of
+  each precision reads the imaginary-axis B side from the forward's panel
+  cache instead (th_tune "adjoint_bcache"), after a forward of that
+  precision.
 """
 import copy
 
@@ -84,6 +85,16 @@ def test_adjoint_matches_fp64_reference(name):
     assert _rel(hh, rh) <= 1e-10
     if useU:
         assert _rel(uh, ru) <= 1e-10
+    # the imaginary-axis B side from the FP64 forward's panel cache
+    h.tune(adjoint_bcache=1)
+    thc, hhc, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"], p2=c["p2"])
+    assert _rel(thc, rt) <= 1e-10 and _rel(hhc, rh) <= 1e-10
+    # ... and from a mixed forward's (split half-precision words)
+    h.chi0(c["q"], omega, m.Ef, c["eta"], c["eta"], c["x2"], c["p2"], precision=B.PREC_MIXED)
+    thm, hhm, _ = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"],
+                            p2=c["p2"], precision=B.PREC_MIXED)
+    assert _rel(thm, rt) <= 1e-5 and _rel(hhm, rh) <= 1e-5
+    h.tune(adjoint_bcache=0)
     # mixed precision: FP32 panels in the two adjoint contractions
     th2, hh2, uh2 = h.adjoint(c["q"], omega, m.Ef, c["eta"], c["eta"], chi0, W, U=U, chi=chi, x2=c["x2"],
                               p2=c["p2"], precision=B.PREC_MIXED)

@@ -164,8 +164,9 @@ class ChiOperator:
     differentiable chi(q, omega) of parameters p (TorchModel.params())."""
 
     def __init__(self, tm, kmesh, n1gl=64, parts=1, n3gl=32, parts3=1, eta=5e-4, etap=None, q=(0.0, 0.0, 0.0),
-                 precision=B.PREC_FP64, adjoint_precision=None):
+                 precision=B.PREC_FP64, adjoint_precision=None, tuning=None):
         self.tm = tm
+        self.tuning = dict(tuning or {})   # th_tune settings of every handle (e.g. an autotuner's best)
         m = tm.m
         self.Ef = m.Ef
         self.eta = float(eta)
@@ -206,6 +207,8 @@ class ChiOperator:
         if self._h is not None and self._key == key:
             return self._h
         h = B.Handle(m.nA, m.nOrb, m.R, m.jb, m.ib, t, h0)
+        if self.tuning:
+            h.tune(**self.tuning)
         h.set_imag_points(self.kim, self.yim, self.wim)
         h.set_real_kmesh(self.kp, self.wk)
         self._h, self._key = h, key

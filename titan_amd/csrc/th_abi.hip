@@ -1094,6 +1094,9 @@ int th_tune(th_handle *h, const char *key, long value)
     } else if (k == "batch_mb") {
         if (value < 16 || value > 65536) { th_set_error("th_tune: batch_mb in 16..65536"); return TH_E_ARG; }
         h->batch_mb = (int)value;
+    } else if (k == "adjoint_bcache") {
+        if (value < 0 || value > 1) { th_set_error("th_tune: adjoint_bcache is 0 or 1"); return TH_E_ARG; }
+        h->adj_bcache = (int)value;
     } else {
         th_set_error("th_tune: unknown key " + k);
         return TH_E_ARG;

@@ -1294,7 +1294,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     need += ((size_t)bp * h->nA2 * nO * nO * sizeof(double2) + 255) & ~(size_t)255;
     need += ((size_t)3 * (h->nhop + h->nA2) * sizeof(int) + 255) & ~(size_t)255;   // hop groups
     need += 2 * (((size_t)(h->n_im + n_re + 2) * sizeof(int) + 255) & ~(size_t)255);   // segments, their order
-    need += (((size_t)bp / TH_RUN + 1) * sizeof(float) + 255) & ~(size_t)255;     // A panel maxima per workgroup
+    need += 2 * ((((size_t)bp / TH_RUN + 1) * sizeof(float) + 255) & ~(size_t)255);   // A, B panel maxima per workgroup
     need += 4096;
     if (need > h->adj_bytes) {
         hipFree(h->d_adj);
@@ -1323,6 +1323,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     int *d_seg = ar.take<int>((size_t)(h->n_im + n_re + 2));
     int *d_sord = ar.take<int>((size_t)(h->n_im + n_re + 2));
     float *d_bmax = ar.take<float>((size_t)bp / TH_RUN + 1);
+    float *d_bmaxB = ar.take<float>((size_t)bp / TH_RUN + 1);
 
     // segments: runs of consecutive pairs on one k-point within a batch (the
     // stored imaginary list keeps a k-point's pairs together in runs; real
@@ -1455,7 +1456,9 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     if ((rc = zero2(h, d_grad, nout))) return rc;
     tm.end(id);
     const size_t accsz = (size_t)h->nA2 * 2 * h->Mpad * h->Mpad;
-    const bool use_bcache = h->d_bcache && h->bcache_prec == precision && h->bcache_key[0] == Ef &&
+    // the B panels are recomputed like the A panels unless "adjoint_bcache" asks
+    // for the forward's omega-independent cache (same E_F, eta', precision)
+    const bool use_bcache = h->adj_bcache && h->d_bcache && h->bcache_prec == precision && h->bcache_key[0] == Ef &&
                             h->bcache_key[1] == etap;
     // split words in the cache are read back as (hi + lo) 2^-e
     const int split_e = (use_bcache && h->bcache_presplit) ? h->bcache_pexp : INT_MIN;
@@ -1538,6 +1541,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             // the imaginary-axis B side does not depend on omega: when the forward's
             // panel cache holds it (same E_F, eta', precision), read G from there
             const void *pB = h->d_pB;
+            long nbmaxB = 0;   // workgroup maxima of a recomputed B panel
             th_rowmap brows;   // rows of pB: this batch's pairs in order
             if (axis == 0 && use_bcache) {
                 // the cache holds the whole list, per batch of the forward's
@@ -1561,12 +1565,15 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
                 a.mode = axis == 0 ? TH_G_IMAG_B : TH_G_REAL_B;
                 a.panel = h->d_pB;
                 a.raw = spec_h ? nullptr : rawB;
+                a.blkmax = precision != TH_PREC_FP64 ? d_bmaxB : nullptr;   // the B panel's scale
                 if ((rc = th_launch_green(h, a))) return rc;
+                a.blkmax = nullptr;
+                nbmaxB = h->last_blkmax;
             }
             tm.end(id);
             id = tm.begin(1);
             if ((rc = th_launch_panel_adjoint(h, pB, precision, That, np, 1, hatA, brows,
-                                              axis == 0 && use_bcache ? split_e : INT_MIN)))
+                                              axis == 0 && use_bcache ? split_e : INT_MIN, d_bmaxB, nbmaxB)))
                 return rc;
             if ((rc = th_launch_panel_adjoint(h, h->d_pA, precision, That, np, 0, hatB, th_rowmap(), INT_MIN, d_bmax,
                                               nbmax)))

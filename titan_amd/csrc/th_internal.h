@@ -127,6 +127,7 @@ struct th_handle {
     double2 *d_wk = nullptr;              // adjoint: per unique k-point and side, the eigenbasis Hhat sums
     size_t wk_bytes = 0;
     long last_blkmax = 0;                 // workgroup maxima the last th_launch_green wrote (0: none)
+    int adj_bcache = 0;                   // 1: the adjoint reads the imaginary-axis B panels from the forward's cache
     // the adjoint's segments (runs of pairs on one k-point within a batch) per
     // axis, for the key {list_version, pairs per batch, real-axis nodes}
     std::vector<int> adj_seg[2];
