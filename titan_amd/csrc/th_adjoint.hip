@@ -1490,7 +1490,7 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
     };
     // Hhat in the spectral form when both sides' eigenpairs are ready (then
     // neither side's G is formed: the Green stage writes panels only)
-    const bool spec_h = (d == 18 || d == 36 || d == 54) && th_side_eig(h, true, q) && th_side_eig(h, false, q);
+    bool spec_h = (d == 18 || d == 36 || d == 54) && th_side_eig(h, true, q) && th_side_eig(h, false, q);
     // spectral form: per side and unique k-point the eigenbasis sum W_k (FP64),
     // and per batch its segments in rounds of distinct k-points (launch after
     // launch, so that the sums add in the segments' order without races)
@@ -1503,10 +1503,15 @@ extern "C" int th_chi_adjoint(th_handle *h, const double *q, double omega, doubl
             hipFree(h->d_wk);
             h->d_wk = nullptr;
             h->wk_bytes = 0;
-            TH_HIP(hipMalloc(&h->d_wk, wb));
-            h->wk_bytes = wb;
+            if (hipMalloc(&h->d_wk, wb) == hipSuccess) {
+                h->wk_bytes = wb;
+            } else {   // no room for the k-points' sums: each pair's G instead
+                (void)hipGetLastError();
+                h->d_wk = nullptr;
+                spec_h = false;
+            }
         }
-        if ((rc = th_launch_zero(h, reinterpret_cast<double *>(h->d_wk), 4 * wk_side))) return rc;
+        if (spec_h && (rc = th_launch_zero(h, reinterpret_cast<double *>(h->d_wk), 4 * wk_side))) return rc;
     }
     // 3. pair batches of each axis
     for (int axis = 0; axis < (real_axis ? 2 : 1); ++axis) {
