@@ -55,6 +55,9 @@ CASES = {
     "bcc1_imag_U": (CF.bcc_1site, 3, 6, 6, 0.0, (0.0, 0.0, 0.0), True),
     # 20 real nodes: the real pairs' grouped order (blocks of 16 nodes, then 4)
     "bcc1_real_nx20_U": (CF.bcc_1site, 2, 4, 20, 3e-3, (0.05, -0.02, 0.1), True),
+    # 40 energies on 2^3 k-points: three runs per k-point (16, 16, 8) in one batch,
+    # so the per-k-point sums of the spectral form take three launch rounds
+    "bcc1_imag_n40_U": (CF.bcc_1site, 2, 40, 4, 0.0, (0.0, 0.0, 0.0), True),
     "bcc2_real_chi0": (lambda: CF.bcc_supercell(2), 2, 4, 4, 2e-3, (0.0, 0.0, 0.07), False),
     "bcc2_imag_U": (lambda: CF.bcc_supercell(2), 2, 4, 4, 0.0, (0.0, 0.0, 0.0), True),
     "pchain_real_U0": (lambda: CF.chain_p(), 5, 6, 6, 2e-3, (0.1, 0.0, 0.0), True),
