@@ -1014,7 +1014,7 @@ __global__ __launch_bounds__(256) void presplit_y_kernel(const double *__restric
 // of 8 consecutive k into one 16-byte LDS store per part; loads use clamped
 // indices (no branch per load) and zero what lies outside.
 constexpr int PAK = 32, PAST = PAK + 8;   // K per stage, LDS row stride in halves
-__global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__restrict__ X,
+__global__ __launch_bounds__(448) __attribute__((amdgpu_waves_per_eu(4))) void panel_adjoint_h2_kernel(const float *__restrict__ X,
                                                                const _Float16 *__restrict__ Ys, int Kp, long np, int Mpad,
                                                                int nA2, int ntile, int trans, double *__restrict__ out,
                                                                th_rowmap rows, float unsplit, int ex_fixed,
@@ -1050,10 +1050,11 @@ __global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__re
             lo[i] = (_Float16)fmaf(v[i], 1.0f, -(float)h);
         }
     };
-    fv4 ap[7], an[7], ai[7];
+    // real part: Ar Br and -Ai Bi in one accumulator, imaginary part Ar Bi + Ai Br
+    fv4 ap[7], ai[7];
 #pragma unroll
     for (int j = 0; j < 7; ++j)
-        for (int q = 0; q < 4; ++q) { ap[j][q] = 0.0f; an[j][q] = 0.0f; ai[j][q] = 0.0f; }
+        for (int q = 0; q < 4; ++q) { ap[j][q] = 0.0f; ai[j][q] = 0.0f; }
     long xrow[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
@@ -1080,8 +1081,8 @@ __global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__re
             yq[u][1] = *reinterpret_cast<const hv8_t *>(Yb + yo + nk);
         }
     };
-    gload(0);
     for (int k0 = 0; k0 < Mpad; k0 += PAK) {
+        gload(k0);
         __syncthreads();                          // the previous stage's reads done
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
@@ -1107,14 +1108,13 @@ __global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__re
             *reinterpret_cast<hv8_t *>(&Ly[comp][0][m * PAST + kg]) = yq[u][0];
             *reinterpret_cast<hv8_t *>(&Ly[comp][1][m * PAST + kg]) = yq[u][1];
         }
-        // the next stage's loads (the last stage re-reads in-range data)
-        gload(k0 + PAK < Mpad ? k0 + PAK : k0);
         __syncthreads();
         const int ao = (16 * wave + (lane & 15)) * PAST + 8 * (lane >> 4);
         const hv8_t arh = *reinterpret_cast<const hv8_t *>(&Lx[0][0][ao]);
         const hv8_t arl = *reinterpret_cast<const hv8_t *>(&Lx[0][1][ao]);
         const hv8_t aih = *reinterpret_cast<const hv8_t *>(&Lx[1][0][ao]);
         const hv8_t ail = *reinterpret_cast<const hv8_t *>(&Lx[1][1][ao]);
+        const hv8_t nih = -aih, nil = -ail;
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
             const int bo = (16 * j + (lane & 15)) * PAST + 8 * (lane >> 4);
@@ -1123,13 +1123,13 @@ __global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__re
             const hv8_t bih = *reinterpret_cast<const hv8_t *>(&Ly[1][0][bo]);
             const hv8_t bil = *reinterpret_cast<const hv8_t *>(&Ly[1][1][bo]);
             ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh, ap[j], 0, 0, 0);
-            an[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih, an[j], 0, 0, 0);
+            ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nih, bih, ap[j], 0, 0, 0);
             ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih, ai[j], 0, 0, 0);
             ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl, ap[j], 0, 0, 0);
-            an[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil, an[j], 0, 0, 0);
+            ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nih, bil, ap[j], 0, 0, 0);
             ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bil, ai[j], 0, 0, 0);
             ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh, ap[j], 0, 0, 0);
-            an[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih, an[j], 0, 0, 0);
+            ap[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(nil, bih, ap[j], 0, 0, 0);
             ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bih, ai[j], 0, 0, 0);
             ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh, ai[j], 0, 0, 0);
             ai[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl, ai[j], 0, 0, 0);
@@ -1145,7 +1145,7 @@ __global__ __launch_bounds__(448) void panel_adjoint_h2_kernel(const float *__re
             const int col = n0 + j * 16 + (lane & 15);
             if (pp < np) {
                 double *o = out + (size_t)pp * pstride + (size_t)ij * 2 * Mpad;
-                o[col] = ((double)ap[j][q] - (double)an[j][q]) * back;
+                o[col] = (double)ap[j][q] * back;
                 o[Mpad + col] = (double)ai[j][q] * back;
             }
         }
