@@ -1115,7 +1115,8 @@ __global__ __launch_bounds__(256) void green_spectral_kernel(green_params g)
 // f_m = 1/(z - lambda_m).  Results go from the accumulators straight into
 // the panel (no staging of G), padding entries are written as zeros.
 template <typename P, int DD, int CH>
-__global__ __launch_bounds__(64 * ((DD + 15) / 16)) void green_spectral_chunk_kernel(green_params g)
+__global__ __launch_bounds__(64 * ((DD + 15) / 16)) __attribute__((amdgpu_waves_per_eu(DD == 18 ? 4 : 1))) void
+green_spectral_chunk_kernel(green_params g)
 {
     typedef P R;
     constexpr int NB = (DD + 15) / 16, NT = 64 * NB, KS = (DD + 3) / 4, KP = 4 * KS, NC = 16 * NB;
